@@ -1,0 +1,234 @@
+/*
+ * tritok_bem.h -- C-ABI of the MI355X (gfx950) boundary-element library
+ * libtritok_amd.so.
+ *
+ * Drop-in boundary for the hot path of the Tritok velocity-vorticity BEM
+ * solver.  A Fortran host binds every function below through ISO_C_BINDING
+ * (fortran/tritok_amd_bind.f90); arrays are passed exactly as the reference
+ * holds them: column-major, index arrays 1-based, INTEGER = int32,
+ * REAL(8) = double.  No function takes or returns a framework type.
+ *
+ * Status codes: 0 = success, negative = error; tt_last_error() returns the
+ * message of the last error on the calling thread.
+ *
+ * Each function names the reference routine it replaces (file:line in the
+ * reference's src/).
+ */
+#ifndef TRITOK_BEM_H
+#define TRITOK_BEM_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Plain views of the arrays the reference keeps in TYPE(meshType)
+ * (inc_types.f90:77-212).  The library never frees them. */
+typedef struct tt_mesh_view {
+  int32_t nnodes, nbnodes, nbelem, nicell, nq;
+  const double *x;    /* x(nnodes,3)                                   */
+  const int32_t *ibc; /* ibc(nbelem,9)  boundary elements              */
+  const int32_t *idc; /* idc(nicell,27) cells                          */
+  const int32_t *ibf; /* ibf(nicell,6,4) flux nodes of cell faces      */
+  const double *xq;   /* xq(nq,3)       flux node coordinates          */
+  const int32_t *flpm;/* flpm(nicell,6) flux sign of a cell face       */
+  const int32_t *gbn; /* gbn(nbnodes)   boundary node -> node          */
+  const int32_t *lbn; /* lbn(nnodes)    node -> boundary node, 0 inside*/
+  const double *nx, *ny, *nz; /* source point normals (nbnodes)        */
+} tt_mesh_view;
+
+/* TYPE(GaussType) (inc_types.f90:367-372) as GaussPosWeights fills it
+ * (lib_int.f:2673-3173) plus GAUS kmBr kmBs kmDr kmDs and IIDI of tri.inp
+ * (lib_IO.f:322-325): rule indices 1..9 (N = 2,4,6,8,16,32,40,48,64), the
+ * number of sub-cells per direction for singular cells, and the table
+ * gi(220), ome(220) with 1-based ranges ng1(9)..ng2(9) per rule.  A NULL
+ * table selects Gauss-Legendre values computed by the library; a Fortran host
+ * passes gp%gi, gp%ome, gp%ng1, gp%ng2 to reproduce its own table. */
+typedef struct tt_quad {
+  int32_t kmBr, kmBs, kmDr, kmDs, iiDiv;
+  const double *gi, *ome;
+  const int32_t *ng1, *ng2;
+} tt_quad;
+
+const char *tt_last_error(void);
+/* library version and the device it runs on (name, CU count, HBM bytes) */
+int tt_device_info(char *name, int32_t name_len, int32_t *num_cu, int64_t *hbm_bytes);
+
+/* ------------------------------------------------------------------------ */
+/* Integral assembly.                                                        */
+
+/* Replaces FMATkmdcSteady (lib_fmat.f:3-446) as called by corMin_kmdcSteady
+ * (lib_int.f:1254-1340): all per-cell macro matrices for every cell, with the
+ * free-term, Ht row-sum and linear-field corrections.  Outputs are host arrays,
+ * column-major: H,B,Abdx,Abdy,Abdz,Dx,Dy,Dz (nicell*51, 27), G (nicell*51, 24),
+ * Htx,Hty,Htz (nicell*27, 27); ont[11] as written to tri.macro.int.
+ * copy = COPY keyword (replicate cell 1). */
+int tt_macro_integrals(const tt_mesh_view *m, const tt_quad *q, int32_t copy, double *H, double *G,
+                       double *B, double *Abdx, double *Abdy, double *Abdz, double *Htx,
+                       double *Hty, double *Htz, double *Dx, double *Dy, double *Dz, double ont[11]);
+
+/* Replaces corMin_modHelm / FMATmodHelm (lib_mHelm.f:1400-1502, :2-483): the
+ * same matrices for the modified-Helmholtz kernel exp(-mu r)/(4 pi r), with mu
+ * given per source point: mu_nodes(nnodes) for node sources and mu_q(nq) for
+ * flux sources (mu = sqrt(beta/diff) at the source, lib_mHelm.f:139,197). */
+int tt_macro_integrals_mh(const tt_mesh_view *m, const tt_quad *q, const double *mu_nodes,
+                          const double *mu_q, double *H, double *G, double *B, double *Abdx,
+                          double *Abdy, double *Abdz, double *Htx, double *Hty, double *Htz,
+                          double *Dx, double *Dy, double *Dz, double ont[11]);
+
+/* Replaces FMATsdkm (lib_sdkm.f:1286-1530) as called by sdKm_corInt
+ * (lib_sdkm.f:2377-2449) for boundary rows row0..row1-1 (0-based; the
+ * reference's zac:kon).  Host outputs with leading dimension row1-row0:
+ * H,Htx,Hty,Htz (rows, nbnodes), Dx,Dy,Dz (rows, nnodes); ont[10] as written
+ * to tri.sdkm.int.N. */
+int tt_sd_integrals(const tt_mesh_view *m, const tt_quad *q, int32_t row0, int32_t row1, double *H,
+                    double *Htx, double *Hty, double *Htz, double *Dx, double *Dy, double *Dz,
+                    double ont[10]);
+
+/* ------------------------------------------------------------------------ */
+/* Sparse least squares (the reference's CSR, 1-based).                      */
+
+typedef struct tt_lsqr tt_lsqr; /* device-resident scaled system */
+
+/* Replaces prelsqr2 (lib_Solve_v11.f:4501-4535): uploads A (m x n, CSR
+ * ia(m+1), ja(nnz), a(nnz)), computes q_j = 1/||A(:,j)|| and scales the device
+ * copy in place.  q (n) is returned to the host when non-NULL, and a is scaled
+ * in place on the host too when scale_host != 0 (the reference's semantics). */
+int tt_lsqr_setup(int32_t m, int32_t n, int32_t nnz, const int32_t *ia, const int32_t *ja,
+                  double *a, double *q, int32_t scale_host, tt_lsqr **out);
+/* Replaces slvlsqr2 + LSQR (lib_Solve_v11.f:4401-4500, :3128-3887): hot start
+ * from x, atol = eps4, btol = eps4*||b||/||b - A x0||, conlim = 1e8, damp = 0.
+ * Returns istop (0..5) in *istop and the iteration count in *nit. */
+int tt_lsqr_solve(tt_lsqr *s, const double *b, double *x, int32_t maxit, float eps4, int32_t *nit,
+                  int32_t *istop);
+void tt_lsqr_free(tt_lsqr *s);
+
+/* Replaces CRSxV (lib_util.f:1351-1372): y = A x on the device for a CSR
+ * matrix given by host arrays (m rows, n columns). */
+int tt_csr_mv(int32_t m, int32_t n, int32_t nnz, const int32_t *ia, const int32_t *ja,
+              const double *a, const double *x, double *y);
+
+/* ------------------------------------------------------------------------ */
+/* Dense single-domain kinematics.                                          */
+
+/* Replaces dgefa (lib_Solve_v11.f:787-904) as used by FormPREfm pret=2:
+ * partial-pivot LU of a(n,n) in place (host array), pivots as doubles. */
+int tt_dense_lu(int32_t n, double *a, double *ipvt);
+/* Replaces dgesl job=0 (lib_Solve_v11.f:905-1037): solves in place. */
+int tt_dense_lu_solve(int32_t n, const double *a, const double *ipvt, double *b);
+
+/* ------------------------------------------------------------------------ */
+/* Compressed matvecs.                                                       */
+
+/* ACA (lib_ACA.f): full-pivot cross approximation of a dense host matrix
+ * a(m,n) to relative Frobenius tolerance eps (ADAPTIVE_CROSS_APPROXIMATION2,
+ * lib_ACA.f:401-437) or to a fixed rank kmax when eps <= 0 (aca_CrossAprox
+ * type 1, :126-181).  Returns rank k and factors u(m,k), v(k,n) (host arrays of
+ * capacity kmax). */
+int tt_aca_compress(int32_t m, int32_t n, const double *a, double eps, int32_t kmax, double *u,
+                    double *v, int32_t *k);
+/* aca_abxv (lib_ACA.f:226-249): y = u (v x). */
+int tt_aca_mv(int32_t m, int32_t n, int32_t k, const double *u, const double *v, const double *x,
+              double *y);
+
+/* Stretched-Haar wavelet matvec Che_WTAWx (lib_WT.f:297-329): y = W^T C (W x)
+ * for a compressed CSR C (nrow x ncol, 1-based) of the transformed matrix. */
+int tt_wt_mv(int32_t nrow, int32_t ncol, int32_t nnz, const int32_t *zvr, const int32_t *c,
+             const double *v, const double *x, double *y);
+
+/* ------------------------------------------------------------------------ */
+/* Whole-case driver: the reference's main.f90 setup (:237-1084) and the
+ * steady Picard loop (:1200-1589) with matrices resident in HBM.            */
+
+typedef struct tt_case tt_case;
+
+/* Case inputs as the reference's host holds them after ReadBic
+ * (lib_bic.f:50-502), DetKMunk3 (lib_km.f:676-727) and FillKobc
+ * (lib_mesh.f:230-276). */
+typedef struct tt_case_view {
+  tt_mesh_view mesh;
+  tt_quad quad;
+  int32_t copy;
+  const int32_t *kode;   /* kode(nbnodes,3)                      */
+  const int32_t *kobc;   /* kobc(nbnodes) (may be NULL if no dv/dn=0) */
+  const int32_t *wkode;  /* wkode(nnodes,3), wqkode(nq,3)        */
+  const int32_t *wqkode;
+  const int32_t *Tkode;  /* Tkode(nnodes), Tqkode(nq)            */
+  const int32_t *Tqkode;
+  int32_t nunk[3], nbk[3], Tnunk, Tnb;
+  const int32_t *kmu;    /* kmp%u(nnodes) */
+  int32_t nsmcol, nrhscol;
+  int32_t iDT;
+  double ReN, RaN, PrN, gx, gy, gz, tstep;
+  double urBw[3], urDv[3], urDw[3], urDT;
+  double eps;
+  int32_t nnlit, lsqs_maxit, iDLSE;
+  float lsqs_eps, dlse_r, dlse_epsth;
+} tt_case_view;
+
+/* precision of the dense kinematics products: storage format of the
+ * sdkm matrices on the device */
+enum { TT_PREC_FP64 = 0, TT_PREC_FP32 = 1, TT_PREC_BF16 = 2, TT_PREC_FP16 = 3 };
+
+/* Builds everything the Picard loop needs on the device: macro and sdkm
+ * integrals, the three LU factorizations, all least-squares systems, and the
+ * fields (initial values vel(nn,3), vort(nn,3), qvort(nq,3), temp(nn),
+ * qtemp(nq)).  sd_prec selects the storage of the dense kinematics matrices;
+ * for anything but FP64 the boundary-vorticity solve is wrapped in FP64
+ * iterative refinement to rel. residual refine_tol. */
+int tt_case_create(const tt_case_view *c, const double *vel, const double *vort,
+                   const double *qvort, const double *temp, const double *qtemp, int32_t sd_prec,
+                   double refine_tol, tt_case **out);
+/* One Picard iteration (main.f90:1200-1589): err[9] = the nine relative
+ * changes (tri.sta), nit[12] = solver iterations (tri.ite).  Returns 1 while
+ * the loop should continue, 0 when converged or at nnlit, < 0 on error. */
+int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]);
+/* Runs Picard iterations until convergence; returns the iteration count. */
+int tt_case_run(tt_case *c, int32_t max_iter, double *sta, int32_t *ite);
+/* Copies the current fields to host arrays (any may be NULL). */
+int tt_case_fields(tt_case *c, double *vel, double *vort, double *qvort, double *temp,
+                   double *qtemp);
+/* Device time of the last Picard step split by phase (ms): [0] sdkm products,
+ * [1] LU solves, [2] KM LSQR, [3] T LSQR, [4] VTE LSQR, [5] RHS builders,
+ * [6] total; and setup times (ms): [7] macro integrals, [8] sdkm integrals,
+ * [9] LU factorization, [10] system build. */
+int tt_case_timings(tt_case *c, double t[11]);
+/* HBM bytes held by the case on the device. */
+int64_t tt_case_device_bytes(tt_case *c);
+void tt_case_free(tt_case *c);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side preprocessing (the reference's Fortran host side; these run on
+ * the CPU and produce the arrays above).                                    */
+
+typedef struct tt_host tt_host; /* mesh + macro mesh + boundary conditions */
+
+/* ReadMeshGid3D (lib_mesh.f:279-504) incl. CheckLinearGeometry and
+ * SourcePointNormals, then GenMakroMesh (lib_mesh.f:865-982). */
+int tt_host_read_geo(const char *path, tt_host **out);
+/* The same from arrays x(nn,3), ibc(ne,9), idc(nc,27), bewn(ne). */
+int tt_host_from_arrays(int32_t nn, int32_t nb, int32_t ne, int32_t nc, const double *x,
+                        const int32_t *ibc, const int32_t *idc, const int32_t *bewn,
+                        tt_host **out);
+/* Structured unit-cube mesh of n^3 27-node cells in the GiD conventions of
+ * the reference's mesh/kva*.geo (walls 1:z=0 2:y=0 3:x=1 4:y=1 5:x=0 6:z=1). */
+int tt_host_cube(int32_t n, tt_host **out);
+/* ReadBic (lib_bic.f:50-502) from the text of a .bic file, DetKMunk3 and
+ * FillKobc. */
+int tt_host_read_bic(tt_host *h, const char *bic_text);
+/* ReadInputFile (lib_IO.f:2-359) keywords that the hot path uses. */
+int tt_host_read_inp(tt_host *h, const char *inp_text);
+/* Fills a case view pointing into h (valid while h lives). */
+int tt_host_case_view(tt_host *h, tt_case_view *v);
+/* Initial fields after ReadBic (pointers into h). */
+int tt_host_fields(tt_host *h, const double **vel, const double **vort, const double **qvort,
+                   const double **temp, const double **qtemp);
+/* Writes the mesh as a GiD .geo file the reference can read. */
+int tt_host_write_geo(tt_host *h, const char *path);
+void tt_host_free(tt_host *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

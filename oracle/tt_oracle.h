@@ -1,0 +1,167 @@
+/*
+ * tt_oracle.h -- CPU restatement of the Tritok boundary-element hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP library
+ * (tritok_amd) is compared against, and the timed CPU baseline.  Only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py load it; the
+ * product never links or calls it.
+ *
+ * Every routine follows one reference routine operation by operation (same loop
+ * order, same expression association) so that results agree with the reference
+ * binary to rounding.  Citations are file:line into the reference's src/.
+ *
+ * Array conventions (the reference's, so a Fortran host can pass its arrays
+ * unchanged): column-major, index arrays 1-based.
+ *   x[nn*3]            node coordinates  x(nnodes,3)
+ *   idc[nc*27]         cell connectivity idc(nicell,27)
+ *   ibc[ne*9]          boundary elements ibc(nbelem,9)
+ *   ibf[nc*6*4]        cell face flux nodes ibf(nicell,6,4)
+ *   xq[nq*3]           flux node coordinates xq(nq,3)
+ *   flpm[nc*6]         flux sign per cell face flpm(nicell,6)
+ *   gbn[nb], lbn[nn]   boundary node list / inverse (0 = interior)
+ *
+ * Pinning (how each function is checked in tests/test_oracle.py):
+ *   or_gauss                 Golub-Welsch closed form (numpy leggauss) and the
+ *                            polynomial-exactness property.
+ *   or_shape27/9/dl34        Kronecker property at nodes, partition of unity,
+ *                            exact integrals of products (lib_int.f:3461 21/4).
+ *   or_inted27, or_inteb9    reference binary matrices (tests/golden npz files),
+ *                            rigid-body / linear-field identities, brute force.
+ *   or_macro_integrals       reference tri.macro.int (dc2, nc2, jit2, dc8).
+ *   or_sd_integrals          reference tri.sdkm.int.1 (dc2, jit2, dc8 rows).
+ *   or_lsqr2, or_dgefa/dgesl scipy lsqr / LAPACK getrf on the same systems; the
+ *                            reference's per-solve iteration counts (tri.ite).
+ *   or_picard                reference converged fields (tri.rst) and per-
+ *                            iteration histories (tri.sta / tri.ite).
+ */
+#ifndef TT_ORACLE_H
+#define TT_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Gauss-Legendre rule by reference index 1..9 -> N = 2,4,6,8,16,32,40,48,64
+ * (GaussPosWeights, lib_int.f:2673-3173), computed.  Returns N, fills gi[N],
+ * ome[N]. */
+int or_gauss(int rule, double *gi, double *ome);
+
+/* Shape functions (lib_interpol.f): 27-node cell (cshape27 :352-405),
+ * 9-node face (cshape9 :597-619), discontinuous 4-node flux (dl34shape4
+ * :571-592). */
+void or_shape27(double xi, double eta, double zeta, double *fi);
+void or_shape9(double xi, double eta, double *fi);
+void or_dl34(double xi, double eta, double *fi);
+
+/* Quadrature configuration: GAUS kmBr kmBs kmDr kmDs and IIDI, and
+ * optionally the table of TYPE(GaussType) (gi(220), ome(220), ng1(9), ng2(9));
+ * with a NULL table the rules come from or_gauss. */
+typedef struct {
+  int kmBr, kmBs, kmDr, kmDs, iiDiv;
+  const double *gi, *ome;
+  const int *ng1, *ng2;
+} or_quad;
+
+/* INTEDc27dcLap (lib_int.f:4-398): integrals over one 27-node cell with 8-node
+ * geometry.  corners[8][3] in the reference's argument order x1..x8 =
+ * idc(6,7,2,3,5,8,1,4).  isrc 0 = regular, 1..27 node, 28..51 flux point.
+ * Outputs be/dxe/dye/dze[27].  Returns the number of kernel evaluations. */
+long or_inted27(const or_quad *q, const double p[3], const double corners[8][3], int isrc,
+                double *be, double *dxe, double *dye, double *dze);
+
+/* INTEBc9kmdclap (lib_int.f:1462-1779) when macro != 0, INTEBc9sdkm
+ * (lib_sdkm.f:2082-2376, no subdivision, no ge/ad) when macro == 0.
+ * corners[4][3] = ibc(1,3,5,7).  isrc 0 regular, 1..9 node, 10..13 flux.
+ * Outputs he, adx, ady, adz, heyz, hezx, hexy [9] and ge[4] (macro only;
+ * pass NULL for unused). Returns the number of kernel evaluations. */
+long or_inteb9(const or_quad *q, int macro, const double p[3], const double corners[4][3], int isrc,
+               double minedge, double *he, double *ge, double *adx, double *ady, double *adz,
+               double *heyz, double *hezx, double *hexy);
+
+/* FMATkmdcSteady (lib_fmat.f:3-446): all per-cell macro matrices, column-major.
+ * Rows of H,G,B,Abd*,D* = nc*51, rows of Ht* = nc*27.  ont[11] in the order of
+ * the reference's cache file.  copy as the COPY keyword.  Cells are processed in
+ * parallel with OpenMP when nthreads > 1. */
+void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *x,
+                        const int *idc, const int *ibf, const double *xq, const int *flpm,
+                        int copy, double *H, double *G, double *B, double *Abdx, double *Abdy,
+                        double *Abdz, double *Htx, double *Hty, double *Htz, double *Dx,
+                        double *Dy, double *Dz, double ont[11], int nthreads);
+
+/* FMATsdkm (lib_sdkm.f:1286-1530) for boundary rows row0..row1-1 (0-based),
+ * matrices column-major with leading dimension ld (>= row1-row0):
+ * H,Htx,Hty,Htz (ld x nb), Dx,Dy,Dz (ld x nn).  ont[10] as the cache file. */
+void or_sd_integrals(const or_quad *q, int nn, int nb, int ne, int nc, const double *x,
+                     const int *ibc, const int *idc, const int *gbn, const int *lbn, int row0,
+                     int row1, int ld, double *H, double *Htx, double *Hty, double *Htz,
+                     double *Dx, double *Dy, double *Dz, double ont[10], int nthreads);
+
+/* ---- linear algebra (lib_Solve_v11.f, lib_Blas.f) ---------------------- */
+
+/* DNRM2 (lib_Blas.f:263-322), scaled two-norm. */
+double or_dnrm2(int n, const double *x);
+
+/* prelsqr2 (lib_Solve_v11.f:4501-4535): q_j = 1/||A(:,j)||, A <- A diag(q).
+ * CSR is 1-based (ia[m+1], ja[nnz]). */
+void or_prelsqr2(int m, int n, const int *ia, const int *ja, double *a, double *q);
+
+/* slvlsqr2 + LSQR (lib_Solve_v11.f:4401-4500, :3128-3887).  eps4 is the
+ * single-precision tolerance the reference passes.  x holds the hot start on
+ * entry and the solution on exit.  Returns istop; *nit = iterations. */
+int or_slvlsqr2(int m, int n, const int *ia, const int *ja, const double *a, const double *q,
+                const double *b, double *x, int maxit, float eps4, int *nit);
+
+/* LINPACK dgefa / dgesl (lib_Solve_v11.f:787-1037), column-major a(lda,n),
+ * pivots stored as doubles (ipvt is REAL(8) in the reference). */
+int or_dgefa(double *a, int lda, int n, double *ipvt);
+void or_dgesl(const double *a, int lda, int n, const double *ipvt, double *b);
+
+/* ---- the Picard driver ------------------------------------------------- */
+
+/* A complete steady case after preprocessing (mesh, boundary conditions,
+ * numbering), as the reference's main.f90 holds it before the time loop.  All
+ * arrays follow the conventions above; codes follow lib_bic.f:411-488
+ * (negative = unknown number, positive = known number). */
+typedef struct {
+  int nn, nb, ne, nc, nq;
+  const double *x;
+  const int *idc, *ibc, *ibf, *flpm, *gbn, *lbn;
+  const double *xq;
+  const double *nx, *ny, *nz;   /* source point normals, SourcePointNormals */
+  const int *kode;              /* kode(nb,3) */
+  const int *kobc;              /* kobc(nb), nearest node for dv/dn=0 */
+  const int *wkode, *wqkode;    /* wkode(nn,3), wqkode(nq,3) */
+  const int *Tkode, *Tqkode;    /* Tkode(nn), Tqkode(nq) */
+  int nunk[3], nbk[3], Tnunk, Tnb;
+  const int *kmu;               /* kmp%u(nn) */
+  int nsmcol, nrhscol;
+  /* physics and numerics (tri.inp) */
+  int iDT;                      /* 0 or 1 */
+  double ReN, RaN, PrN, gx, gy, gz, tstep;
+  double urBw[3], urDv[3], urDw[3], urDT;
+  double eps;                   /* RELA eps */
+  int nnlit, lsqs_maxit, iDLSE;
+  float lsqs_eps, dlse_r, dlse_epsth;
+  or_quad quad;
+  int copy;
+} or_case;
+
+/* Fields in/out: velocity(nn,3), vorticity(nn,3), qvorticity(nq,3), temp(nn),
+ * qtemp(nq), initialised as after ReadBic.  History out (may be NULL):
+ * sta[maxhist*9] relative changes, ite[maxhist*12] solver iterations.
+ * Returns the number of Picard iterations run. */
+int or_picard(const or_case *c, double *velocity, double *vorticity, double *qvorticity,
+              double *temp, double *qtemp, int maxhist, double *sta, int *ite, int nthreads,
+              double *seconds_per_iter);
+
+/* Pieces of the Picard step, exported for unit tests and the CPU baseline.
+ * sdkm_rhs (lib_sdkm.f:349-519) for equation eqn (1..3) over rows r0..r1-1. */
+void or_sdkm_rhs(int eqn, int nn, int nb, const int *lbn, const int *gbn, const int *kode,
+                 const double *nx, const double *ny, const double *nz, const double *H,
+                 const double *Htx, const double *Hty, const double *Htz, const double *Dx,
+                 const double *Dy, const double *Dz, int ld, const double *vel, const double *vort,
+                 double *dfi, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

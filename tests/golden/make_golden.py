@@ -1,0 +1,315 @@
+#!/usr/bin/env python3
+"""Generate golden fixtures by running the reference Tritok binary.
+
+The binary is built from the reference's own sources by ``oracle/Makefile.ref``
+into ``oracle/_ref/Tritok``.  This script (run in the build container, where the
+reference tree exists) writes each case's inputs into a scratch directory, runs
+the reference, and converts its outputs into ``tests/golden/<case>.npz``:
+
+* ``tri.macro.int``  -> per-cell macro integral matrices (Fortran unformatted,
+  written by WriteMacroIntDisk, reference src/lib_int.f:1341-1377);
+* ``tri.sdkm.int.1`` -> single-domain kinematics matrices (WriteSDIntDisk,
+  src/lib_sdkm.f:2641-2676);
+* ``tri.rst``        -> converged fields (WrstF, src/lib_IO.f:604-685);
+* ``tri.ite`` / ``tri.sta`` / ``tri.sol`` -> per-Picard-iteration solver
+  iteration counts, relative changes and dynamic LSQR tolerances;
+* ``gauss_ref.npz``  -> the reference's Gauss-Legendre table as its own
+  GaussPosWeights (src/lib_int.f:2673-3173) fills it, written by the small
+  Fortran harness dump_gauss.f90 linked against the reference's objects.
+
+The mesh arrays (coordinates and connectivity, as read from the .geo) and the
+boundary-condition file text are stored too, so the tests can feed the very same
+inputs to this package without the reference tree.
+
+Usage:  python tests/golden/make_golden.py [case ...]
+"""
+import os
+import shutil
+import struct
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REFBIN = os.path.join(REPO, "oracle", "_ref", "Tritok")
+REFMESH = "/root/reference/mesh"
+
+BASE_INP = """MDIR ./
+LGEO {geo}
+LBIC {bic}
+URBW 1.0 1.0 1.0
+URDV 1.0 1.0 1.0
+URDW {urdw} {urdw} {urdw}
+URDT {urdt}
+URDC 1.0
+{phys}
+PARA 0 0
+GRAV 0 0 -1
+EQNS 1 1 1 {idt} 0 0
+RELA {nnlit} 1.0 1.0D-6
+TIME 1.0D20 1 1
+LSQS 5000 1.0E-7
+DLSE 1 10.0 0.001
+PKMS 500 1.0E-7
+GAUS 4 4 4 4
+IIDI 2
+SCRO 0
+REST 0
+"""
+
+CASES = {
+    # run/dc/tri.inp of the reference, as shipped: cavity, 2^3 cells, Re=1.
+    "dc2": dict(geo="kva2x2x2.geo", bic="3Dldc-X.bic", urdw=0.3, urdt=0.1,
+                phys="REYN 1.0", idt=0, nnlit=10000),
+    # run/nc/tri.inp of the reference: natural convection, 2^3 cells, Ra=1e3.
+    "nc2": dict(geo="kva2x2x2.geo", bic="NaturalConvection-X.bic", urdw=0.3, urdt=0.1,
+                phys="RAPR 1000.0 0.71\nREYN 1.0", idt=1, nnlit=10000),
+    # the 2^3 cube with in-plane jittered cell corners (every cell distinct),
+    # cavity at Re=10: exercises distorted-cell quadrature and the full loop.
+    "jit2": dict(geo="jit2x2x2.geo", bic="3Dldc-X.bic", urdw=0.3, urdt=0.1,
+                 phys="REYN 10.0", idt=0, nnlit=10000, jitter=(0.1, 20251203)),
+    # lid-driven cavity on the 8^3 mesh at Re=100, relaxation 0.1.
+    "dc8": dict(geo="kva8x8x8.geo", bic="3Dldc-X.bic", urdw=0.1, urdt=0.1,
+                phys="REYN 100.0", idt=0, nnlit=10000, sd_rows=5),
+    # run/dc with tight tolerances (RELA eps 1e-10, LSQR eps 1e-12, no dynamic
+    # LSQR tolerance): the converged fields are then fixed to ~1e-10 by the
+    # discrete equations rather than by the solver trajectory.
+    "dc2t": dict(geo="kva2x2x2.geo", bic="3Dldc-X.bic", urdw=0.3, urdt=0.1,
+                 phys="REYN 1.0", idt=0, nnlit=10000, tight=True),
+    "nc2t": dict(geo="kva2x2x2.geo", bic="NaturalConvection-X.bic", urdw=0.3, urdt=0.1,
+                 phys="RAPR 1000.0 0.71\nREYN 1.0", idt=1, nnlit=10000, tight=True),
+}
+
+
+# ----------------------------------------------------------------------------
+# .geo reading / writing (GiD format, reference src/lib_mesh.f:279-391)
+# ----------------------------------------------------------------------------
+def _data_lines(path):
+    with open(path, "r", newline="") as f:
+        for raw in f:
+            line = raw.rstrip("\r\n")
+            if not line.strip() or "#" in line:
+                continue
+            yield line
+
+
+def read_geo(path):
+    it = _data_lines(path)
+    next(it)  # VERS
+    next(it)  # title
+    nbelem, nicell, nnodes, nbnodes, npob, npoc, npx = [int(v) for v in next(it).split()[:7]]
+    x = np.zeros((nnodes, 3))
+    for _ in range(nnodes):
+        t = next(it).split()
+        x[int(t[0]) - 1] = [float(v) for v in t[1:4]]
+    ibc = np.zeros((nbelem, npob), np.int32)
+    for _ in range(nbelem):
+        t = [int(v) for v in next(it).split()]
+        ibc[t[0] - 1] = t[1:1 + npob]
+    idc = np.zeros((nicell, npoc), np.int32)
+    for _ in range(nicell):
+        t = [int(v) for v in next(it).split()]
+        idc[t[0] - 1] = t[1:1 + npoc]
+    bewn = np.zeros(nbelem, np.int32)
+    for _ in range(nbelem):
+        t = [int(v) for v in next(it).split()]
+        bewn[t[0] - 1] = t[1]
+    return dict(x=x, ibc=ibc, idc=idc, bewn=bewn, nbnodes=nbnodes)
+
+
+def write_geo(path, g, title="jittered"):
+    x, ibc, idc, bewn = g["x"], g["ibc"], g["idc"], g["bewn"]
+    with open(path, "w", newline="") as f:
+        w = lambda s: f.write(s + "\r\n")
+        w("VERS GiD-3D-1.0")
+        w(title)
+        w("%7d %7d %7d %7d %7d %7d %7d" % (len(ibc), len(idc), len(x), g["nbnodes"], 9, 27, 3))
+        for k, p in enumerate(x):
+            w("%8d %23.16e %23.16e %23.16e" % (k + 1, p[0], p[1], p[2]))
+        for k, r in enumerate(ibc):
+            w("%7d " % (k + 1) + " ".join("%7d" % v for v in r))
+        for k, r in enumerate(idc):
+            w("%7d " % (k + 1) + " ".join("%7d" % v for v in r))
+        for k, v in enumerate(bewn):
+            w("%7d %d" % (k + 1, v))
+
+
+def jitter_corners(g, frac, seed):
+    """Move every cell-corner node by U(-frac*h, frac*h) per coordinate, keeping
+    nodes on the unit-cube walls in their wall planes.  Non-corner nodes are left
+    where they are: the reference re-places them trilinearly
+    (CheckLinearGeometry, src/lib_mesh.f:135-196)."""
+    rng = np.random.default_rng(seed)
+    x = g["x"].copy()
+    corners = np.unique(g["idc"][:, [5, 6, 1, 2, 4, 7, 0, 3]].ravel()) - 1
+    h = 1.0 / round(len(g["idc"]) ** (1.0 / 3.0))
+    d = rng.uniform(-frac * h, frac * h, size=(len(corners), 3))
+    on_wall = (np.abs(x[corners]) < 1e-12) | (np.abs(x[corners] - 1.0) < 1e-12)
+    d[on_wall] = 0.0
+    x[corners] += d
+    out = dict(g)
+    out["x"] = x
+    return out
+
+
+# ----------------------------------------------------------------------------
+# Fortran unformatted sequential records (4-byte markers, little endian)
+# ----------------------------------------------------------------------------
+def read_records(path):
+    recs = []
+    with open(path, "rb") as f:
+        data = f.read()
+    p = 0
+    while p < len(data):
+        (n,) = struct.unpack_from("<i", data, p)
+        recs.append(data[p + 4:p + 4 + n])
+        (n2,) = struct.unpack_from("<i", data, p + 4 + n)
+        assert n == n2, "bad record marker"
+        p += 8 + n
+    return recs
+
+
+def doubles(recs):
+    return np.frombuffer(b"".join(recs), dtype="<f8").copy()
+
+
+def parse_macro(path, nicell):
+    recs = read_records(path)
+    v = doubles(recs[2:])
+    nsp, npoc = 51, 27
+    m1, m2 = nicell * nsp, nicell * npoc
+    shapes = [("H", m1, 27), ("G", m1, 24), ("B", m1, 27), ("Abdx", m1, 27), ("Abdy", m1, 27),
+              ("Abdz", m1, 27), ("Htx", m2, 27), ("Hty", m2, 27), ("Htz", m2, 27),
+              ("Dx", m1, 27), ("Dy", m1, 27), ("Dz", m1, 27)]
+    out, p = {}, 0
+    for name, r, c in shapes:
+        out["macro_" + name] = v[p:p + r * c].reshape((c, r)).T.copy()  # column-major
+        p += r * c
+    out["macro_ont"] = v[p:p + 11]
+    assert p + 11 == len(v)
+    return out
+
+
+def parse_sdkm(path, nb, nn, rows=None):
+    recs = read_records(path)
+    v = doubles(recs[2:])
+    out, p = {}, 0
+    for name, c in [("H", nb), ("Htx", nb), ("Hty", nb), ("Htz", nb), ("Dx", nn), ("Dy", nn), ("Dz", nn)]:
+        m = v[p:p + nb * c].reshape((c, nb)).T
+        out["sd_" + name] = (m if rows is None else m[rows]).copy()
+        p += nb * c
+    out["sd_ont"] = v[p:p + 10]
+    if rows is not None:
+        out["sd_rows"] = np.asarray(rows, np.int32)
+    return out
+
+
+def parse_rst(path, nn, nq):
+    recs = read_records(path)
+    v = doubles(recs[3:])
+    names = ([("velocity", nn, 3), ("pnlsvelocity", nn, 3), ("vorticity", nn, 3),
+              ("ptsvorticity", nn, 3), ("pptsvorticity", nn, 3), ("pnlsvorticity", nn, 3),
+              ("qvorticity", nq, 3), ("temp", nn, 1), ("ptstemp", nn, 1), ("pptstemp", nn, 1),
+              ("pnlstemp", nn, 1), ("qtemp", nq, 1)])
+    out, p = {}, 0
+    for name, r, c in names:
+        out["rst_" + name] = v[p:p + r * c].reshape((c, r)).T.copy()
+        p += r * c
+    return out
+
+
+def parse_table(path, ncol):
+    rows = []
+    with open(path) as f:
+        for line in f.readlines()[2:]:
+            t = line.split()
+            if len(t) >= ncol:
+                rows.append([float(s.replace("D", "E")) for s in t[:ncol]])
+    return np.array(rows)
+
+
+def dump_gauss():
+    objdir = os.path.join(REPO, "oracle", "_ref", "obj")
+    objs = sorted(os.path.join(objdir, f) for f in os.listdir(objdir)
+                  if f.endswith(".o") and f != "main.o")
+    work = tempfile.mkdtemp(prefix="tt_gauss_")
+    exe = os.path.join(work, "dump_gauss")
+    subprocess.run(["amdflang", "-O2", "-I" + objdir, os.path.join(HERE, "dump_gauss.f90")] + objs +
+                   ["-L/opt/conda/lib", "-lmpifort", "-lmpi", "-Wl,-rpath,/opt/conda/lib", "-o", exe], check=True)
+    subprocess.run([exe], cwd=work, check=True)
+    with open(os.path.join(work, "gauss_ref.bin"), "rb") as f:
+        b = f.read()
+    ng = np.frombuffer(b[:72], np.int32)
+    v = np.frombuffer(b[72:], np.float64)
+    np.savez_compressed(os.path.join(HERE, "gauss_ref.npz"), ng1=ng[:9].copy(), ng2=ng[9:].copy(),
+                        gi=v[:220].copy(), ome=v[220:].copy())
+    shutil.rmtree(work)
+    print("gauss_ref ->", os.path.join(HERE, "gauss_ref.npz"))
+
+
+def run_case(name, spec, keep=None):
+    geo_src = os.path.join(REFMESH, spec["geo"].replace("jit", "kva"))
+    g = read_geo(geo_src)
+    work = keep or tempfile.mkdtemp(prefix="tt_golden_%s_" % name)
+    os.makedirs(work, exist_ok=True)
+    if "jitter" in spec:
+        g = jitter_corners(g, *spec["jitter"])
+        write_geo(os.path.join(work, spec["geo"]), g)
+    else:
+        shutil.copy(geo_src, os.path.join(work, spec["geo"]))
+    shutil.copy(os.path.join(REFMESH, spec["bic"]), os.path.join(work, spec["bic"]))
+    inp = BASE_INP.format(**spec)
+    if spec.get("tight"):
+        inp = (inp.replace("RELA 10000 1.0 1.0D-6", "RELA 10000 1.0 1.0D-10")
+               .replace("LSQS 5000 1.0E-7", "LSQS 5000 1.0E-12").replace("DLSE 1 10.0 0.001", "DLSE 0 10.0 0.001"))
+    with open(os.path.join(work, "tri.inp"), "w") as f:
+        f.write(inp)
+    if not os.path.exists(os.path.join(work, "tri.rst")):
+        subprocess.run([REFBIN], cwd=work, check=True, stdout=subprocess.DEVNULL)
+    nn, nb, nc = len(g["x"]), g["nbnodes"], len(g["idc"])
+    nq = 12 * round(nc ** (1 / 3)) ** 2 * (round(nc ** (1 / 3)) + 1)
+    out = dict(mesh_x=g["x"], mesh_ibc=g["ibc"], mesh_idc=g["idc"], mesh_bewn=g["bewn"],
+               mesh_nbnodes=np.int32(nb))
+    with open(os.path.join(work, spec["bic"])) as f:
+        out["bic_text"] = np.frombuffer(f.read().encode(), np.uint8)
+    with open(os.path.join(work, "tri.inp")) as f:
+        out["inp_text"] = np.frombuffer(f.read().encode(), np.uint8)
+    if spec.get("tight"):
+        pass  # same integrals as the default-tolerance case: fields only
+    elif nc <= 8:
+        out.update(parse_macro(os.path.join(work, "tri.macro.int"), nc))
+    else:  # keep the first and last cell blocks only
+        m = parse_macro(os.path.join(work, "tri.macro.int"), nc)
+        for k in list(m):
+            if k == "macro_ont":
+                continue
+            per = m[k].shape[0] // nc
+            m[k] = np.concatenate([m[k][:per], m[k][-per:]])
+        m["macro_cells"] = np.array([1, nc], np.int32)
+        out.update(m)
+    rows = None
+    if spec.get("sd_rows"):
+        rows = np.unique(np.linspace(0, nb - 1, spec["sd_rows"]).round().astype(int))
+    if not spec.get("tight"):
+        out.update(parse_sdkm(os.path.join(work, "tri.sdkm.int.1"), nb, nn, rows))
+    out.update(parse_rst(os.path.join(work, "tri.rst"), nn, nq))
+    out["ite"] = parse_table(os.path.join(work, "tri.ite"), 14).astype(np.int32)
+    out["sta"] = parse_table(os.path.join(work, "tri.sta"), 11)
+    out["sol"] = parse_table(os.path.join(work, "tri.sol"), 11)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "->", os.path.join(HERE, name + ".npz"), "picard its:", len(out["sta"]))
+    if keep is None:
+        shutil.rmtree(work)
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or ["gauss"] + list(CASES)
+    keep = os.environ.get("TT_GOLDEN_KEEP")
+    if "gauss" in names:
+        dump_gauss()
+        names = [n for n in names if n != "gauss"]
+    for n in names:
+        run_case(n, CASES[n], keep=os.path.join(keep, n) if keep else None)

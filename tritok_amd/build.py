@@ -1,0 +1,86 @@
+"""Builds libtritok_amd.so (HIP for gfx950 + host C++) in-tree, and the CPU
+oracle used by the tests.
+
+    python tritok_amd/build.py          # library + oracle (+ reference if present)
+"""
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libtritok_amd.so")
+OBJDIR = os.path.join(HERE, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(REPO, "include"), "-I" + CSRC,
+          "-Wall", "-Wno-unused-function"]
+# device code: gfx950 only; packed FP32 instructions disabled
+HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-ffp-contract=fast", "-munsafe-fp-atomics"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def _stale(obj, srcs):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(s) > t for s in srcs)
+
+
+def build_library(force=True, jobs=8):
+    os.makedirs(OBJDIR, exist_ok=True)
+    headers = glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(REPO, "include", "*.h"))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.cpp")) + glob.glob(os.path.join(CSRC, "*.hip")))
+    procs, objs = [], []
+    for s in srcs:
+        o = os.path.join(OBJDIR, os.path.basename(s) + ".o")
+        objs.append(o)
+        if not force and not _stale(o, [s] + headers):
+            continue
+        if s.endswith(".hip"):
+            cmd = [HIPCC, "-x", "hip"] + COMMON + HIPFLAGS + ["-c", s, "-o", o]
+        else:
+            cmd = [HIPCC, "-x", "c++", "-D__HIP_PLATFORM_AMD__"] + COMMON + ["-c", s, "-o", o]
+        print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+        if len(procs) >= jobs:
+            for p in procs:
+                if p.wait() != 0:
+                    raise RuntimeError("compilation failed")
+            procs = []
+    for p in procs:
+        if p.wait() != 0:
+            raise RuntimeError("compilation failed")
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lamdhip64", "-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    _run(["make", "-C", os.path.join(REPO, "oracle"), "-B"])
+
+
+def build_reference():
+    """Compiles the reference binary into oracle/_ref when its sources exist."""
+    if os.path.isdir("/root/reference/src"):
+        _run(["make", "-f", os.path.join(REPO, "oracle", "Makefile.ref"), "-C", REPO, "-j8"])
+
+
+def main():
+    build_library(force="--incremental" not in sys.argv)
+    build_oracle()
+    if "--no-ref" not in sys.argv:
+        build_reference()
+
+
+if __name__ == "__main__":
+    main()
