@@ -1,0 +1,165 @@
+"""Picard iterations/s of the steady natural-convection case on one MI355X.
+
+Workload: differentially heated unit cube at n^3 = 20^3 subdomains (27-node
+cells, 41^3 = 68,921 nodes, 9,602 boundary nodes), Ra = 1e5, Pr = 0.71, the
+reference's iteration parameters (GAUS 4 4 4 4, IIDI 2, LSQS 5000 1e-7,
+DLSE 1 10 0.001, URDW 0.3, URDT 0.1).  Setup (integrals, factorisations,
+least-squares systems) is done once; the matrices stay resident in HBM.
+
+A step is one Picard iteration (main.f90:1200-1589): boundary vorticities,
+kinematics x,y,z, energy, vorticity transport x,y,z, convergence test and
+dynamic LSQR tolerance.  The FP64 leg and the mixed leg (FP32 least-squares
+blocks, FP64 hot-start residuals and scalars) start from the same initial
+fields; `value` is the mixed leg's rate, the FP64 leg's is reported beside
+it together with the relative L2 difference of the two legs' fields.
+
+    python bench.py [--steps K] [--warmup W] [--n 20] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "Picard iters/s at 20³ subdomains, mixed vs FP64 at 1e-8 rel; p99 solve latency"
+HBM_PEAK_GBS = 8000.0
+FIELDS = ("velocity", "vorticity", "qvorticity", "temp", "qtemp")
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def run_leg(case, prec, steps, warmup, probe_stride):
+    case.set_precision(prec)
+    case.reset()
+    for _ in range(warmup):
+        case.step()
+    case.probe(probe_stride)
+    lat, nits, phases = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        rc, err, nit = case.step()
+        lat.append(time.perf_counter() - ts)
+        nits.append(nit.copy())
+        phases.append(case.timings())
+    total = time.perf_counter() - t0
+    stats51 = case.probe_stats(51)
+    stats27 = case.probe_stats(27)
+    case.probe(0)
+    return dict(total_s=total, lat=np.array(lat), nits=np.array(nits), phases=phases, probe51=stats51,
+                probe27=stats27, fields=case.fields())
+
+
+def lsqr_matrix_bytes(info, value_bytes):
+    """Algorithmic bytes of the two sparse products of one LSQR iteration for a
+    CSR system (values + int32 column indices, row pointers), per the
+    reference's CSR form: 2 * ((value_bytes + 4) * nnz + 4 * (m + 1))."""
+    return 2.0 * ((value_bytes + 4) * info["nnz"] + 4 * (info["m"] + 1))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=20)
+    ap.add_argument("--probe-stride", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one process drives one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from tritok_amd import workloads
+    from tritok_amd.case import Case
+    from tritok_amd import device_info
+
+    dev = device_info()
+    log("device", dev)
+    n = args.n
+    t0 = time.perf_counter()
+    host = workloads.natural_convection(n, ra=1e5, pr=0.71)
+    view = host.view()
+    log("host preprocessing %.2fs  nnodes=%d nbnodes=%d cells=%d" % (time.perf_counter() - t0, view.mesh.nnodes,
+                                                                     view.mesh.nbnodes, view.mesh.nicell))
+    t0 = time.perf_counter()
+    case = Case(host, prec="fp32")
+    setup_s = time.perf_counter() - t0
+    tm = case.timings()
+    log("setup %.2fs (macro %.0f ms, sdkm %.0f ms, LU %.0f ms, systems %.0f ms), device bytes %.2f GB" % (
+        setup_s, tm["setup_macro"], tm["setup_sdkm"], tm["setup_lu"], tm["setup_systems"],
+        case.device_bytes() / 1e9))
+
+    legs = {}
+    for prec in ("fp64", "fp32"):
+        legs[prec] = run_leg(case, prec, args.steps, args.warmup, args.probe_stride)
+        L = legs[prec]
+        log("%s: %.3f s for %d steps (%.1f ms/step), LSQR iterations per step %s" % (
+            prec, L["total_s"], args.steps, 1e3 * L["total_s"] / args.steps, L["nits"].sum(0)[3:10] / args.steps))
+    f64, f32 = legs["fp64"]["fields"], legs["fp32"]["fields"]
+    diff = {k: rel_l2(f32[k], f64[k]) for k in FIELDS}
+    log("mixed vs fp64 rel L2 after %d iterations: %s" % (args.warmup + args.steps, diff))
+
+    mixed = legs["fp32"]
+    value = args.steps / mixed["total_s"]
+    fp64_rate = args.steps / legs["fp64"]["total_s"]
+    lat = np.sort(mixed["lat"]) * 1e3
+
+    # roofline of the dominant kernel: lsqr_A on the 51-row systems (VTE x, y, z and T)
+    infos = [case.system_info(s) for s in range(4)]
+    nit = mixed["nits"].sum(0)
+    w = np.array([nit[6], nit[7], nit[8], nit[9]], float)
+    alg = np.array([lsqr_matrix_bytes(i, 4) for i in infos])
+    alg_bytes = float((w * alg).sum() / max(w.sum(), 1.0))
+    avg_ms = mixed["probe51"]["avg_ms"]
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
+    roofline = {"kernel": "lsqr_A<float> (51-row blocks)", "bound": "hbm",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                "avg_launch_ms": avg_ms, "launches_timed": mixed["probe51"]["sampled"],
+                "launches": mixed["probe51"]["launches"], "algorithmic_bytes_per_launch": alg_bytes,
+                "block_bytes_per_launch": float(infos[0]["cells"] * 51 * 51 * 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import cpu_sample  # the oracle: CPU baseline leg only
+        cpu = cpu_sample.picard_rate(host, case, legs["fp64"]["nits"], budget_s=args.cpu_seconds, log=log)
+
+    ph = mixed["phases"]
+    out = {
+        "metric": METRIC, "value": value, "unit": "Picard iterations/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * mixed["total_s"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
+        "config": {"workload": "natural convection, differentially heated cube, %d^3 subdomains (%d nodes, %d "
+                               "boundary nodes), Ra=1e5, Pr=0.71" % (n, view.mesh.nnodes, view.mesh.nbnodes),
+                   "subdomains": n ** 3, "precision": "FP32 least-squares blocks, FP64 residual anchors"},
+        "fp64_iters_per_s": fp64_rate, "mixed_over_fp64": value / fp64_rate,
+        "mixed_vs_fp64_rel_l2": max(diff.values()),
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                            "max": float(lat[-1])},
+        "phase_ms_mean": {k: float(np.mean([p[k] for p in ph])) for k in
+                          ("boundary_vorticity", "km", "temperature", "vte", "convergence", "step_total")},
+        "lsqr_iters_per_step": {"fp64": (legs["fp64"]["nits"].sum(0)[3:10] / args.steps).tolist(),
+                                "mixed": (nit[3:10] / args.steps).tolist()},
+        "setup_s": setup_s, "device": dev, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    case.close()
+
+
+if __name__ == "__main__":
+    main()

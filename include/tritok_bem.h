@@ -167,19 +167,40 @@ typedef struct tt_case_view {
   float lsqs_eps, dlse_r, dlse_epsth;
 } tt_case_view;
 
-/* precision of the dense kinematics products: storage format of the
- * sdkm matrices on the device */
+/* storage precision of the iteration matrices */
 enum { TT_PREC_FP64 = 0, TT_PREC_FP32 = 1, TT_PREC_BF16 = 2, TT_PREC_FP16 = 3 };
 
 /* Builds everything the Picard loop needs on the device: macro and sdkm
- * integrals, the three LU factorizations, all least-squares systems, and the
+ * integrals, the three factorizations, all least-squares systems, and the
  * fields (initial values vel(nn,3), vort(nn,3), qvort(nq,3), temp(nn),
- * qtemp(nq)).  sd_prec selects the storage of the dense kinematics matrices;
- * for anything but FP64 the boundary-vorticity solve is wrapped in FP64
- * iterative refinement to rel. residual refine_tol. */
+ * qtemp(nq)).  prec = TT_PREC_FP64 runs every least-squares iteration on the
+ * FP64 blocks; TT_PREC_FP32 runs the bidiagonalisation products on FP32
+ * copies of the scaled blocks while the initial residual of every solve
+ * (b - A x0, the hot start), the LSQR scalars and norms and the solution
+ * update stay FP64, so the Picard loop refines the FP32 corrections against
+ * the FP64 equations.  refine_tol is reserved (pass 0). */
 int tt_case_create(const tt_case_view *c, const double *vel, const double *vort,
-                   const double *qvort, const double *temp, const double *qtemp, int32_t sd_prec,
+                   const double *qvort, const double *temp, const double *qtemp, int32_t prec,
                    double refine_tol, tt_case **out);
+/* Restarts the iteration from new initial fields on the resident matrices
+ * (the state main.f90:1145-1199 sets up before the first Picard step). */
+int tt_case_reset(tt_case *c, const double *vel, const double *vort, const double *qvort,
+                  const double *temp, const double *qtemp);
+/* Switches the least-squares storage precision (TT_PREC_FP64 / TT_PREC_FP32)
+ * of a resident case; FP32 copies are made on first use. */
+int tt_case_set_precision(tt_case *c, int32_t prec);
+/* Measurement hooks (no reference counterpart): time every stride-th launch
+ * of the least-squares block kernel with HIP events on the case's stream
+ * (stride 0 = off; resets the statistics), and read the average duration,
+ * timed and total launch counts per system shape (rows = 51: vorticity and
+ * energy systems, rows = 27: kinematics systems). */
+int tt_case_probe(tt_case *c, int32_t stride);
+int tt_case_probe_stats(tt_case *c, int32_t rows, double *avg_ms, int64_t *sampled,
+                        int64_t *launches);
+/* Sizes of a least-squares system (sys 0..2 vorticity x,y,z, 3 energy, 4
+ * kinematics): info = {rows m, unknowns n, nonzeros, cells, rows per cell,
+ * columns per cell}. */
+int tt_case_system_info(tt_case *c, int32_t sys, int64_t info[6]);
 /* One Picard iteration (main.f90:1200-1589): err[9] = the nine relative
  * changes (tri.sta), nit[12] = solver iterations (tri.ite).  Returns 1 while
  * the loop should continue, 0 when converged or at nnlit, < 0 on error. */
@@ -189,10 +210,11 @@ int tt_case_run(tt_case *c, int32_t max_iter, double *sta, int32_t *ite);
 /* Copies the current fields to host arrays (any may be NULL). */
 int tt_case_fields(tt_case *c, double *vel, double *vort, double *qvort, double *temp,
                    double *qtemp);
-/* Device time of the last Picard step split by phase (ms): [0] sdkm products,
- * [1] LU solves, [2] KM LSQR, [3] T LSQR, [4] VTE LSQR, [5] RHS builders,
- * [6] total; and setup times (ms): [7] macro integrals, [8] sdkm integrals,
- * [9] LU factorization, [10] system build. */
+/* Device time of the last Picard step split by phase (ms): [0] boundary
+ * vorticities (sdkm products and solves), [1] domain velocities (KM), [2]
+ * temperature, [3] domain vorticities (VTE), [4] relative changes, [5] unused,
+ * [6] whole step; and setup times (ms): [7] macro integrals, [8] sdkm
+ * integrals, [9] LU factorization and inversion, [10] system build. */
 int tt_case_timings(tt_case *c, double t[11]);
 /* HBM bytes held by the case on the device. */
 int64_t tt_case_device_bytes(tt_case *c);

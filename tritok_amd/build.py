@@ -60,7 +60,7 @@ def build_library(force=True, jobs=8):
         if p.wait() != 0:
             raise RuntimeError("compilation failed")
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lamdhip64", "-lpthread"])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-lamdhip64", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 
