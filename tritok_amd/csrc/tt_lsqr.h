@@ -15,24 +15,33 @@
 
 namespace tt {
 
-// LSQR scalar state, shared by the kernels of one solve (device memory).
+// LSQR scalar state (device memory, two copies): the cell kernel of
+// iteration k reads copy (k-1)&1 and its block 0 writes copy k&1; the
+// unknown kernel of iteration k reads copy k&1 and its block 0 stores
+// beta_{k+1} there for the next cell kernel.  No kernel writes a copy that
+// another block of the same launch reads.
 struct LsqrState {
-  double alpha, beta;                 // current alpha_k, beta_{k+1}
+  double alpha, beta;                 // alpha_k and beta_k (beta_{k+1} after the unknown kernel)
   double rhobar, phibar, anorm, dnorm, xnorm, xnorm1, cs2, sn2, z, res2, rnorm, arnorm, bnorm, acond;
   double t1, t2, t3;                  // update coefficients of the last rotation
   double atol, btol, ctol, nrm_b, nrm_r;
   double pending_phi, pending_rho, pending_theta, pending_tau;  // rotation awaiting its test
   int itn, istop, done, itnlim;
-  unsigned counter;                   // last-block counter of the current kernel
-  int pad;
 };
 
+// Block storage: per cell an R x L row-major block, padded to `stride`
+// elements (a multiple of 4) so that every block starts on a 16-byte boundary
+// in both precisions and is read with 16-byte vector loads.
+inline int block_stride(int R, int L) { return (R * L + 3) / 4 * 4; }
+
 struct BlockSys {
-  int ncell = 0, R = 0, L = 0, n = 0;
-  DBuf<double> A;         // scaled blocks, row-major per cell: ncell*R*L
+  int ncell = 0, R = 0, L = 0, n = 0, stride = 0;
+  DBuf<double> A;         // scaled blocks: ncell*stride
   DBuf<float> A32;        // the same in single precision (mixed-precision solves)
   DBuf<int> colmap;       // ncell*L
   DBuf<int> tptr, tidx;   // for unknown j: entries tidx[tptr[j]..tptr[j+1]) = cell*L + l
+  int gw = 0;             // gather width: max cells per unknown, rounded up to 4
+  DBuf<int> gtab;         // n*gw: the entries of tidx per unknown in cell order, -1 padded
   DBuf<double> q;         // column scaling
   bool has32 = false;
   int last_nit = 0;       // iterations of the previous solve (sizes the first launch batch)
@@ -46,8 +55,10 @@ void block_setup(BlockSys &s, int ncell, int R, int L, int n, const std::vector<
 
 struct LsqrWork {
   int m = 0, n = 0, ncell = 0, L = 0;
-  DBuf<double> u, v, w, x, P, part1, part2, part3;
-  DBuf<LsqrState> state;
+  DBuf<double> u, v, w, x, P;
+  DBuf<double> pa1, pa2;    // per-cell partial sums of the cell kernels
+  DBuf<double> pb1, pb2;    // per-block partial sums of the unknown kernel
+  DBuf<LsqrState> state;    // two copies
   LsqrState *host_state = nullptr;  // pinned
   void ensure(int m, int n, int ncell, int L);
   ~LsqrWork();

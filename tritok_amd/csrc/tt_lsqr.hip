@@ -1,17 +1,27 @@
 // tt_lsqr.hip -- LSQR on the device for the block-structured subdomain systems.
 //
-// Per iteration two kernels, no host synchronisation:
-//   lsqr_A (one wavefront per cell): u' <- A v - alpha u, ||u'||^2 partial,
-//           P <- A^T u' per cell (the block is read once, from LDS, for both);
-//           the last block to finish reduces ||u'||^2 -> beta.
-//   lsqr_B (one thread per unknown): finishes the x/w update of the previous
-//           iteration, gathers A^T u' from the cells of its column,
-//           v' <- A^T u / beta - beta v, ||v'||^2 partial; the last block
-//           reduces alpha, runs the previous iteration's stopping tests and
-//           the new plane rotation.
-// Every reduction uses a fixed assignment of terms to threads, so a solve is
-// bit-for-bit repeatable.  The scalar recurrences are those of the reference
-// LSQR (lib_Solve_v11.f:3283-3420) with damp = 0.
+// Per iteration two kernels, no host synchronisation, no atomics and no
+// device-scope fences:
+//   lsqr_A (one wavefront per cell): reduces the unknown kernel's partial
+//           sums (the same terms in the same order in every block, so every
+//           block obtains the same alpha), runs the scalar recurrences
+//           (stopping tests of iteration k-2, plane rotation of iteration
+//           k-1), then u' <- A v - alpha u, ||u'||^2 partial and P <- A^T u'
+//           for its cell.  The block is read once from HBM with 16-byte loads
+//           that are all in flight together, staged in LDS and used for both
+//           products.  Block 0 stores the updated scalars.
+//   lsqr_B (one thread per unknown): reduces the cell partials to beta,
+//           finishes the x/w update of the previous iteration, gathers
+//           A^T u' from the cells of its column (fixed-width index table,
+//           loads issued together), v' <- A^T u / beta - beta v, and writes
+//           the partial sums of ||v'||^2 and of the step norm.
+// Results are therefore bit-for-bit repeatable.  The scalar recurrences are
+// those of the reference LSQR (lib_Solve_v11.f:3283-3420) with damp = 0; they
+// run one kernel later than in a sequential code, so termination at
+// iteration k is seen by the cell kernel of iteration k+2, after the unknown
+// kernel of iteration k+1 has applied the x update of iteration k and
+// nothing more.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -28,40 +38,68 @@ __device__ __forceinline__ double d2norm(double a, double b) {
   return scale * sqrt(sa * sa + sb * sb);
 }
 
-// deterministic block-wide sum (blockDim power of two, <= 256)
+// sum over the 64 lanes of a wavefront (butterfly; every lane gets the total)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// block of NT threads (multiple of 64): wave sums combined in wave order
 template <int NT>
 __device__ double block_sum(double v, double *sh) {
-  sh[threadIdx.x] = v;
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
   __syncthreads();
+  double r = 0.0;
 #pragma unroll
-  for (int o = NT / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  double r = sh[0];
+  for (int i = 0; i < NT / 64; ++i) r += sh[i];
   __syncthreads();
   return r;
 }
 
-// Signals block completion; returns true in the last block (after an acquire
-// fence, so the other blocks' partials are visible).
-__device__ bool last_block(unsigned *counter, bool *flag) {
-  __threadfence();
-  if (threadIdx.x == 0) {
-    unsigned t = atomicAdd(counter, 1u);
-    *flag = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  bool last = *flag;
-  if (last) __threadfence();
-  return last;
-}
-
+// sum of p[0..n) with lane i taking i, i+NT, ...; deterministic
 template <int NT>
 __device__ double reduce_partials(const double *p, int n, double *sh) {
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += NT) acc += p[i];
-  return block_sum<NT>(acc, sh);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * NT < n; i += 4 * NT) {
+    a0 += p[i];
+    a1 += p[i + NT];
+    a2 += p[i + 2 * NT];
+    a3 += p[i + 3 * NT];
+  }
+  for (; i < n; i += NT) a0 += p[i];
+  return block_sum<NT>((a0 + a1) + (a2 + a3), sh);
+}
+
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<double> {
+  using type = double2;
+};
+template <>
+struct Vec16<float> {
+  using type = float4;
+};
+
+// Loads one padded block into LDS: all 16-byte loads of a lane are issued
+// before the first LDS store.
+template <typename T, int S>
+__device__ __forceinline__ void stage_block(const T *__restrict__ src, T *blk) {
+  using V = typename Vec16<T>::type;
+  constexpr int NV = S * (int)sizeof(T) / 16, PER = (NV + 63) / 64;
+  const V *s = reinterpret_cast<const V *>(src);
+  V *d = reinterpret_cast<V *>(blk);
+  V reg[PER];
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < PER; ++i) reg[i] = (t + 64 * i < NV) ? s[t + 64 * i] : V{};
+#pragma unroll
+  for (int i = 0; i < PER; ++i)
+    if (t + 64 * i < NV) d[t + 64 * i] = reg[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -79,43 +117,55 @@ __global__ void colnorm_kernel(int n, int R, int L, const int *__restrict__ tptr
   q[j] = 1.0 / sqrt(s);
 }
 
-__global__ void scale_kernel(size_t total, int R, int L, const int *__restrict__ colmap, const double *__restrict__ q,
-                             const double *__restrict__ Afull, double *__restrict__ A, float *__restrict__ A32) {
+// scaled, padded blocks from the unpadded full blocks (known columns -> 0)
+__global__ void scale_kernel(size_t total, int R, int L, int S, const int *__restrict__ colmap,
+                             const double *__restrict__ q, const double *__restrict__ Afull, double *__restrict__ A,
+                             float *__restrict__ A32) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= total) return;
-  size_t c = k / ((size_t)R * L);
-  int l = (int)(k % L);
-  int j = colmap[c * L + l];
-  double v = j >= 0 ? Afull[k] * q[j] : 0.0;
+  size_t c = k / S;
+  int e = (int)(k % S);
+  double v = 0.0;
+  if (e < R * L) {
+    int l = e % L;
+    int j = colmap[c * L + l];
+    if (j >= 0) v = Afull[c * R * L + e] * q[j];
+  }
   A[k] = v;
   if (A32) A32[k] = (float)v;
 }
 
+__global__ void to32_kernel(size_t n, const double *__restrict__ a, float *__restrict__ b) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) b[k] = (float)a[k];
+}
+
+constexpr int kNB = 1024;  // threads per block of the unknown kernel
+
 // ---------------------------------------------------------------------------
-// r0 = b - A (x0/q), P = A^T r0; last block: norms and tolerances
-template <typename T>
-__global__ void __launch_bounds__(64) lsqr_r0(int R, int L, const T *__restrict__ A, const int *__restrict__ colmap,
+// r0 = b - A (x0/q), P = A^T r0; per-cell partials of ||r0||^2 and ||b||^2
+template <int R, int L>
+__global__ void __launch_bounds__(64) lsqr_r0(const double *__restrict__ A, const int *__restrict__ colmap,
                                               const double *__restrict__ q, const double *__restrict__ x0,
                                               const double *__restrict__ b, double *__restrict__ u,
-                                              double *__restrict__ P, double *__restrict__ part1,
-                                              double *__restrict__ part2, LsqrState *st) {
-  extern __shared__ unsigned char lds_raw[];
-  T *blk = reinterpret_cast<T *>(lds_raw);
-  __shared__ double xs[64], rr[64], sh[64];
-  __shared__ bool flag;
+                                              double *__restrict__ P, double *__restrict__ pa1,
+                                              double *__restrict__ pa2) {
+  constexpr int S = (R * L + 3) / 4 * 4;
+  __shared__ __align__(16) double blk[S];
+  __shared__ double xs[64], rr[64];
   const int c = blockIdx.x, t = threadIdx.x;
-  const T *Ab = A + (size_t)c * R * L;
-  for (int k = t; k < R * L; k += 64) blk[k] = Ab[k];
+  stage_block<double, S>(A + (size_t)c * S, blk);
   if (t < L) {
     int j = colmap[(size_t)c * L + t];
     xs[t] = j >= 0 ? x0[j] / q[j] : 0.0;
   }
+  const double bb = t < R ? b[(size_t)c * R + t] : 0.0;
   __syncthreads();
-  double r = 0.0, bb = 0.0;
+  double r = 0.0;
   if (t < R) {
     double s = 0.0;
-    for (int l = 0; l < L; ++l) s += (double)blk[t * L + l] * xs[l];
-    bb = b[(size_t)c * R + t];
+#pragma unroll
+    for (int l = 0; l < L; ++l) s += blk[t * L + l] * xs[l];
     r = bb - s;
     u[(size_t)c * R + t] = r;
     rr[t] = r;
@@ -123,136 +173,41 @@ __global__ void __launch_bounds__(64) lsqr_r0(int R, int L, const T *__restrict_
   __syncthreads();
   if (t < L) {
     double s = 0.0;
-    for (int i = 0; i < R; ++i) s += (double)blk[i * L + t] * rr[i];
+#pragma unroll
+    for (int i = 0; i < R; ++i) s += blk[i * L + t] * rr[i];
     P[(size_t)c * L + t] = s;
   }
-  double s1 = block_sum<64>(r * r, sh), s2 = block_sum<64>(bb * bb, sh);
-  if (t == 0) part1[c] = s1, part2[c] = s2;
-  if (!last_block(&st->counter, &flag)) return;
-  double nr = sqrt(reduce_partials<64>(part1, gridDim.x, sh));
-  double nb = sqrt(reduce_partials<64>(part2, gridDim.x, sh));
-  if (t == 0) {
-    st->counter = 0;
-    st->nrm_r = nr, st->nrm_b = nb;
-    st->btol = st->atol * nb / nr;
-    st->beta = nr;
-  }
+  const double s1 = wave_sum(r * r), s2 = wave_sum(bb * bb);
+  if (t == 0) pa1[c] = s1, pa2[c] = s2;
 }
 
-// ---------------------------------------------------------------------------
-// one iteration, cell part: u' = A v - alpha u ; P = A^T u'
-template <typename T>
-__global__ void __launch_bounds__(64) lsqr_A(int R, int L, const T *__restrict__ A, const int *__restrict__ colmap,
-                                             const double *__restrict__ vp, double *__restrict__ u,
-                                             double *__restrict__ P, double *__restrict__ part1, LsqrState *st) {
-  if (st->done) return;
-  extern __shared__ unsigned char lds_raw[];
-  T *blk = reinterpret_cast<T *>(lds_raw);
-  __shared__ double vs[64], uu[64], sh[64];
-  __shared__ bool flag;
-  const int c = blockIdx.x, t = threadIdx.x;
-  const double alpha = st->alpha, beta = st->beta;
-  const T *Ab = A + (size_t)c * R * L;
-  for (int k = t; k < R * L; k += 64) blk[k] = Ab[k];
-  if (t < L) {
-    int j = colmap[(size_t)c * L + t];
-    vs[t] = j >= 0 ? (alpha > 0.0 ? vp[j] / alpha : vp[j]) : 0.0;
-  }
-  __syncthreads();
-  double un = 0.0;
-  if (t < R) {
-    double s = 0.0;
-    for (int l = 0; l < L; ++l) s += (double)blk[t * L + l] * vs[l];
-    double uo = u[(size_t)c * R + t];
-    if (beta > 0.0) uo = uo / beta;
-    un = -alpha * uo + s;
-    u[(size_t)c * R + t] = un;
-    uu[t] = un;
-  }
-  __syncthreads();
-  if (t < L) {
-    double s = 0.0;
-    for (int i = 0; i < R; ++i) s += (double)blk[i * L + t] * uu[i];
-    P[(size_t)c * L + t] = s;
-  }
-  double s1 = block_sum<64>(un * un, sh);
-  if (t == 0) part1[c] = s1;
-  if (!last_block(&st->counter, &flag)) return;
-  double nb = sqrt(reduce_partials<64>(part1, gridDim.x, sh));
-  if (t == 0) {
-    st->counter = 0;
-    st->beta = nb;  // beta_{k+1}
-  }
-}
-
-// ---------------------------------------------------------------------------
-// one iteration, unknown part.  k = iteration number (0: initialisation)
-__global__ void __launch_bounds__(256) lsqr_B(int n, int k, const int *__restrict__ tptr, const int *__restrict__ tidx,
-                                              const double *__restrict__ P, double *__restrict__ vp,
-                                              double *__restrict__ w, double *__restrict__ x,
-                                              double *__restrict__ part1, double *__restrict__ part2, LsqrState *st) {
-  if (st->done) return;
-  __shared__ double sh[256];
-  __shared__ bool flag;
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  const double alpha = st->alpha, beta = st->beta;
-  const double t1 = st->t1, t2 = st->t2, t3 = st->t3;
-  double nv = 0.0, dk = 0.0;
-  if (j < n) {
-    double vk = 0.0;
-    if (k >= 1) {
-      vk = alpha > 0.0 ? vp[j] / alpha : vp[j];
-      if (k == 1) {
-        w[j] = vk;
-        x[j] = 0.0;
-      } else {
-        double wo = w[j];
-        x[j] = t1 * wo + x[j];
-        w[j] = t2 * wo + vk;
-        double tt = t3 * wo;
-        dk = tt * tt;
-      }
-    }
-    double s = 0.0;
-    for (int e = tptr[j]; e < tptr[j + 1]; ++e) s += P[tidx[e]];
-    double nvp;
-    if (beta > 0.0)
-      nvp = s / beta - beta * vk;
-    else
-      nvp = k == 0 ? 0.0 : vp[j];  // the reference keeps v and alpha when beta = 0
-    vp[j] = nvp;
-    nv = nvp * nvp;
-  }
-  double s1 = block_sum<256>(nv, sh), s2 = block_sum<256>(dk, sh);
-  if (threadIdx.x == 0) part1[blockIdx.x] = s1, part2[blockIdx.x] = s2;
-  if (!last_block(&st->counter, &flag)) return;
-  double nvs = reduce_partials<256>(part1, gridDim.x, sh);
-  double dks = reduce_partials<256>(part2, gridDim.x, sh);
-  if (threadIdx.x != 0) return;
-  st->counter = 0;
-  LsqrState &S = *st;
-  if (k == 0) {
-    // initialisation: alpha_1, and the reference's exit when A^T b = 0
-    double a1 = sqrt(nvs);
+// Scalar recurrences run by the cell kernel of iteration k (k >= 1) from the
+// state after iteration k-1's cell kernel (in) and the unknown kernel's sums:
+// nvs = ||v_k||^2 (unnormalised), dks = ||t3 w||^2 of iteration k-2's update.
+__device__ void scalar_update(int k, const LsqrState &in, double nvs, double dks, LsqrState &S) {
+  S = in;
+  if (in.done) return;
+  if (k == 1) {
+    // initialisation: alpha_1 and the exit when A^T b = 0 (phibar, bnorm and
+    // rnorm were set to beta_1 by the unknown kernel of iteration 0)
+    const double a1 = sqrt(nvs);
     S.alpha = a1;
     S.rhobar = a1;
-    S.phibar = S.beta;
-    S.bnorm = S.beta;
-    S.rnorm = S.beta;
-    S.arnorm = a1 * S.beta;
+    S.arnorm = a1 * in.beta;
     if (S.arnorm == 0.0) S.done = 1, S.istop = 0, S.itn = 0;
     return;
   }
-  if (k >= 2) {
-    // finish iteration k-1 (whose x/w update this kernel applied)
-    double dknorm = sqrt(dks);
+  const double alpha = in.alpha, beta = in.beta;  // alpha_{k-1}, beta_k
+  if (k >= 3) {
+    // finish iteration k-2 (whose x/w update the unknown kernel of iteration k-1 applied)
+    const double dknorm = sqrt(dks);
     S.dnorm = d2norm(S.dnorm, dknorm);
-    double rho = S.pending_rho, phi = S.pending_phi, theta = S.pending_theta, tau = S.pending_tau;
-    double delta = S.sn2 * rho, gambar = -S.cs2 * rho;
-    double rhs = phi - delta * S.z;
-    double zbar = rhs / gambar;
+    const double rho = S.pending_rho, phi = S.pending_phi, theta = S.pending_theta, tau = S.pending_tau;
+    const double delta = S.sn2 * rho, gambar = -S.cs2 * rho;
+    const double rhs = phi - delta * S.z;
+    const double zbar = rhs / gambar;
     S.xnorm = d2norm(S.xnorm1, zbar);
-    double gamma = d2norm(gambar, theta);
+    const double gamma = d2norm(gambar, theta);
     S.cs2 = gambar / gamma;
     S.sn2 = theta / gamma;
     S.z = rhs / gamma;
@@ -261,14 +216,16 @@ __global__ void __launch_bounds__(256) lsqr_B(int n, int k, const int *__restric
     S.res2 = d2norm(S.res2, 0.0);
     S.rnorm = d2norm(S.res2, S.phibar);
     S.arnorm = alpha * fabs(tau);
-    double test1 = S.rnorm / S.bnorm, test2 = 0.0;
+    const double test1 = S.rnorm / S.bnorm;
+    double test2 = 0.0;
     if (S.rnorm > 0.0) test2 = S.arnorm / (S.anorm * S.rnorm);
-    double test3 = 1.0 / S.acond;
+    const double test3 = 1.0 / S.acond;
     double tt1 = test1 / (1.0 + S.anorm * S.xnorm / S.bnorm);
-    double rtol = S.btol + S.atol * S.anorm * S.xnorm / S.bnorm;
-    double tt3 = 1.0 + test3, tt2 = 1.0 + test2;
+    const double rtol = S.btol + S.atol * S.anorm * S.xnorm / S.bnorm;
+    const double tt3 = 1.0 + test3, tt2 = 1.0 + test2;
     tt1 = 1.0 + tt1;
-    int istop = 0, itn = k - 1;
+    int istop = 0;
+    const int itn = k - 2;
     if (itn >= S.itnlim) istop = 5;
     if (tt3 <= 1.0) istop = 4;
     if (tt2 <= 1.0) istop = 2;
@@ -281,30 +238,154 @@ __global__ void __launch_bounds__(256) lsqr_B(int n, int k, const int *__restric
       return;
     }
   }
-  // iteration k: anorm with alpha_k, beta_{k+1}; then alpha_{k+1} and the rotation
-  double anew = beta > 0.0 ? sqrt(nvs) : alpha;
+  // rest of iteration k-1: anorm with alpha_{k-1}, beta_k; alpha_k; plane rotation
+  const double anew = beta > 0.0 ? sqrt(nvs) : alpha;  // the reference keeps alpha when beta = 0
   double temp = d2norm(alpha, beta);
   temp = d2norm(temp, 0.0);
   S.anorm = d2norm(S.anorm, temp);
-  double rhbar1 = S.rhobar;
-  double rho = d2norm(rhbar1, beta);
-  double cs = rhbar1 / rho, sn = beta / rho;
-  double theta = sn * anew;
+  const double rhbar1 = S.rhobar;
+  const double rho = d2norm(rhbar1, beta);
+  const double cs = rhbar1 / rho, sn = beta / rho;
+  const double theta = sn * anew;
   S.rhobar = -cs * anew;
-  double phi = cs * S.phibar;
+  const double phi = cs * S.phibar;
   S.phibar = sn * S.phibar;
-  double tau = sn * phi;
+  const double tau = sn * phi;
   S.t1 = phi / rho;
   S.t2 = -theta / rho;
   S.t3 = 1.0 / rho;
   S.pending_rho = rho, S.pending_phi = phi, S.pending_theta = theta, S.pending_tau = tau;
   S.alpha = anew;
-  S.itn = k;
+  S.itn = k - 1;
 }
 
-__global__ void to32_kernel(size_t n, const double *__restrict__ a, float *__restrict__ b) {
-  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) b[k] = (float)a[k];
+// ---------------------------------------------------------------------------
+// iteration k, cell part: scalars, then u' = A v - alpha u ; P = A^T u'
+template <typename T, int R, int L>
+__global__ void __launch_bounds__(64) lsqr_A(int k, const T *__restrict__ A, const int *__restrict__ colmap,
+                                             const double *__restrict__ vp, double *__restrict__ u,
+                                             double *__restrict__ P, double *__restrict__ pa1,
+                                             const double *__restrict__ pb1, const double *__restrict__ pb2, int nb,
+                                             const LsqrState *__restrict__ sin, LsqrState *__restrict__ sout) {
+  constexpr int S = (R * L + 3) / 4 * 4;
+  __shared__ __align__(16) T blk[S];
+  __shared__ double vs[64], uu[64], sh[1];
+  __shared__ double s_alpha;
+  __shared__ int s_done;
+  const int c = blockIdx.x, t = threadIdx.x;
+  if (sin->done) {  // finished earlier: carry the final state forward
+    if (c == 0 && t == 0) *sout = *sin;
+    return;
+  }
+  stage_block<T, S>(A + (size_t)c * S, blk);
+  const int j = t < L ? colmap[(size_t)c * L + t] : -1;
+  const double uo = t < R ? u[(size_t)c * R + t] : 0.0;
+  const double vj = j >= 0 ? vp[j] : 0.0;
+  const double beta = sin->beta;
+  const double nvs = reduce_partials<64>(pb1, nb, sh);
+  const double dks = k >= 3 ? reduce_partials<64>(pb2, nb, sh) : 0.0;
+  if (t == 0) {
+    LsqrState o;
+    scalar_update(k, *sin, nvs, dks, o);
+    s_alpha = o.alpha;
+    s_done = o.done;
+    if (c == 0) *sout = o;
+  }
+  __syncthreads();
+  if (s_done) return;
+  const double alpha = s_alpha;
+  if (t < L) vs[t] = j >= 0 ? (alpha > 0.0 ? vj / alpha : vj) : 0.0;
+  __syncthreads();
+  double un = 0.0;
+  if (t < R) {
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) s += (double)blk[t * L + l] * vs[l];
+    const double uold = beta > 0.0 ? uo / beta : uo;
+    un = -alpha * uold + s;
+    u[(size_t)c * R + t] = un;
+    uu[t] = un;
+  }
+  __syncthreads();
+  if (t < L) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < R; ++i) s += (double)blk[i * L + t] * uu[i];
+    P[(size_t)c * L + t] = s;
+  }
+  const double s1 = wave_sum(un * un);
+  if (t == 0) pa1[c] = s1;
+}
+
+// ---------------------------------------------------------------------------
+// iteration k, unknown part (k = 0: initialisation after r0)
+template <int W>
+__global__ void __launch_bounds__(kNB) lsqr_B(int n, int k, const int *__restrict__ gtab, const double *__restrict__ P,
+                                              double *__restrict__ vp, double *__restrict__ w,
+                                              double *__restrict__ x, const double *__restrict__ pa1,
+                                              const double *__restrict__ pa2, int na, double *__restrict__ pb1,
+                                              double *__restrict__ pb2, LsqrState *st) {
+  __shared__ double sh[kNB / 64];
+  if (k >= 1 && st->done) return;
+  const int j = blockIdx.x * kNB + threadIdx.x;
+  int idx[W];
+  if (j < n) {
+    const int4 *g = reinterpret_cast<const int4 *>(gtab + (size_t)j * W);
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+      int4 v = g[q];
+      idx[4 * q] = v.x, idx[4 * q + 1] = v.y, idx[4 * q + 2] = v.z, idx[4 * q + 3] = v.w;
+    }
+  }
+  const double beta = sqrt(reduce_partials<kNB>(pa1, na, sh));  // beta_{k+1}
+  const double alpha = st->alpha, t1 = st->t1, t2 = st->t2, t3 = st->t3;
+  double nv = 0.0, dk = 0.0;
+  if (j < n) {
+    double pv[W];
+#pragma unroll
+    for (int e = 0; e < W; ++e) pv[e] = idx[e] >= 0 ? P[idx[e]] : 0.0;
+    const double vold = vp[j];
+    double vk = 0.0;
+    if (k >= 1) {
+      vk = alpha > 0.0 ? vold / alpha : vold;
+      if (k == 1) {
+        w[j] = vk;
+        x[j] = 0.0;
+      } else {
+        // x/w update of iteration k-1 (rotation computed by this iteration's cell kernel)
+        const double wo = w[j];
+        x[j] = t1 * wo + x[j];
+        w[j] = t2 * wo + vk;
+        const double tt = t3 * wo;
+        dk = tt * tt;
+      }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < W; ++e)
+      if (idx[e] >= 0) s += pv[e];
+    double nvp;
+    if (beta > 0.0)
+      nvp = s / beta - beta * vk;
+    else
+      nvp = k == 0 ? 0.0 : vold;  // the reference keeps v and alpha when beta = 0
+    vp[j] = nvp;
+    nv = nvp * nvp;
+  }
+  const double s1 = block_sum<kNB>(nv, sh), s2 = block_sum<kNB>(dk, sh);
+  if (threadIdx.x == 0) pb1[blockIdx.x] = s1, pb2[blockIdx.x] = s2;
+  if (blockIdx.x == 0) {
+    double nbv = 0.0;
+    if (k == 0) nbv = sqrt(reduce_partials<kNB>(pa2, na, sh));
+    if (threadIdx.x == 0) {
+      st->beta = beta;
+      if (k == 0) {
+        st->nrm_r = beta, st->nrm_b = nbv;
+        st->btol = st->atol * nbv / beta;
+        st->phibar = beta, st->bnorm = beta, st->rnorm = beta;
+      }
+    }
+  }
 }
 
 __global__ void finish_kernel(int n, const double *__restrict__ q, const double *__restrict__ dx,
@@ -314,54 +395,52 @@ __global__ void finish_kernel(int n, const double *__restrict__ q, const double 
   x[j] = q[j] * dx[j] + x0[j];
 }
 
-}  // namespace
-
-// ---------------------------------------------------------------------------
-void block_setup(BlockSys &s, int ncell, int R, int L, int n, const std::vector<int> &colmap, const double *Afull,
-                 bool want32, hipStream_t st) {
-  if (R > 64 || L > 64) throw std::runtime_error("block LSQR supports at most 64x64 blocks");
-  s.ncell = ncell, s.R = R, s.L = L, s.n = n;
-  // transpose gather lists in cell order
-  std::vector<int> cnt(n + 1, 0), idx;
-  for (size_t k = 0; k < colmap.size(); ++k)
-    if (colmap[k] >= 0) cnt[colmap[k] + 1]++;
-  for (int j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
-  for (int j = 0; j < n; ++j)
-    if (cnt[j + 1] == cnt[j]) throw std::runtime_error("an unknown appears in no equation");
-  idx.resize(cnt[n]);
-  std::vector<int> pos(cnt.begin(), cnt.end() - 1);
-  for (size_t k = 0; k < colmap.size(); ++k)
-    if (colmap[k] >= 0) idx[pos[colmap[k]]++] = (int)k;
-  s.colmap.upload(colmap, st);
-  s.tptr.upload(cnt, st);
-  s.tidx.upload(idx, st);
-  s.q.alloc(n);
-  size_t total = (size_t)ncell * R * L;
-  s.A.alloc(total);
-  s.has32 = want32;
-  if (want32) s.A32.alloc(total);
-  hipLaunchKernelGGL(colnorm_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, R, L, s.tptr.p, s.tidx.p, Afull,
-                     s.q.p);
-  TT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(scale_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, st, total, R, L, s.colmap.p, s.q.p,
-                     Afull, s.A.p, want32 ? s.A32.p : nullptr);
-  TT_LAUNCH_CHECK();
+// launch helpers over the supported block shapes and gather widths
+template <int R, int L>
+void launch_r0(BlockSys &s, LsqrWork &wk, const double *b, const double *x, hipStream_t st) {
+  hipLaunchKernelGGL((lsqr_r0<R, L>), dim3(s.ncell), dim3(64), 0, st, s.A.p, s.colmap.p, s.q.p, x, b, wk.u.p, wk.P.p,
+                     wk.pa1.p, wk.pa2.p);
+}
+template <typename T, int R, int L>
+void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st) {
+  hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(s.ncell), dim3(64), 0, st, k, A, s.colmap.p, wk.v.p, wk.u.p, wk.P.p,
+                     wk.pa1.p, wk.pb1.p, wk.pb2.p, (int)ceil_div(s.n, kNB), wk.state.p + ((k - 1) & 1),
+                     wk.state.p + (k & 1));
+}
+template <int W>
+void launch_B(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
+  hipLaunchKernelGGL((lsqr_B<W>), dim3(ceil_div(s.n, kNB)), dim3(kNB), 0, st, s.n, k, s.gtab.p, wk.P.p, wk.v.p,
+                     wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, s.ncell, wk.pb1.p, wk.pb2.p, wk.state.p + (k & 1));
 }
 
-void block_make32(BlockSys &s, hipStream_t st) {
-  if (s.has32) return;
-  size_t total = (size_t)s.ncell * s.R * s.L;
-  s.A32.alloc(total);
-  hipLaunchKernelGGL(to32_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, st, total, s.A.p, s.A32.p);
-  TT_LAUNCH_CHECK();
-  s.has32 = true;
+void r0(BlockSys &s, LsqrWork &wk, const double *b, const double *x, hipStream_t st) {
+  if (s.R == 51)
+    launch_r0<51, 51>(s, wk, b, x, st);
+  else
+    launch_r0<27, 27>(s, wk, b, x, st);
+}
+void stepA(BlockSys &s, LsqrWork &wk, int k, bool use32, hipStream_t st) {
+  if (s.R == 51) {
+    if (use32)
+      launch_A<float, 51, 51>(s, wk, k, s.A32.p, st);
+    else
+      launch_A<double, 51, 51>(s, wk, k, s.A.p, st);
+  } else {
+    if (use32)
+      launch_A<float, 27, 27>(s, wk, k, s.A32.p, st);
+    else
+      launch_A<double, 27, 27>(s, wk, k, s.A.p, st);
+  }
+}
+void stepB(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
+  switch (s.gw) {
+    case 4: launch_B<4>(s, wk, k, st); break;
+    case 8: launch_B<8>(s, wk, k, st); break;
+    case 12: launch_B<12>(s, wk, k, st); break;
+    default: launch_B<16>(s, wk, k, st); break;
+  }
 }
 
-LsqrProbe::~LsqrProbe() {
-  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
-}
-
-namespace {
 // records the start event of a sampled launch; returns the pair index or -1
 long probe_begin(LsqrProbe *p, hipStream_t st) {
   if (!p || p->stride <= 0) return -1;
@@ -392,17 +471,75 @@ void probe_collect(LsqrProbe *p) {
   }
   p->used = 0;
 }
+
 }  // namespace
+
+// ---------------------------------------------------------------------------
+void block_setup(BlockSys &s, int ncell, int R, int L, int n, const std::vector<int> &colmap, const double *Afull,
+                 bool want32, hipStream_t st) {
+  if (!((R == 51 && L == 51) || (R == 27 && L == 27)))
+    throw std::runtime_error("block LSQR supports 51x51 and 27x27 cell blocks");
+  s.ncell = ncell, s.R = R, s.L = L, s.n = n, s.stride = block_stride(R, L);
+  // transpose gather lists in cell order
+  std::vector<int> cnt(n + 1, 0), idx;
+  for (size_t k = 0; k < colmap.size(); ++k)
+    if (colmap[k] >= 0) cnt[colmap[k] + 1]++;
+  int wmax = 0;
+  for (int j = 0; j < n; ++j) {
+    if (cnt[j + 1] == 0) throw std::runtime_error("an unknown appears in no equation");
+    wmax = std::max(wmax, cnt[j + 1]);
+  }
+  if (wmax > 16) throw std::runtime_error("an unknown is shared by more than 16 cells");
+  for (int j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
+  idx.resize(cnt[n]);
+  std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+  for (size_t k = 0; k < colmap.size(); ++k)
+    if (colmap[k] >= 0) idx[pos[colmap[k]]++] = (int)k;
+  s.gw = (wmax + 3) / 4 * 4;
+  std::vector<int> g((size_t)n * s.gw, -1);
+  for (int j = 0; j < n; ++j)
+    for (int e = cnt[j]; e < cnt[j + 1]; ++e) g[(size_t)j * s.gw + (e - cnt[j])] = idx[e];
+  s.colmap.upload(colmap, st);
+  s.tptr.upload(cnt, st);
+  s.tidx.upload(idx, st);
+  s.gtab.upload(g, st);
+  s.q.alloc(n);
+  size_t total = (size_t)ncell * s.stride;
+  s.A.alloc(total);
+  s.has32 = want32;
+  if (want32) s.A32.alloc(total);
+  hipLaunchKernelGGL(colnorm_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, R, L, s.tptr.p, s.tidx.p, Afull,
+                     s.q.p);
+  TT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scale_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, st, total, R, L, s.stride, s.colmap.p,
+                     s.q.p, Afull, s.A.p, want32 ? s.A32.p : nullptr);
+  TT_LAUNCH_CHECK();
+  TT_HIP(hipStreamSynchronize(st));  // host vectors above are released on return
+}
+
+void block_make32(BlockSys &s, hipStream_t st) {
+  if (s.has32) return;
+  size_t total = (size_t)s.ncell * s.stride;
+  s.A32.alloc(total);
+  hipLaunchKernelGGL(to32_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, st, total, s.A.p, s.A32.p);
+  TT_LAUNCH_CHECK();
+  s.has32 = true;
+}
+
+LsqrProbe::~LsqrProbe() {
+  for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
 
 void LsqrWork::ensure(int m_, int n_, int ncell_, int L_) {
   if (m_ > m) u.alloc(m_), m = m_;
   if (n_ > n) v.alloc(n_), w.alloc(n_), x.alloc(n_), n = n_;
   if ((size_t)ncell_ * L_ > P.n) P.alloc((size_t)ncell_ * L_);
-  size_t np = std::max<size_t>((size_t)ncell_, ceil_div(n_, 256)) + 1;
-  if (np > part1.n) part1.alloc(np), part2.alloc(np), part3.alloc(np);
+  if ((size_t)ncell_ > pa1.n) pa1.alloc(ncell_), pa2.alloc(ncell_);
+  const size_t nb = ceil_div(n_, kNB);
+  if (nb > pb1.n) pb1.alloc(nb), pb2.alloc(nb);
   ncell = std::max(ncell, ncell_), L = std::max(L, L_);
   if (!state.p) {
-    state.alloc(1);
+    state.alloc(2);
     TT_HIP(hipHostMalloc((void **)&host_state, sizeof(LsqrState), hipHostMallocDefault));
   }
 }
@@ -422,35 +559,27 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
   init.atol = (double)eps4;
   init.ctol = 1.0 / 1.0e8;
   init.itnlim = maxit == 0 ? 10 * (m + n + 50) : maxit;
-  TT_HIP(hipMemcpyAsync(wk.state.p, &init, sizeof(init), hipMemcpyHostToDevice, st));
-  const unsigned gc = s.ncell, gn = ceil_div(n, 256);
+  LsqrState init2[2] = {init, init};
+  TT_HIP(hipMemcpyAsync(wk.state.p, init2, sizeof(init2), hipMemcpyHostToDevice, st));
   TT_HIP(hipMemsetAsync(wk.x.p, 0, sizeof(double) * n, st));
-  const size_t lds64 = sizeof(double) * s.R * s.L, lds32 = sizeof(float) * s.R * s.L;
-  hipLaunchKernelGGL(lsqr_r0<double>, dim3(gc), dim3(64), lds64, st, s.R, s.L, s.A.p, s.colmap.p, s.q.p, x, b, wk.u.p,
-                     wk.P.p, wk.part1.p, wk.part2.p, wk.state.p);
+  r0(s, wk, b, x, st);
   TT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(lsqr_B, dim3(gn), dim3(256), 0, st, n, 0, s.tptr.p, s.tidx.p, wk.P.p, wk.v.p, wk.w.p, wk.x.p,
-                     wk.part1.p, wk.part2.p, wk.state.p);
+  stepB(s, wk, 0, st);
   TT_LAUNCH_CHECK();
   // the previous solve's count predicts this one's (the Picard loop changes
-  // only the right-hand side): termination is seen after nit + 1 launch pairs
-  int k = 1, batch = s.last_nit > 0 ? s.last_nit + 1 : 4;
+  // only the right-hand side): termination after nit iterations is seen by
+  // the cell kernel of iteration nit + 2
+  int k = 1, batch = s.last_nit > 0 ? s.last_nit + 2 : 4;
   LsqrResult res;
   for (;;) {
     for (int b2 = 0; b2 < batch; ++b2, ++k) {
       long pi = probe_begin(probe, st);
-      if (use32)
-        hipLaunchKernelGGL(lsqr_A<float>, dim3(gc), dim3(64), lds32, st, s.R, s.L, s.A32.p, s.colmap.p, wk.v.p, wk.u.p,
-                           wk.P.p, wk.part1.p, wk.state.p);
-      else
-        hipLaunchKernelGGL(lsqr_A<double>, dim3(gc), dim3(64), lds64, st, s.R, s.L, s.A.p, s.colmap.p, wk.v.p, wk.u.p,
-                           wk.P.p, wk.part1.p, wk.state.p);
+      stepA(s, wk, k, use32, st);
       probe_end(probe, pi, st);
-      hipLaunchKernelGGL(lsqr_B, dim3(gn), dim3(256), 0, st, n, k, s.tptr.p, s.tidx.p, wk.P.p, wk.v.p, wk.w.p, wk.x.p,
-                         wk.part1.p, wk.part2.p, wk.state.p);
+      stepB(s, wk, k, st);
     }
     TT_LAUNCH_CHECK();
-    TT_HIP(hipMemcpyAsync(wk.host_state, wk.state.p, sizeof(LsqrState), hipMemcpyDeviceToHost, st));
+    TT_HIP(hipMemcpyAsync(wk.host_state, wk.state.p + ((k - 1) & 1), sizeof(LsqrState), hipMemcpyDeviceToHost, st));
     TT_HIP(hipStreamSynchronize(st));
     probe_collect(probe);
     if (wk.host_state->done) {
@@ -458,7 +587,7 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
       res.istop = wk.host_state->istop;
       break;
     }
-    if (k > init.itnlim + 2) throw std::runtime_error("LSQR did not signal termination");
+    if (k > init.itnlim + 3) throw std::runtime_error("LSQR did not signal termination");
     batch = k < 16 ? 4 : std::min(std::max(4, k / 4), 64);
   }
   s.last_nit = res.nit;
