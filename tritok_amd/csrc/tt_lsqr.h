@@ -45,6 +45,7 @@ struct BlockSys {
   DBuf<double> q;         // column scaling
   bool has32 = false;
   int last_nit = 0;       // iterations of the previous solve (sizes the first launch batch)
+  float last_eps = 0.f;   // its tolerance
 };
 
 // Builds the solver structure from the full unscaled blocks on the device
@@ -56,7 +57,8 @@ void block_setup(BlockSys &s, int ncell, int R, int L, int n, const std::vector<
 struct LsqrWork {
   int m = 0, n = 0, ncell = 0, L = 0;
   DBuf<double> u, v, w, x, P;
-  DBuf<double> pa1, pa2;    // per-cell partial sums of the cell kernels
+  DBuf<double> pa1, pa2;    // partial sums of the cell kernels (per cell for r0, per block after)
+  int na = 0;               // partials written by the last cell kernel
   DBuf<double> pb1, pb2;    // per-block partial sums of the unknown kernel
   DBuf<LsqrState> state;    // two copies
   LsqrState *host_state = nullptr;  // pinned
@@ -72,6 +74,7 @@ struct LsqrProbe {
   unsigned long long sampled = 0;  // launches timed
   double total_ms = 0.0;           // summed duration of the timed launches
   std::vector<hipEvent_t> pool;    // event pool, pairs (start, stop)
+  std::vector<int> iter;           // iteration index of each recorded pair
   size_t used = 0;                 // pairs recorded and not yet read
   ~LsqrProbe();
 };

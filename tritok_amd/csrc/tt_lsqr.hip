@@ -59,19 +59,30 @@ __device__ double block_sum(double v, double *sh) {
   return r;
 }
 
-// sum of p[0..n) with lane i taking i, i+NT, ...; deterministic
+// sum of p[0..n) with lane i taking i, i+NT, ...; 16 loads per lane in
+// flight at a time; deterministic
 template <int NT>
-__device__ double reduce_partials(const double *p, int n, double *sh) {
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int i = threadIdx.x;
-  for (; i + 3 * NT < n; i += 4 * NT) {
-    a0 += p[i];
-    a1 += p[i + NT];
-    a2 += p[i + 2 * NT];
-    a3 += p[i + 3 * NT];
+__device__ double reduce_partials(const double *__restrict__ p, int n, double *sh) {
+  double acc = 0.0;
+  for (int base = 0; base < n; base += 16 * NT) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int i = base + q * NT + (int)threadIdx.x;
+      v[q] = i < n ? p[i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc += v[q];
   }
-  for (; i < n; i += NT) a0 += p[i];
-  return block_sum<NT>((a0 + a1) + (a2 + a3), sh);
+  return block_sum<NT>(acc, sh);
+}
+
+// value of v in lane `lane` of the wavefront, as a scalar (no LDS traffic)
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 template <typename T>
@@ -101,6 +112,28 @@ __device__ __forceinline__ void stage_block(const T *__restrict__ src, T *blk) {
   for (int i = 0; i < PER; ++i)
     if (t + 64 * i < NV) d[t + 64 * i] = reg[i];
 }
+
+// A block held in registers between its HBM load and its LDS store, so that
+// a wavefront streams the next cell's block while it computes on this one.
+template <typename T, int S>
+struct RegBlock {
+  using V = typename Vec16<T>::type;
+  static constexpr int NV = S * (int)sizeof(T) / 16, PER = (NV + 63) / 64;
+  V r[PER];
+  __device__ __forceinline__ void load(const T *__restrict__ src) {
+    const V *s = reinterpret_cast<const V *>(src);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) r[i] = (t + 64 * i < NV) ? s[t + 64 * i] : V{};
+  }
+  __device__ __forceinline__ void store(T *blk) const {
+    V *d = reinterpret_cast<V *>(blk);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      if (t + 64 * i < NV) d[t + 64 * i] = r[i];
+  }
+};
 
 // ---------------------------------------------------------------------------
 // setup kernels
@@ -140,7 +173,7 @@ __global__ void to32_kernel(size_t n, const double *__restrict__ a, float *__res
   if (k < n) b[k] = (float)a[k];
 }
 
-constexpr int kNB = 1024;  // threads per block of the unknown kernel
+constexpr int kNB = 256;  // threads per block of the unknown kernel
 
 // ---------------------------------------------------------------------------
 // r0 = b - A (x0/q), P = A^T r0; per-cell partials of ||r0||^2 and ||b||^2
@@ -260,9 +293,17 @@ __device__ void scalar_update(int k, const LsqrState &in, double nvs, double dks
 }
 
 // ---------------------------------------------------------------------------
-// iteration k, cell part: scalars, then u' = A v - alpha u ; P = A^T u'
+// iteration k, cell part.  Block 0 runs the scalar recurrences (stopping
+// tests of iteration k-2, rotation of iteration k-1) and stores the state;
+// blocks 1..G do u' = A v - alpha u and P = A^T u' for cells b-1, b-1+G, ...
+// (G sized to the blocks resident on the device at once), prefetching the
+// next cell's block into registers while computing on the current one.  The
+// cell blocks need only alpha_k = ||v_k||, which every block reduces from the
+// unknown kernel's partials in the same order; when block 0 finds that the
+// iteration stopped, the cell blocks' work of this launch is simply unused
+// (the unknown kernel that follows sees the stop and leaves x unchanged).
 template <typename T, int R, int L>
-__global__ void __launch_bounds__(64) lsqr_A(int k, const T *__restrict__ A, const int *__restrict__ colmap,
+__global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restrict__ A, const int *__restrict__ colmap,
                                              const double *__restrict__ vp, double *__restrict__ u,
                                              double *__restrict__ P, double *__restrict__ pa1,
                                              const double *__restrict__ pb1, const double *__restrict__ pb2, int nb,
@@ -270,51 +311,71 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, const T *__restrict__ A, con
   constexpr int S = (R * L + 3) / 4 * 4;
   __shared__ __align__(16) T blk[S];
   __shared__ double vs[64], uu[64], sh[1];
-  __shared__ double s_alpha;
-  __shared__ int s_done;
-  const int c = blockIdx.x, t = threadIdx.x;
-  if (sin->done) {  // finished earlier: carry the final state forward
-    if (c == 0 && t == 0) *sout = *sin;
+  const int t = threadIdx.x;
+  if (blockIdx.x == 0) {
+    const double nvs = reduce_partials<64>(pb1, nb, sh);
+    const double dks = k >= 3 ? reduce_partials<64>(pb2, nb, sh) : 0.0;
+    if (t == 0) {
+      LsqrState o;
+      scalar_update(k, *sin, nvs, dks, o);
+      *sout = o;
+    }
     return;
   }
-  stage_block<T, S>(A + (size_t)c * S, blk);
-  const int j = t < L ? colmap[(size_t)c * L + t] : -1;
-  const double uo = t < R ? u[(size_t)c * R + t] : 0.0;
-  const double vj = j >= 0 ? vp[j] : 0.0;
-  const double beta = sin->beta;
+  if (sin->done) return;
+  const int G = gridDim.x - 1;
+  int c = blockIdx.x - 1;
+  int j = t < L ? colmap[(size_t)c * L + t] : -1;
+  double uo = t < R ? u[(size_t)c * R + t] : 0.0;
+  RegBlock<T, S> rb;
+  rb.load(A + (size_t)c * S);
+  double vj = j >= 0 ? vp[j] : 0.0;
+  const double beta = sin->beta, alpha_prev = sin->alpha;
   const double nvs = reduce_partials<64>(pb1, nb, sh);
-  const double dks = k >= 3 ? reduce_partials<64>(pb2, nb, sh) : 0.0;
-  if (t == 0) {
-    LsqrState o;
-    scalar_update(k, *sin, nvs, dks, o);
-    s_alpha = o.alpha;
-    s_done = o.done;
-    if (c == 0) *sout = o;
-  }
-  __syncthreads();
-  if (s_done) return;
-  const double alpha = s_alpha;
-  if (t < L) vs[t] = j >= 0 ? (alpha > 0.0 ? vj / alpha : vj) : 0.0;
-  __syncthreads();
-  double un = 0.0;
-  if (t < R) {
-    double s = 0.0;
+  // alpha_k as the recurrences define it (kept from k-1 when beta_k = 0)
+  const double alpha = (k == 1 || beta > 0.0) ? sqrt(nvs) : alpha_prev;
+  double acc = 0.0;
+  for (;;) {
+    rb.store(blk);
+    vs[t] = (t < L && j >= 0) ? (alpha > 0.0 ? vj / alpha : vj) : 0.0;
+    __syncthreads();
+    const int cn = c + G;
+    int jn = -1;
+    double uon = 0.0;
+    if (cn < ncell) {
+      jn = t < L ? colmap[(size_t)cn * L + t] : -1;
+      uon = t < R ? u[(size_t)cn * R + t] : 0.0;
+      rb.load(A + (size_t)cn * S);
+    }
+    double un = 0.0;
+    {
+      double s = 0.0;
+      const int row = t < R ? t : R - 1;
 #pragma unroll
-    for (int l = 0; l < L; ++l) s += (double)blk[t * L + l] * vs[l];
-    const double uold = beta > 0.0 ? uo / beta : uo;
-    un = -alpha * uold + s;
-    u[(size_t)c * R + t] = un;
+      for (int l = 0; l < L; ++l) s += (double)blk[row * L + l] * vs[l];
+      const double uold = beta > 0.0 ? uo / beta : uo;
+      if (t < R) {
+        un = -alpha * uold + s;
+        u[(size_t)c * R + t] = un;
+      }
+    }
     uu[t] = un;
-  }
-  __syncthreads();
-  if (t < L) {
-    double s = 0.0;
+    __syncthreads();
+    {
+      double s = 0.0;
+      const int col = t < L ? t : L - 1;
 #pragma unroll
-    for (int i = 0; i < R; ++i) s += (double)blk[i * L + t] * uu[i];
-    P[(size_t)c * L + t] = s;
+      for (int i = 0; i < R; ++i) s += (double)blk[i * L + col] * uu[i];
+      if (t < L) P[(size_t)c * L + t] = s;
+    }
+    acc += un * un;
+    if (cn >= ncell) break;
+    vj = jn >= 0 ? vp[jn] : 0.0;
+    j = jn, uo = uon, c = cn;
+    __syncthreads();  // blk, vs and uu are rewritten next
   }
-  const double s1 = wave_sum(un * un);
-  if (t == 0) pa1[c] = s1;
+  const double s1 = wave_sum(acc);
+  if (t == 0) pa1[blockIdx.x - 1] = s1;
 }
 
 // ---------------------------------------------------------------------------
@@ -401,16 +462,39 @@ void launch_r0(BlockSys &s, LsqrWork &wk, const double *b, const double *x, hipS
   hipLaunchKernelGGL((lsqr_r0<R, L>), dim3(s.ncell), dim3(64), 0, st, s.A.p, s.colmap.p, s.q.p, x, b, wk.u.p, wk.P.p,
                      wk.pa1.p, wk.pa2.p);
 }
+// resident blocks of a cell-kernel instantiation on the whole device
+template <typename T, int R, int L>
+int persistent_grid(int ncell) {
+  static int per_cu = -1, cus = 0;
+  if (per_cu < 0) {
+    int dev = 0;
+    TT_HIP(hipGetDevice(&dev));
+    TT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lsqr_A<T, R, L>, 64, 0));
+    hipFuncAttributes fa;
+    TT_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(lsqr_A<T, R, L>)));
+    int lds_per_cu = 0;
+    TT_HIP(hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    // static LDS rounded to the 512-byte allocation granule
+    const size_t lds = (fa.sharedSizeBytes + 511) / 512 * 512;
+    if (lds > 0 && lds_per_cu > 0) per_cu = std::min<int>(per_cu, (int)(lds_per_cu / lds));
+    if (per_cu < 1) per_cu = 1;
+  }
+  return std::min(ncell, per_cu * cus);
+}
 template <typename T, int R, int L>
 void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st) {
-  hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(s.ncell), dim3(64), 0, st, k, A, s.colmap.p, wk.v.p, wk.u.p, wk.P.p,
+  const int g = persistent_grid<T, R, L>(s.ncell);
+  wk.na = g;
+  hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, k, s.ncell, A, s.colmap.p, wk.v.p, wk.u.p, wk.P.p,
                      wk.pa1.p, wk.pb1.p, wk.pb2.p, (int)ceil_div(s.n, kNB), wk.state.p + ((k - 1) & 1),
                      wk.state.p + (k & 1));
 }
 template <int W>
 void launch_B(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
   hipLaunchKernelGGL((lsqr_B<W>), dim3(ceil_div(s.n, kNB)), dim3(kNB), 0, st, s.n, k, s.gtab.p, wk.P.p, wk.v.p,
-                     wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, s.ncell, wk.pb1.p, wk.pb2.p, wk.state.p + (k & 1));
+                     wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, k == 0 ? s.ncell : wk.na, wk.pb1.p, wk.pb2.p,
+                     wk.state.p + (k & 1));
 }
 
 void r0(BlockSys &s, LsqrWork &wk, const double *b, const double *x, hipStream_t st) {
@@ -441,11 +525,11 @@ void stepB(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
   }
 }
 
-// records the start event of a sampled launch; returns the pair index or -1
-long probe_begin(LsqrProbe *p, hipStream_t st) {
+// records the start event of a sampled launch of iteration k; returns the pair index or -1
+long probe_begin(LsqrProbe *p, int k, hipStream_t st) {
   if (!p || p->stride <= 0) return -1;
-  unsigned long long k = p->seen++;
-  if (k % (unsigned long long)p->stride) return -1;
+  unsigned long long n = p->seen++;
+  if (n % (unsigned long long)p->stride) return -1;
   if (2 * (p->used + 1) > p->pool.size()) {
     for (int i = 0; i < 64; ++i) {
       hipEvent_t e;
@@ -454,16 +538,20 @@ long probe_begin(LsqrProbe *p, hipStream_t st) {
     }
   }
   long idx = (long)p->used++;
+  if (p->iter.size() < p->used) p->iter.resize(p->used);
+  p->iter[idx] = k;
   TT_HIP(hipEventRecord(p->pool[2 * idx], st));
   return idx;
 }
 void probe_end(LsqrProbe *p, long idx, hipStream_t st) {
   if (idx >= 0) TT_HIP(hipEventRecord(p->pool[2 * idx + 1], st));
 }
-// after the stream has been synchronised
-void probe_collect(LsqrProbe *p) {
+// after the solve: keeps the launches that did the cell work (iterations
+// 1..nit+1; later launches of the batch return at once)
+void probe_collect(LsqrProbe *p, int nit) {
   if (!p) return;
   for (size_t i = 0; i < p->used; ++i) {
+    if (p->iter[i] > nit + 1) continue;
     float ms = 0.f;
     TT_HIP(hipEventElapsedTime(&ms, p->pool[2 * i], p->pool[2 * i + 1]));
     p->total_ms += ms;
@@ -567,13 +655,19 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
   stepB(s, wk, 0, st);
   TT_LAUNCH_CHECK();
   // the previous solve's count predicts this one's (the Picard loop changes
-  // only the right-hand side): termination after nit iterations is seen by
-  // the cell kernel of iteration nit + 2
-  int k = 1, batch = s.last_nit > 0 ? s.last_nit + 2 : 4;
+  // only the right-hand side and, through the dynamic tolerance, eps):
+  // termination after nit iterations is seen by the cell kernel of
+  // iteration nit + 2
+  int k = 1, batch = 8;
+  if (s.last_nit > 0) {
+    batch = s.last_nit + 2;
+    if (eps4 > s.last_eps) batch = std::max(8, batch / 2);  // looser tolerance: fewer iterations
+  }
+  const int more = std::max(4, s.last_nit / 8);
   LsqrResult res;
   for (;;) {
     for (int b2 = 0; b2 < batch; ++b2, ++k) {
-      long pi = probe_begin(probe, st);
+      long pi = probe_begin(probe, k, st);
       stepA(s, wk, k, use32, st);
       probe_end(probe, pi, st);
       stepB(s, wk, k, st);
@@ -581,15 +675,16 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
     TT_LAUNCH_CHECK();
     TT_HIP(hipMemcpyAsync(wk.host_state, wk.state.p + ((k - 1) & 1), sizeof(LsqrState), hipMemcpyDeviceToHost, st));
     TT_HIP(hipStreamSynchronize(st));
-    probe_collect(probe);
     if (wk.host_state->done) {
       res.nit = wk.host_state->itn;
       res.istop = wk.host_state->istop;
       break;
     }
     if (k > init.itnlim + 3) throw std::runtime_error("LSQR did not signal termination");
-    batch = k < 16 ? 4 : std::min(std::max(4, k / 4), 64);
+    batch = std::max(more, std::min(64, k / 4));
   }
+  probe_collect(probe, res.nit);
+  s.last_eps = eps4;
   s.last_nit = res.nit;
   // x = q * dx + x0 (x0 is still in x: the kernels only wrote wk.x)
   hipLaunchKernelGGL(finish_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, s.q.p, wk.x.p, x, x);
