@@ -3,8 +3,11 @@
 Workload: differentially heated unit cube at n^3 = 20^3 subdomains (27-node
 cells, 41^3 = 68,921 nodes, 9,602 boundary nodes), Ra = 1e5, Pr = 0.71, the
 reference's iteration parameters (GAUS 4 4 4 4, IIDI 2, LSQS 5000 1e-7,
-DLSE 1 10 0.001, URDW 0.3, URDT 0.1).  Setup (integrals, factorisations,
+DLSE 1 10 0.001) with the relaxation that converges at this Rayleigh number
+from rest (URDW 0.1, URDT 0.05).  Setup (integrals, factorisations,
 least-squares systems) is done once; the matrices stay resident in HBM.
+The CPU baseline is the oracle timed on a bounded sample of the same
+workload (oracle/cpu_sample.py).
 
 A step is one Picard iteration (main.f90:1200-1589): boundary vorticities,
 kinematics x,y,z, energy, vorticity transport x,y,z, convergence test and
@@ -136,7 +139,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle import cpu_sample  # the oracle: CPU baseline leg only
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import cpu_sample  # the oracle: CPU baseline leg only
         cpu = cpu_sample.picard_rate(host, case, legs["fp64"]["nits"], budget_s=args.cpu_seconds, log=log)
 
     ph = mixed["phases"]

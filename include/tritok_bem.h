@@ -68,15 +68,6 @@ int tt_macro_integrals(const tt_mesh_view *m, const tt_quad *q, int32_t copy, do
                        double *B, double *Abdx, double *Abdy, double *Abdz, double *Htx,
                        double *Hty, double *Htz, double *Dx, double *Dy, double *Dz, double ont[11]);
 
-/* Replaces corMin_modHelm / FMATmodHelm (lib_mHelm.f:1400-1502, :2-483): the
- * same matrices for the modified-Helmholtz kernel exp(-mu r)/(4 pi r), with mu
- * given per source point: mu_nodes(nnodes) for node sources and mu_q(nq) for
- * flux sources (mu = sqrt(beta/diff) at the source, lib_mHelm.f:139,197). */
-int tt_macro_integrals_mh(const tt_mesh_view *m, const tt_quad *q, const double *mu_nodes,
-                          const double *mu_q, double *H, double *G, double *B, double *Abdx,
-                          double *Abdy, double *Abdz, double *Htx, double *Hty, double *Htz,
-                          double *Dx, double *Dy, double *Dz, double ont[11]);
-
 /* Replaces FMATsdkm (lib_sdkm.f:1286-1530) as called by sdKm_corInt
  * (lib_sdkm.f:2377-2449) for boundary rows row0..row1-1 (0-based; the
  * reference's zac:kon).  Host outputs with leading dimension row1-row0:
@@ -113,29 +104,27 @@ int tt_csr_mv(int32_t m, int32_t n, int32_t nnz, const int32_t *ia, const int32_
 /* Dense single-domain kinematics.                                          */
 
 /* Replaces dgefa (lib_Solve_v11.f:787-904) as used by FormPREfm pret=2:
- * partial-pivot LU of a(n,n) in place (host array), pivots as doubles. */
+ * partial-pivot LU of a(n,n) in place (host array) in LINPACK's storage
+ * (negated multipliers, each column's multipliers in the row order of its own
+ * elimination step), pivots ipvt(n) as doubles.  Returns 0, or k > 0 when
+ * U(k,k) = 0 (dgefa's info), or < 0 on error. */
 int tt_dense_lu(int32_t n, double *a, double *ipvt);
-/* Replaces dgesl job=0 (lib_Solve_v11.f:905-1037): solves in place. */
+/* Replaces dgesl job=0 (lib_Solve_v11.f:905-1037): solves in place with the
+ * factor of tt_dense_lu (or of dgefa). */
 int tt_dense_lu_solve(int32_t n, const double *a, const double *ipvt, double *b);
 
-/* ------------------------------------------------------------------------ */
-/* Compressed matvecs.                                                       */
-
-/* ACA (lib_ACA.f): full-pivot cross approximation of a dense host matrix
- * a(m,n) to relative Frobenius tolerance eps (ADAPTIVE_CROSS_APPROXIMATION2,
- * lib_ACA.f:401-437) or to a fixed rank kmax when eps <= 0 (aca_CrossAprox
- * type 1, :126-181).  Returns rank k and factors u(m,k), v(k,n) (host arrays of
- * capacity kmax). */
-int tt_aca_compress(int32_t m, int32_t n, const double *a, double eps, int32_t kmax, double *u,
-                    double *v, int32_t *k);
-/* aca_abxv (lib_ACA.f:226-249): y = u (v x). */
-int tt_aca_mv(int32_t m, int32_t n, int32_t k, const double *u, const double *v, const double *x,
-              double *y);
-
-/* Stretched-Haar wavelet matvec Che_WTAWx (lib_WT.f:297-329): y = W^T C (W x)
- * for a compressed CSR C (nrow x ncol, 1-based) of the transformed matrix. */
-int tt_wt_mv(int32_t nrow, int32_t ncol, int32_t nnz, const int32_t *zvr, const int32_t *c,
-             const double *v, const double *x, double *y);
+typedef struct tt_sd tt_sd; /* device-resident single-domain kinematics */
+/* Replaces sdKm_corInt (lib_sdkm.f:2377-2449) for all boundary rows,
+ * sdKm_sysm (lib_sdkm.f:1171-1210) and FormPREfm pret=2 (lib_Solve_v11.f:
+ * 403-454) for the three equations: the seven dense matrices, the three
+ * system matrices and their factorisations stay on the device.  kode(nbnodes,3)
+ * as ReadBic sets it; the mesh view must carry the source point normals. */
+int tt_sd_create(const tt_mesh_view *m, const tt_quad *q, const int32_t *kode, tt_sd **out);
+/* Replaces sdkm_solve (lib_sdkm.f:23-121) for eqn = 1, 2, 3 in turn:
+ * boundary vorticities from vel(nnodes,3) and vort(nnodes,3), relaxed with
+ * urBw on rows with kode 0/3 and written back into vort. */
+int tt_sd_solve(tt_sd *h, const double *vel, double *vort, const double urBw[3]);
+void tt_sd_free(tt_sd *h);
 
 /* ------------------------------------------------------------------------ */
 /* Whole-case driver: the reference's main.f90 setup (:237-1084) and the

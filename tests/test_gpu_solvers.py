@@ -1,0 +1,160 @@
+"""GPU solver entry points against the oracle.
+
+Parity bars: column scaling q and the scaled values within 5e-16 / 1e-15
+relative (prelsqr2's sums are formed in the same order; the device square
+root and division round differently in the last bits); LSQR solutions within 1e-8 relative of the oracle's with
+iteration counts within 2 (different summation order in the products);
+LU factors within 1e-12 relative with identical pivots; the dense
+boundary-vorticity update within 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import host_for, load_golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_csr(m, n, per_row, seed):
+    rng = np.random.default_rng(seed)
+    ja = np.concatenate([np.sort(rng.choice(n, per_row, replace=False)) for _ in range(m)]) + 1
+    ia = np.arange(0, m * per_row + 1, per_row) + 1
+    a = rng.standard_normal(m * per_row)
+    return ia.astype(np.int32), ja.astype(np.int32), a
+
+
+def test_csr_mv_matches_dense_product():
+    from tritok_amd import solvers as S
+    ia, ja, a = _random_csr(1000, 700, 13, 1)
+    x = np.random.default_rng(2).standard_normal(700)
+    dense = np.zeros((1000, 700))
+    for i in range(1000):
+        dense[i, ja[ia[i] - 1:ia[i + 1] - 1] - 1] = a[ia[i] - 1:ia[i + 1] - 1]
+    y = S.csr_mv(ia, ja, a, x)
+    assert np.allclose(y, dense @ x, rtol=1e-13, atol=1e-13)
+
+
+def _vte_system(g, e=0):
+    """The reference's VTE system of a fixture case (sMat2crsSysRhsB_w_fill):
+    H + beta*Re*B on unknown node columns, -G on unknown flux columns, columns
+    in increasing unknown order."""
+    h = host_for(g)
+    v = h.view()
+    m = v.mesh
+    nc, nn, nq = m.nicell, m.nnodes, m.nq
+    arr = lambda p, n: np.ctypeslib.as_array(p, shape=(n,)).copy()
+    idc = arr(m.idc, nc * 27).reshape(27, nc).T - 1
+    ibf = arr(m.ibf, nc * 24).reshape(4, 6, nc).transpose(2, 1, 0).reshape(nc, 24) - 1
+    wk = arr(v.wkode, nn * 3).reshape(3, nn)[e]
+    wq = arr(v.wqkode, nq * 3).reshape(3, nq)[e]
+    beta = 1.5 / v.tstep
+    H, G, B = g["macro_H"], g["macro_G"], g["macro_B"]
+    rows_i, rows_j, vals = [1], [], []
+    for c in range(nc):
+        codes = np.concatenate([wk[idc[c]], wq[ibf[c]]])
+        for r in range(51):
+            row = c * 51 + r
+            ent = []
+            for col in range(51):
+                if codes[col] < 0:
+                    val = H[row, col] + B[row, col] * beta * v.ReN if col < 27 else -G[row, col - 27]
+                    ent.append((-codes[col], val))
+            ent.sort()
+            rows_j += [j for j, _ in ent]
+            vals += [x for _, x in ent]
+            rows_i.append(len(rows_j) + 1)
+    return np.array(rows_i, np.int32), np.array(rows_j, np.int32), np.array(vals), int(v.nunk[e])
+
+
+@pytest.mark.parametrize("kind", ["random", "vte_dc2", "vte_jit2"])
+def test_lsqr_matches_oracle(kind):
+    from tritok_amd import solvers as S
+    if kind == "random":
+        ia, ja, a = _random_csr(3000, 1000, 11, 3)
+        n = 1000
+    else:
+        ia, ja, a, n = _vte_system(load_golden(kind.split("_")[1]))
+    m = len(ia) - 1
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(m)
+    x0 = 0.1 * rng.standard_normal(n)
+    ao = a.copy()
+    q_ref = O.prelsqr2(m, n, ia, ja, ao)  # scales ao in place
+    s = S.Lsqr(ia, ja, a, n)
+    dq = np.max(np.abs(s.q - q_ref) / q_ref)
+    da = np.max(np.abs(s.a - ao) / np.maximum(np.abs(ao), 1e-300))
+    print(kind, "q max rel diff", dq, "scaled values", da, flush=True)
+    assert dq <= 5e-16 and da <= 1e-15  # a few roundings: the device square root and division
+    for eps in (1e-7, 1e-10):
+        x_ref, istop_ref, nit_ref = O.slvlsqr2(m, n, ia, ja, ao, q_ref, b, x0, 5000, eps)
+        x, istop, nit = s.solve(b, x0, 5000, eps)
+        print(kind, eps, "nit", nit, nit_ref, "istop", istop, istop_ref, "rel", rel_l2(x, x_ref), flush=True)
+        assert istop == istop_ref
+        assert abs(nit - nit_ref) <= 2
+        assert rel_l2(x, x_ref) < (1e-8 if eps < 1e-9 else 1e-5)
+    s.close()
+
+
+def test_dense_lu_and_solve_match_linpack():
+    from tritok_amd import solvers as S
+    rng = np.random.default_rng(11)
+    n = 300
+    a = rng.standard_normal((n, n))
+    lu_ref, piv_ref, info_ref = O.dgefa(a)
+    lu, piv, info = S.dense_lu(a)
+    assert info == info_ref == 0
+    assert np.array_equal(piv, piv_ref)
+    assert rel_l2(lu, lu_ref) < 1e-12
+    b = rng.standard_normal(n)
+    x_ref = O.dgesl(lu_ref, piv_ref, b)
+    x = S.dense_lu_solve(lu_ref, piv_ref, b)
+    assert rel_l2(x, x_ref) < 1e-13
+    assert rel_l2(a @ x, b) < 1e-10
+
+
+def _oracle_bw_step(h, g, vel, vort, urbw):
+    v = h.view()
+    m = v.mesh
+    nb, nn = m.nbnodes, m.nnodes
+    arr = lambda p, n, dt=np.int32: np.ctypeslib.as_array(p, shape=(n,)).copy()
+    lbn, gbn = arr(m.lbn, nn), arr(m.gbn, nb)
+    kode = arr(v.kode, nb * 3)
+    nx, ny, nz = (np.ctypeslib.as_array(p, shape=(nb,)).copy() for p in (m.nx, m.ny, m.nz))
+    mats = [np.asfortranarray(g["sd_" + k]) for k in ("H", "Htx", "Hty", "Htz", "Dx", "Dy", "Dz")]
+    vel = np.asfortranarray(vel)
+    w = np.array(vort, order="F")
+    for e in range(3):
+        kd = kode.reshape(3, nb)[e]
+        A = np.zeros((nb, nb))
+        for i in range(nb):
+            if kd[i] in (0, 3):
+                A[i] = nx[i] * mats[4][i, gbn - 1] + ny[i] * mats[5][i, gbn - 1] + nz[i] * mats[6][i, gbn - 1]
+            else:
+                A[i, i] = 1.0
+        lu, piv, _ = O.dgefa(A)
+        dfi = np.zeros(nb)
+        O._sdkm_rhs(e + 1, nn, nb, O._i(lbn), O._i(gbn), O._i(kode), O._d(nx), O._d(ny), O._d(nz),
+                    *[O._d(x) for x in mats], nb, O._d(vel), O._d(w), O._d(dfi), 1)
+        dfi = np.zeros(nb) if O.dnrm2(dfi) == 0.0 else O.dgesl(lu, piv, dfi)
+        rows = np.isin(kd, (0, 3))
+        w[gbn[rows] - 1, e] = w[gbn[rows] - 1, e] * (1.0 - urbw[e]) + urbw[e] * dfi[rows]
+    return w
+
+
+@pytest.mark.parametrize("case", ["dc2", "jit2"])
+def test_sd_kinematics_step_matches_oracle(case, gauss_table):
+    from tritok_amd import integrals as I, solvers as S
+    g = load_golden(case)
+    h = host_for(g)
+    nn = h.view().mesh.nnodes
+    rng = np.random.default_rng(21)
+    vel, vort = rng.standard_normal((nn, 3)), rng.standard_normal((nn, 3))
+    urbw = (1.0, 0.7, 0.4)
+    sd = S.SdKinematics(h, I.quad(4, 4, 4, 4, 2, table=gauss_table))
+    w = sd.solve(vel, vort, urbw)
+    w_ref = _oracle_bw_step(h, g, vel, vort, urbw)
+    e = rel_l2(w, w_ref)
+    print(case, "bw step rel", e, flush=True)
+    assert e < 1e-10
+    sd.close()

@@ -1,0 +1,159 @@
+// tt_dense_abi.hip -- C-ABI of the dense single-domain kinematics:
+//   tt_dense_lu / tt_dense_lu_solve: the LINPACK pair dgefa / dgesl
+//     (lib_Solve_v11.f:787-1037) on host arrays, computed on the device and
+//     returned in LINPACK's storage (negated multipliers, each column's
+//     multipliers in the row order of its own elimination step, pivots as
+//     REAL(8));
+//   tt_sd_create / tt_sd_solve / tt_sd_free: the device-resident form the
+//     Picard loop uses -- integrals (sdKm_corInt, lib_sdkm.f:2377-2449),
+//     system matrices and factorisation (sdKm_sysm + FormPREfm,
+//     lib_sdkm.f:1171-1210, lib_Solve_v11.f:403-454) once, then one
+//     boundary-vorticity update for x, y, z per call (sdkm_solve,
+//     lib_sdkm.f:23-121).
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "tritok_bem.h"
+#include "tt_dense.h"
+#include "tt_integrals.h"
+
+namespace tt {
+namespace {
+
+// dgesl job = 0 on LINPACK storage: one workgroup walks the n elimination
+// steps (interchange, then an axpy with the column's multipliers), then the
+// back substitution, column by column (coalesced column reads).
+__global__ void __launch_bounds__(1024) linpack_solve_kernel(int n, const double *__restrict__ a,
+                                                             const double *__restrict__ ipvt, double *__restrict__ b) {
+  __shared__ double t_sh;
+  const int tid = threadIdx.x;
+  for (int k = 0; k < n - 1; ++k) {
+    if (tid == 0) {
+      const int l = (int)ipvt[k] - 1;
+      const double t = b[l];
+      if (l != k) b[l] = b[k], b[k] = t;
+      t_sh = t;
+    }
+    __syncthreads();
+    const double t = t_sh;
+    const double *col = a + (size_t)n * k;
+    for (int i = k + 1 + tid; i < n; i += 1024) b[i] = b[i] + t * col[i];
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    if (tid == 0) {
+      b[k] = b[k] / a[k + (size_t)n * k];
+      t_sh = -b[k];
+    }
+    __syncthreads();
+    const double t = t_sh;
+    const double *col = a + (size_t)n * k;
+    for (int i = tid; i < k; i += 1024) b[i] = b[i] + t * col[i];
+    __syncthreads();
+  }
+}
+
+}  // namespace
+}  // namespace tt
+
+using namespace tt;
+
+extern "C" int tt_dense_lu(int32_t n, double *a, double *ipvt) {
+  return guarded("tt_dense_lu", [&]() -> int {
+    if (n <= 0 || !a || !ipvt) throw std::runtime_error("bad argument");
+    DBuf<double> da;
+    da.upload(a, (size_t)n * n);
+    DBuf<int> piv(n), info(1);
+    rocblas_handle h;
+    if (rocblas_create_handle(&h) != rocblas_status_success) throw std::runtime_error("rocblas_create_handle failed");
+    const rocblas_status st = rocsolver_dgetrf(h, n, n, da.p, n, piv.p, info.p);
+    rocblas_destroy_handle(h);
+    if (st != rocblas_status_success) throw std::runtime_error("rocsolver_dgetrf failed");
+    std::vector<int> hp(n);
+    int hinfo = 0;
+    piv.download(hp.data(), n);
+    info.download(&hinfo, 1);
+    da.download(a, (size_t)n * n);
+    TT_HIP(hipDeviceSynchronize());
+    // LAPACK applies every later interchange to the earlier multiplier
+    // columns; LINPACK leaves them in the order of their own step and stores
+    // them negated.  Undo the later interchanges column by column.
+    for (int k = 0; k < n - 1; ++k) {
+      double *col = a + (size_t)n * k;
+      for (int j = n - 2; j > k; --j) {
+        const int l = hp[j] - 1;
+        if (l != j) std::swap(col[j], col[l]);
+      }
+      for (int i = k + 1; i < n; ++i) col[i] = -col[i];
+    }
+    for (int k = 0; k < n; ++k) ipvt[k] = (double)hp[k];
+    ipvt[n - 1] = (double)n;
+    return hinfo;  // 0, or k when U(k,k) = 0 (as dgefa's info)
+  });
+}
+
+extern "C" int tt_dense_lu_solve(int32_t n, const double *a, const double *ipvt, double *b) {
+  return guarded("tt_dense_lu_solve", [&] {
+    if (n <= 0 || !a || !ipvt || !b) throw std::runtime_error("bad argument");
+    DBuf<double> da, dp, db;
+    da.upload(a, (size_t)n * n), dp.upload(ipvt, n), db.upload(b, n);
+    hipLaunchKernelGGL(linpack_solve_kernel, dim3(1), dim3(1024), 0, 0, n, da.p, dp.p, db.p);
+    TT_LAUNCH_CHECK();
+    db.download(b, n);
+    TT_HIP(hipDeviceSynchronize());
+    return 0;
+  });
+}
+
+struct tt_sd {
+  hipStream_t s = nullptr;
+  DevMesh mesh;
+  DenseKin dk;
+  int nn = 0;
+  DBuf<double> vel, vort;
+  ~tt_sd() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+extern "C" int tt_sd_create(const tt_mesh_view *m, const tt_quad *q, const int32_t *kode, tt_sd **out) {
+  return guarded("tt_sd_create", [&] {
+    if (!m || !q || !kode || !out) throw std::runtime_error("NULL argument");
+    if (!m->nx || !m->ny || !m->nz) throw std::runtime_error("the mesh view needs source point normals");
+    auto h = std::make_unique<tt_sd>();
+    TT_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+    upload_mesh(*m, h->mesh, h->s);
+    QuadDev qh;
+    build_quad(*q, qh);
+    double ont[10];
+    sd_integrals_dev(h->mesh, *m, qh, 0, m->nbnodes, h->dk.mats, ont, h->s);
+    double ms = 0;
+    dense_setup(h->dk, h->mesh, kode, SdPrec::F64, h->s, &ms);
+    h->nn = m->nnodes;
+    h->vel.alloc((size_t)m->nnodes * 3);
+    h->vort.alloc((size_t)m->nnodes * 3);
+    TT_HIP(hipStreamSynchronize(h->s));
+    *out = h.release();
+    return 0;
+  });
+}
+
+extern "C" int tt_sd_solve(tt_sd *h, const double *vel, double *vort, const double urBw[3]) {
+  return guarded("tt_sd_solve", [&] {
+    if (!h || !vel || !vort || !urBw) throw std::runtime_error("NULL argument");
+    const size_t n3 = (size_t)h->nn * 3;
+    h->vel.upload(vel, n3, h->s);
+    h->vort.upload(vort, n3, h->s);
+    dense_bw_step(h->dk, h->mesh, h->vel.p, h->vort.p, urBw, h->s);
+    h->vort.download(vort, n3, h->s);
+    TT_HIP(hipStreamSynchronize(h->s));
+    return 0;
+  });
+}
+
+extern "C" void tt_sd_free(tt_sd *h) { delete h; }

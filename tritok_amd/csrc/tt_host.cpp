@@ -43,7 +43,7 @@ struct tt_host {
   std::vector<double> vel, vort, qvort, temp, qtemp;
   // tri.inp
   tt_quad quad{8, 8, 4, 4, 1};
-  int copy = 0, iBw = 1, iDv = 1, iDw = 1, iDT = 0;
+  int copy = 0, iBw = 1, iDv = 1, iDw = 1, iDT = 0, iDC = 0, imH = 0;
   double ReN = 1.0, RaN = 1.0, PrN = 1.0, gx = 0.0, gy = 0.0, gz = -1.0, tstep = 1.0e20;
   double urBw[3] = {0.1, 0.1, 0.1}, urDv[3] = {0.1, 0.1, 0.1}, urDw[3] = {0.1, 0.1, 0.1}, urDT = 0.1;
   double eps = 1.0e-7;
@@ -655,7 +655,7 @@ extern "C" int tt_host_read_inp(tt_host *h, const char *text) {
     if (kw == "TIME") h->tstep = D(1);
     else if (kw == "LSQS") { h->lsqs_maxit = I(1); h->lsqs_eps = F(2); }
     else if (kw == "DLSE") { h->iDLSE = I(1); h->dlse_r = F(2); h->dlse_epsth = F(3); }
-    else if (kw == "EQNS") { h->iBw = I(1); h->iDv = I(2); h->iDw = I(3); h->iDT = I(4); }
+    else if (kw == "EQNS") { h->iBw = I(1); h->iDv = I(2); h->iDw = I(3); h->iDT = I(4); h->iDC = I(5); h->imH = I(6); }
     else if (kw == "GRAV") { h->gx = D(1); h->gy = D(2); h->gz = D(3); }
     else if (kw == "REYN") h->ReN = D(1);
     else if (kw == "RAPR") { h->RaN = D(1); h->PrN = D(2); }
@@ -668,9 +668,9 @@ extern "C" int tt_host_read_inp(tt_host *h, const char *text) {
     else if (kw == "GAUS") { h->quad.kmBr = I(1); h->quad.kmBs = I(2); h->quad.kmDr = I(3); h->quad.kmDs = I(4); }
     else if (kw == "IIDI") h->quad.iiDiv = I(1);
   }
-  if (h->iBw != 1 || h->iDv != 1 || h->iDw != 1 || h->iDT < 0 || h->iDT > 1) {
-    tt_set_error("EQNS %d %d %d %d: only the least-squares path 1 1 1 {0,1} is implemented", h->iBw, h->iDv,
-                 h->iDw, h->iDT);
+  if (h->iBw != 1 || h->iDv != 1 || h->iDw != 1 || h->iDT < 0 || h->iDT > 1 || h->iDC != 0 || h->imH != 0) {
+    tt_set_error("EQNS %d %d %d %d %d %d: only the least-squares path 1 1 1 {0,1} 0 0 is implemented", h->iBw,
+                 h->iDv, h->iDw, h->iDT, h->iDC, h->imH);
     return -4;
   }
   return 0;

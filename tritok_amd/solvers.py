@@ -1,0 +1,122 @@
+"""Sparse and dense solver entry points of the C-ABI: CRSxV, prelsqr2 +
+slvlsqr2 (LSQR on a CSR system), dgefa / dgesl, and the device-resident
+single-domain kinematics (sdKm_corInt + sdKm_sysm + FormPREfm, sdkm_solve).
+Index arrays are the reference's: 1-based, int32."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import P, check, i32
+
+PD, PI = _lib.PD, _lib.PI
+_csr_mv = _lib._sig("tt_csr_mv", i32, i32, i32, i32, PI, PI, PD, PD, PD)
+_lsqr_setup = _lib._sig("tt_lsqr_setup", i32, i32, i32, i32, PI, PI, PD, PD, i32, C.POINTER(P))
+_lsqr_solve = _lib._sig("tt_lsqr_solve", i32, P, PD, PD, i32, C.c_float, PI, PI)
+_lsqr_free = _lib._sig("tt_lsqr_free", None, P)
+_lu = _lib._sig("tt_dense_lu", i32, i32, PD, PD)
+_lu_solve = _lib._sig("tt_dense_lu_solve", i32, i32, PD, PD, PD)
+_sd_create = _lib._sig("tt_sd_create", i32, C.POINTER(_lib.MeshView), C.POINTER(_lib.Quad), PI, C.POINTER(P))
+_sd_solve = _lib._sig("tt_sd_solve", i32, P, PD, PD, PD)
+_sd_free = _lib._sig("tt_sd_free", None, P)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def csr_mv(ia, ja, a, x, m=None):
+    """CRSxV: y = A x for a 1-based CSR with m = len(ia) - 1 rows."""
+    ia, ja, a, x = _i32(ia), _i32(ja), _f64(a), _f64(x)
+    m = len(ia) - 1 if m is None else m
+    y = np.zeros(m)
+    check(_csr_mv(m, len(x), len(ja), ia.ctypes.data_as(PI), ja.ctypes.data_as(PI), a.ctypes.data_as(PD),
+                  x.ctypes.data_as(PD), y.ctypes.data_as(PD)), "tt_csr_mv")
+    return y
+
+
+class Lsqr:
+    """prelsqr2 at construction (q and, when scale_host, the scaled values in
+    self.a), slvlsqr2 in solve()."""
+
+    def __init__(self, ia, ja, a, n, scale_host=True):
+        self.ia, self.ja, self.a = _i32(ia), _i32(ja), np.array(a, dtype=np.float64)
+        self.m, self.n = len(self.ia) - 1, int(n)
+        self.q = np.zeros(self.n)
+        h = P()
+        check(_lsqr_setup(self.m, self.n, len(self.ja), self.ia.ctypes.data_as(PI), self.ja.ctypes.data_as(PI),
+                          self.a.ctypes.data_as(PD), self.q.ctypes.data_as(PD), int(scale_host), C.byref(h)),
+              "tt_lsqr_setup")
+        self._h = h
+
+    def solve(self, b, x0, maxit, eps4):
+        """Returns (x, istop, nit) like slvlsqr2."""
+        b, x = _f64(b), np.array(x0, dtype=np.float64)
+        nit, istop = i32(), i32()
+        check(_lsqr_solve(self._h, b.ctypes.data_as(PD), x.ctypes.data_as(PD), int(maxit), float(eps4),
+                          C.byref(nit), C.byref(istop)), "tt_lsqr_solve")
+        return x, istop.value, nit.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lsqr_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dense_lu(a):
+    """dgefa: returns (factor in LINPACK storage, pivots as float64, info)."""
+    a = np.array(a, dtype=np.float64, order="F")
+    n = a.shape[0]
+    piv = np.zeros(n)
+    info = check(_lu(n, a.ctypes.data_as(PD), piv.ctypes.data_as(PD)), "tt_dense_lu")
+    return a, piv, info
+
+
+def dense_lu_solve(lu, piv, b):
+    """dgesl job = 0."""
+    lu = np.asfortranarray(lu, dtype=np.float64)
+    piv, b = _f64(piv), np.array(b, dtype=np.float64)
+    check(_lu_solve(lu.shape[0], lu.ctypes.data_as(PD), piv.ctypes.data_as(PD), b.ctypes.data_as(PD)),
+          "tt_dense_lu_solve")
+    return b
+
+
+class SdKinematics:
+    """Device-resident dense single-domain kinematics of a Host's mesh."""
+
+    def __init__(self, host, quad):
+        self.host = host
+        self.view = host.view()
+        self._quad = quad
+        h = P()
+        check(_sd_create(C.byref(self.view.mesh), C.byref(quad), self.view.kode, C.byref(h)), "tt_sd_create")
+        self._h = h
+
+    def solve(self, vel, vort, urbw=(1.0, 1.0, 1.0)):
+        """One boundary-vorticity update (x, y, z); returns the new vorticity."""
+        vel = np.asfortranarray(vel, dtype=np.float64)
+        w = np.array(vort, dtype=np.float64, order="F")
+        ur = _f64(urbw)
+        check(_sd_solve(self._h, vel.ctypes.data_as(PD), w.ctypes.data_as(PD), ur.ctypes.data_as(PD)), "tt_sd_solve")
+        return w
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _sd_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
