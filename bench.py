@@ -65,6 +65,52 @@ def run_leg(case, prec, steps, warmup, probe_stride):
                 probe27=stats27, fields=case.fields())
 
 
+def copy_for_calibration(nbytes):
+    """One device-to-device copy of nbytes (the runtime's copy kernel), so that
+    a counter run sees a kernel of known traffic: nbytes read, nbytes written."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a), hip.hipFree(b)
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/*traffic*.md: a line 'traffic_bytes_per_launch <kernel> <bytes>')."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(REPO, "profiles", "*traffic*.md"))):
+        for line in open(f):
+            t = line.split()
+            if len(t) == 3 and t[0] == "traffic_bytes_per_launch" and t[1] == kernel:
+                return float(t[2])
+    return None
+
+
+def accuracy_leg(n):
+    """Converged fields of the heated cube at tight tolerances (RELA 1e-10,
+    LSQR eps 1e-12, fixed tolerance), FP64 and mixed, from the same start."""
+    from tritok_amd import workloads
+    from tritok_amd.case import Case
+    host = workloads.natural_convection(n, eps=1e-10, lsqs_eps=1e-12, dlse=(0, 10.0, 0.001), nnlit=20000)
+    out = {}
+    with Case(host, prec="fp32") as c:
+        for prec in ("fp64", "fp32"):
+            c.set_precision(prec)
+            c.reset()
+            t0 = time.perf_counter()
+            sta, ite = c.run()
+            out[prec] = (c.fields(), len(sta), time.perf_counter() - t0)
+            log("accuracy %d^3 %s: %d Picard iterations to 1e-10 in %.1f s" % (n, prec, len(sta), out[prec][2]))
+    diff = {k: rel_l2(out["fp32"][0][k], out["fp64"][0][k]) for k in FIELDS}
+    return {"workload": "heated cube %d^3, Ra=1e5, Picard to 1e-10, LSQR eps 1e-12" % n,
+            "rel_l2": diff, "max_rel_l2": max(diff.values()), "iterations": {p: out[p][1] for p in out},
+            "seconds": {p: out[p][2] for p in out}}
+
+
 def lsqr_matrix_bytes(info, value_bytes):
     """Algorithmic bytes of the two sparse products of one LSQR iteration for a
     CSR system (values + int32 column indices, row pointers), per the
@@ -81,6 +127,12 @@ def main():
     ap.add_argument("--probe-stride", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--accuracy-n", type=int, default=6,
+                    help="cube size of the converged mixed-vs-FP64 comparison (0: skip)")
+    ap.add_argument("--no-kinematics", action="store_true",
+                    help="EQNS iBw = 0: skip the single-domain kinematics (least-squares systems only)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="MiB of a device-to-device copy issued first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one process drives one MI355X); "
@@ -93,9 +145,11 @@ def main():
 
     dev = device_info()
     log("device", dev)
+    if args.calibrate_copy:
+        copy_for_calibration(args.calibrate_copy << 20)
     n = args.n
     t0 = time.perf_counter()
-    host = workloads.natural_convection(n, ra=1e5, pr=0.71)
+    host = workloads.natural_convection(n, ra=1e5, pr=0.71, ibw=0 if args.no_kinematics else 1)
     view = host.view()
     log("host preprocessing %.2fs  nnodes=%d nbnodes=%d cells=%d" % (time.perf_counter() - t0, view.mesh.nnodes,
                                                                      view.mesh.nbnodes, view.mesh.nicell))
@@ -129,10 +183,14 @@ def main():
     alg = np.array([lsqr_matrix_bytes(i, 4) for i in infos])
     alg_bytes = float((w * alg).sum() / max(w.sum(), 1.0))
     avg_ms = mixed["probe51"]["avg_ms"]
+    traffic = read_traffic("lsqr_A<float,51,51>")  # measured HBM bytes per launch (profiles/)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
     roofline = {"kernel": "lsqr_A<float> (51-row blocks)", "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                "traffic": traffic,
+                "traffic_GBps": traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None,
+                "traffic_frac": traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic and avg_ms > 0 else None,
                 "avg_launch_ms": avg_ms, "launches_timed": mixed["probe51"]["sampled"],
                 "launches": mixed["probe51"]["launches"], "algorithmic_bytes_per_launch": alg_bytes,
                 "block_bytes_per_launch": float(infos[0]["cells"] * 51 * 51 * 4)}
@@ -143,6 +201,8 @@ def main():
         import cpu_sample  # the oracle: CPU baseline leg only
         cpu = cpu_sample.picard_rate(host, case, legs["fp64"]["nits"], budget_s=args.cpu_seconds, log=log)
 
+    acc = accuracy_leg(args.accuracy_n) if args.accuracy_n > 0 else None
+
     ph = mixed["phases"]
     out = {
         "metric": METRIC, "value": value, "unit": "Picard iterations/s", "n_gpus": 1, "steps": args.steps,
@@ -152,7 +212,8 @@ def main():
                                "boundary nodes), Ra=1e5, Pr=0.71" % (n, view.mesh.nnodes, view.mesh.nbnodes),
                    "subdomains": n ** 3, "precision": "FP32 least-squares blocks, FP64 residual anchors"},
         "fp64_iters_per_s": fp64_rate, "mixed_over_fp64": value / fp64_rate,
-        "mixed_vs_fp64_rel_l2": max(diff.values()),
+        "mixed_vs_fp64_rel_l2_converged": acc["max_rel_l2"] if acc else None, "accuracy": acc,
+        "mixed_vs_fp64_rel_l2_after_steps": max(diff.values()),
         "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                             "max": float(lat[-1])},
         "phase_ms_mean": {k: float(np.mean([p[k] for p in ph])) for k in

@@ -154,6 +154,7 @@ typedef struct tt_case_view {
   double eps;
   int32_t nnlit, lsqs_maxit, iDLSE;
   float lsqs_eps, dlse_r, dlse_epsth;
+  int32_t iBw;           /* EQNS iBw: 1 single-domain kinematics, 0 none (main.f90:410,1207) */
 } tt_case_view;
 
 /* storage precision of the iteration matrices */

@@ -278,16 +278,21 @@ int or_picard(const or_case *c, double *vel, double *vort, double *qvort, double
   or_macro_integrals(&c->quad, nn, nc, nq, c->x, c->idc, c->ibf, c->xq, c->flpm, c->copy, mH, mG, mB, mAx, mAy,
                      mAz, mHtx, mHty, mHtz, mDx, mDy, mDz, ontm, nthreads);
   /* ---- single-domain kinematics ---- */
-  double *sH = malloc(sizeof(double) * (size_t)nb * nb), *sHtx = malloc(sizeof(double) * (size_t)nb * nb);
-  double *sHty = malloc(sizeof(double) * (size_t)nb * nb), *sHtz = malloc(sizeof(double) * (size_t)nb * nb);
-  double *sDx = malloc(sizeof(double) * (size_t)nb * nn), *sDy = malloc(sizeof(double) * (size_t)nb * nn);
-  double *sDz = malloc(sizeof(double) * (size_t)nb * nn);
-  or_sd_integrals(&c->quad, nn, nb, c->ne, nc, c->x, c->ibc, c->idc, c->gbn, c->lbn, 0, nb, nb, sH, sHtx, sHty,
+  /* (main.f90:410: only with EQNS iBw != 0) */
+  const int kin = c->iBw == 1;
+  const size_t nbb = kin ? (size_t)nb : 0;
+  double *sH = malloc(sizeof(double) * nbb * nb + 8), *sHtx = malloc(sizeof(double) * nbb * nb + 8);
+  double *sHty = malloc(sizeof(double) * nbb * nb + 8), *sHtz = malloc(sizeof(double) * nbb * nb + 8);
+  double *sDx = malloc(sizeof(double) * nbb * nn + 8), *sDy = malloc(sizeof(double) * nbb * nn + 8);
+  double *sDz = malloc(sizeof(double) * nbb * nn + 8);
+  if (kin)
+    or_sd_integrals(&c->quad, nn, nb, c->ne, nc, c->x, c->ibc, c->idc, c->gbn, c->lbn, 0, nb, nb, sH, sHtx, sHty,
                   sHtz, sDx, sDy, sDz, onts, nthreads);
   double *sA[3], *piv[3];
   for (int e = 0; e < 3; ++e) {
-    sA[e] = malloc(sizeof(double) * (size_t)nb * nb);
+    sA[e] = malloc(sizeof(double) * nbb * nb + 8);
     piv[e] = malloc(sizeof(double) * nb);
+    if (!kin) continue;
     for (int j = 0; j < nb; ++j) {
       int dn = c->gbn[j] - 1;
       for (int i = 0; i < nb; ++i) {
@@ -356,8 +361,8 @@ int or_picard(const or_case *c, double *vel, double *vort, double *qvort, double
   for (;;) {
     ++it;
     int nit[12] = {0};
-    /* ---- boundary vorticities: sdkm_solve eqn 1..3 ---- */
-    for (int e = 0; e < 3; ++e) {
+    /* ---- boundary vorticities: sdkm_solve eqn 1..3 (main.f90:1207, iBw = 1) ---- */
+    for (int e = 0; e < 3 && kin; ++e) {
       or_sdkm_rhs(e + 1, nn, nb, c->lbn, c->gbn, c->kode, c->nx, c->ny, c->nz, sH, sHtx, sHty, sHtz, sDx, sDy, sDz,
                   nb, vel, vort, dfi, nthreads);
       if (or_dnrm2(nb, dfi) == 0.0) {

@@ -36,7 +36,7 @@ class OrCase(C.Structure):
                 ("urBw", C.c_double * 3), ("urDv", C.c_double * 3), ("urDw", C.c_double * 3), ("urDT", C.c_double),
                 ("eps", C.c_double), ("nnlit", C.c_int), ("lsqs_maxit", C.c_int), ("iDLSE", C.c_int),
                 ("lsqs_eps", C.c_float), ("dlse_r", C.c_float), ("dlse_epsth", C.c_float),
-                ("quad", OrQuad), ("copy", C.c_int)]
+                ("quad", OrQuad), ("copy", C.c_int), ("iBw", C.c_int)]
 
 
 def _sig(name, res, *args):
@@ -214,7 +214,7 @@ def case_from_view(v):
         c.nunk[k], c.nbk[k] = v.nunk[k], v.nbk[k]
         c.urBw[k], c.urDv[k], c.urDw[k] = v.urBw[k], v.urDv[k], v.urDw[k]
     for f in ["Tnunk", "Tnb", "nsmcol", "nrhscol", "iDT", "ReN", "RaN", "PrN", "gx", "gy", "gz", "tstep", "urDT",
-              "eps", "nnlit", "lsqs_maxit", "iDLSE", "lsqs_eps", "dlse_r", "dlse_epsth", "copy"]:
+              "eps", "nnlit", "lsqs_maxit", "iDLSE", "lsqs_eps", "dlse_r", "dlse_epsth", "copy", "iBw"]:
         setattr(c, f, getattr(v, f))
     c.quad = OrQuad(v.quad.kmBr, v.quad.kmBs, v.quad.kmDr, v.quad.kmDs, v.quad.iiDiv,
                     C.cast(v.quad.gi, PD), C.cast(v.quad.ome, PD), C.cast(v.quad.ng1, PI), C.cast(v.quad.ng2, PI))

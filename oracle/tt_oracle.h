@@ -143,6 +143,7 @@ typedef struct {
   float lsqs_eps, dlse_r, dlse_epsth;
   or_quad quad;
   int copy;
+  int iBw;                      /* 1: single-domain kinematics, 0: none */
 } or_case;
 
 /* Fields in/out: velocity(nn,3), vorticity(nn,3), qvorticity(nq,3), temp(nn),

@@ -100,3 +100,28 @@ def test_case_step_api_and_timings(gauss_table):
         t = c.timings()
         assert t["step_total"] > 0 and t["setup_macro"] > 0
         assert c.device_bytes() > 0
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_case_without_kinematics_matches_oracle(prec, gauss_table):
+    """EQNS iBw = 0 (no single-domain kinematics, main.f90:410,1207): the
+    boundary vorticity keeps its boundary-condition value; the least-squares
+    systems alone converge to the oracle's fields (tight tolerances)."""
+    import oracle as O
+    from tritok_amd import integrals as I, workloads as W
+    from tritok_amd.case import Case
+    h = W.natural_convection(3, ra=1e3, eps=1e-10, lsqs_eps=1e-12, dlse=(0, 10.0, 0.001), ibw=0)
+    q = I.quad(4, 4, 4, 4, 2, table=gauss_table)
+    oc = O.case_from_view(h.view())
+    oc.quad = O.quad(4, 4, 4, 4, 2, table=gauss_table)
+    of, osta, oite, _ = O.picard(oc, h.fields())
+    with Case(h, prec=prec, quad=q) as c:
+        sta, ite = c.run()
+        f = c.fields()
+    print("iBw=0", prec, "iterations", len(sta), "oracle", len(osta), flush=True)
+    assert abs(len(sta) - len(osta)) <= 3
+    for k in FIELDS:
+        if np.linalg.norm(of[k]) > 0:
+            e = rel_l2(np.asarray(f[k]).reshape(-1), np.asarray(of[k]).reshape(-1))
+            print("iBw=0", prec, k, e, flush=True)
+            assert e < 1e-8, (k, e)

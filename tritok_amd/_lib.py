@@ -45,7 +45,7 @@ class CaseView(C.Structure):
                 ("ReN", dbl), ("RaN", dbl), ("PrN", dbl), ("gx", dbl), ("gy", dbl), ("gz", dbl), ("tstep", dbl),
                 ("urBw", dbl * 3), ("urDv", dbl * 3), ("urDw", dbl * 3), ("urDT", dbl), ("eps", dbl),
                 ("nnlit", i32), ("lsqs_maxit", i32), ("iDLSE", i32),
-                ("lsqs_eps", flt), ("dlse_r", flt), ("dlse_epsth", flt)]
+                ("lsqs_eps", flt), ("dlse_r", flt), ("dlse_epsth", flt), ("iBw", i32)]
 
 
 def _sig(name, res, *args):

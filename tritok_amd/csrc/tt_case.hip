@@ -392,13 +392,16 @@ extern "C" int tt_case_create(const tt_case_view *cv, const double *vel, const d
       macro_integrals_dev(c->mesh, c->qh, cv->copy, c->macro, ont, s);
       c->t_phase[7] = t.stop();
     }
-    {
-      EvTimer t(s);
-      double onts[10];
-      sd_integrals_dev(c->mesh, m, c->qh, 0, nb, c->dense.mats, onts, s);
-      c->t_phase[8] = t.stop();
+    if (cv->iBw != 0 && cv->iBw != 1) throw std::runtime_error("iBw must be 0 or 1");
+    if (cv->iBw == 1) {  // single-domain kinematics (main.f90:410-513)
+      {
+        EvTimer t(s);
+        double onts[10];
+        sd_integrals_dev(c->mesh, m, c->qh, 0, nb, c->dense.mats, onts, s);
+        c->t_phase[8] = t.stop();
+      }
+      dense_setup(c->dense, c->mesh, c->kode_h.data(), SdPrec::F64, s, &c->t_phase[9]);
     }
-    dense_setup(c->dense, c->mesh, c->kode_h.data(), SdPrec::F64, s, &c->t_phase[9]);
     // ---- time scheme (scheme 1) and the subdomain systems
     c->beta = 1.50 / cv->tstep, c->beta2 = 2.00 / cv->tstep, c->beta3 = 0.50 / cv->tstep;
     c->ren = cv->ReN;
@@ -455,7 +458,7 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
     // ---- boundary vorticities
     {
       EvTimer t(s);
-      dense_bw_step(c->dense, c->mesh, c->vel.p, c->vort.p, cv.urBw, s);
+      if (cv.iBw == 1) dense_bw_step(c->dense, c->mesh, c->vel.p, c->vort.p, cv.urBw, s);
       tp[0] = t.stop();
     }
     // ---- domain velocities
@@ -622,7 +625,7 @@ extern "C" int tt_device_info(char *name, int32_t name_len, int32_t *num_cu, int
     hipDeviceProp_t p;
     TT_HIP(hipGetDeviceProperties(&p, dev));
     if (name && name_len > 0) {
-      std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+      std::snprintf(name, name_len, "%s%s%s", p.name, p.name[0] ? " " : "", p.gcnArchName);
     }
     if (num_cu) *num_cu = p.multiProcessorCount;
     if (hbm_bytes) *hbm_bytes = (int64_t)p.totalGlobalMem;

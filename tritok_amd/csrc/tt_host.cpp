@@ -668,8 +668,8 @@ extern "C" int tt_host_read_inp(tt_host *h, const char *text) {
     else if (kw == "GAUS") { h->quad.kmBr = I(1); h->quad.kmBs = I(2); h->quad.kmDr = I(3); h->quad.kmDs = I(4); }
     else if (kw == "IIDI") h->quad.iiDiv = I(1);
   }
-  if (h->iBw != 1 || h->iDv != 1 || h->iDw != 1 || h->iDT < 0 || h->iDT > 1 || h->iDC != 0 || h->imH != 0) {
-    tt_set_error("EQNS %d %d %d %d %d %d: only the least-squares path 1 1 1 {0,1} 0 0 is implemented", h->iBw,
+  if (h->iBw < 0 || h->iBw > 1 || h->iDv != 1 || h->iDw != 1 || h->iDT < 0 || h->iDT > 1 || h->iDC != 0 || h->imH != 0) {
+    tt_set_error("EQNS %d %d %d %d %d %d: only the least-squares path {0,1} 1 1 {0,1} 0 0 is implemented", h->iBw,
                  h->iDv, h->iDw, h->iDT, h->iDC, h->imH);
     return -4;
   }
@@ -717,6 +717,7 @@ extern "C" int tt_host_case_view(tt_host *hp, tt_case_view *v) {
   v->nsmcol = h.nsmcol;
   v->nrhscol = h.nrhscol;
   v->iDT = h.iDT;
+  v->iBw = h.iBw;
   v->ReN = h.ReN;
   v->RaN = h.RaN;
   v->PrN = h.PrN;

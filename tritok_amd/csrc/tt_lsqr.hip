@@ -25,6 +25,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hip/hip_ext.h>
+
 #include "tt_lsqr.h"
 #include "tt_lsqr_dev.h"
 
@@ -362,13 +364,20 @@ int persistent_grid(int ncell) {
   }
   return std::min(ncell, per_cu * cus);
 }
+// e0/e1 non-null: the dispatch itself records its start and end (sampled timing)
 template <typename T, int R, int L>
-void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st) {
+void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
   const int g = persistent_grid<T, R, L>(s.ncell);
   wk.na = g;
-  hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, k, s.ncell, A, s.colmap.p, wk.v.p, wk.u.p, wk.P.p,
-                     wk.pa1.p, wk.pb1.p, wk.pb2.p, (int)ceil_div(s.n, kNB), wk.state.p + ((k - 1) & 1),
-                     wk.state.p + (k & 1));
+  if (e0)
+    hipExtLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, e0, e1, 0, k, s.ncell, A, s.colmap.p,
+                          (const double *)wk.v.p, wk.u.p, wk.P.p, wk.pa1.p, (const double *)wk.pb1.p,
+                          (const double *)wk.pb2.p, (int)ceil_div(s.n, kNB),
+                          (const LsqrState *)(wk.state.p + ((k - 1) & 1)), wk.state.p + (k & 1));
+  else
+    hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, k, s.ncell, A, s.colmap.p, wk.v.p, wk.u.p,
+                       wk.P.p, wk.pa1.p, wk.pb1.p, wk.pb2.p, (int)ceil_div(s.n, kNB), wk.state.p + ((k - 1) & 1),
+                       wk.state.p + (k & 1));
 }
 template <int W>
 void launch_B(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
@@ -383,17 +392,17 @@ void r0(BlockSys &s, LsqrWork &wk, const double *b, const double *x, hipStream_t
   else
     launch_r0<27, 27>(s, wk, b, x, st);
 }
-void stepA(BlockSys &s, LsqrWork &wk, int k, bool use32, hipStream_t st) {
+void stepA(BlockSys &s, LsqrWork &wk, int k, bool use32, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
   if (s.R == 51) {
     if (use32)
-      launch_A<float, 51, 51>(s, wk, k, s.A32.p, st);
+      launch_A<float, 51, 51>(s, wk, k, (const float *)s.A32.p, st, e0, e1);
     else
-      launch_A<double, 51, 51>(s, wk, k, s.A.p, st);
+      launch_A<double, 51, 51>(s, wk, k, (const double *)s.A.p, st, e0, e1);
   } else {
     if (use32)
-      launch_A<float, 27, 27>(s, wk, k, s.A32.p, st);
+      launch_A<float, 27, 27>(s, wk, k, (const float *)s.A32.p, st, e0, e1);
     else
-      launch_A<double, 27, 27>(s, wk, k, s.A.p, st);
+      launch_A<double, 27, 27>(s, wk, k, (const double *)s.A.p, st, e0, e1);
   }
 }
 void stepB(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
@@ -405,8 +414,8 @@ void stepB(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
   }
 }
 
-// records the start event of a sampled launch of iteration k; returns the pair index or -1
-long probe_begin(LsqrProbe *p, int k, hipStream_t st) {
+// events of a sampled launch of iteration k (recorded by the dispatch); -1 if not sampled
+long probe_begin(LsqrProbe *p, int k) {
   if (!p || p->stride <= 0) return -1;
   unsigned long long n = p->seen++;
   if (n % (unsigned long long)p->stride) return -1;
@@ -420,11 +429,7 @@ long probe_begin(LsqrProbe *p, int k, hipStream_t st) {
   long idx = (long)p->used++;
   if (p->iter.size() < p->used) p->iter.resize(p->used);
   p->iter[idx] = k;
-  TT_HIP(hipEventRecord(p->pool[2 * idx], st));
   return idx;
-}
-void probe_end(LsqrProbe *p, long idx, hipStream_t st) {
-  if (idx >= 0) TT_HIP(hipEventRecord(p->pool[2 * idx + 1], st));
 }
 // after the solve: keeps the launches that did the cell work (iterations
 // 1..nit+1; later launches of the batch return at once)
@@ -547,9 +552,8 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
   LsqrResult res;
   for (;;) {
     for (int b2 = 0; b2 < batch; ++b2, ++k) {
-      long pi = probe_begin(probe, k, st);
-      stepA(s, wk, k, use32, st);
-      probe_end(probe, pi, st);
+      const long pi = probe_begin(probe, k);
+      stepA(s, wk, k, use32, st, pi >= 0 ? probe->pool[2 * pi] : nullptr, pi >= 0 ? probe->pool[2 * pi + 1] : nullptr);
       stepB(s, wk, k, st);
     }
     TT_LAUNCH_CHECK();

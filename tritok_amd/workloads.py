@@ -60,14 +60,15 @@ def bic_heated(t_x0=0.5, t_x1=-0.5):
 
 
 def inp_text(re=1.0, rapr=None, urdw=0.3, urdt=0.1, eps=1e-6, nnlit=10000, lsqs_eps=1e-7, lsqs_maxit=5000,
-             dlse=(1, 10.0, 0.001), gauss=(4, 4, 4, 4), iidiv=2):
-    """tri.inp with the keywords the hot path reads."""
+             dlse=(1, 10.0, 0.001), gauss=(4, 4, 4, 4), iidiv=2, ibw=1):
+    """tri.inp with the keywords the hot path reads (ibw = EQNS iBw: 1 solves
+    the single-domain kinematics for the boundary vorticity, 0 keeps it)."""
     eq_t = 1 if rapr is not None else 0
     lines = ["MDIR ./", "URBW 1.0 1.0 1.0", "URDV 1.0 1.0 1.0", "URDW %r %r %r" % (urdw, urdw, urdw),
              "URDT %r" % urdt, "URDC 1.0"]
     if rapr is not None:
         lines.append("RAPR %r %r" % (float(rapr[0]), float(rapr[1])))
-    lines += ["REYN %r" % float(re), "PARA 0 0", "GRAV 0 0 -1", "EQNS 1 1 1 %d 0 0" % eq_t,
+    lines += ["REYN %r" % float(re), "PARA 0 0", "GRAV 0 0 -1", "EQNS %d 1 1 %d 0 0" % (ibw, eq_t),
               "RELA %d 1.0 %r" % (nnlit, eps), "TIME 1.0D20 1 1", "LSQS %d %r" % (lsqs_maxit, lsqs_eps),
               "DLSE %d %r %r" % dlse, "GAUS %d %d %d %d" % tuple(gauss), "IIDI %d" % iidiv, "REST 0"]
     return "\n".join(lines) + "\n"
