@@ -110,3 +110,29 @@ def test_read_inp_parses_hot_path_keywords():
     assert list(v.urDw) == [0.1, 0.1, 0.1] and v.urDT == pytest.approx(0.05)
     with pytest.raises(Exception):
         W.Host.cube(2).read_bic(W.bic_heated()).read_inp("EQNS 1 1 1 1 1 0\n")  # diffusion-convection: not on this path
+
+
+AMDFLANG = "/opt/rocm/bin/amdflang"
+
+
+@pytest.mark.skipif(not os.path.exists(AMDFLANG), reason="amdflang not installed")
+def test_fortran_binding_compiles_and_matches_c_layout(tmp_path):
+    """tritok_amd/fortran/tritok_amd_bind.f90 compiles, and its BIND(C) types
+    have the sizes of the C structs (so a Fortran host passes them as-is)."""
+    import subprocess
+    src = os.path.join(REPO, "tritok_amd", "fortran", "tritok_amd_bind.f90")
+    prog = tmp_path / "sizes.f90"
+    prog.write_text("PROGRAM sizes\n  USE tritok_amd_bind\n  TYPE(tt_mesh_view) :: m\n  TYPE(tt_quad) :: q\n"
+                    "  TYPE(tt_case_view) :: v\n  PRINT '(3I6)', C_SIZEOF(m), C_SIZEOF(q), C_SIZEOF(v)\nEND PROGRAM\n")
+    subprocess.run([AMDFLANG, "-c", src, "-o", str(tmp_path / "b.o"), "-module-dir", str(tmp_path)], check=True,
+                   cwd=tmp_path)
+    libdir = os.path.join(REPO, "tritok_amd")
+    subprocess.run([AMDFLANG, str(prog), str(tmp_path / "b.o"), "-I", str(tmp_path), "-o", str(tmp_path / "sizes"),
+                    "-L" + libdir, "-ltritok_amd", "-Wl,-rpath," + libdir], check=True, cwd=tmp_path)
+    fsz = subprocess.run([str(tmp_path / "sizes")], check=True, capture_output=True, text=True).stdout.split()
+    csrc = tmp_path / "sizes.c"
+    csrc.write_text('#include <stdio.h>\n#include "tritok_bem.h"\nint main(void){printf("%zu %zu %zu\\n",'
+                    'sizeof(tt_mesh_view),sizeof(tt_quad),sizeof(tt_case_view));return 0;}\n')
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), str(csrc), "-o", str(tmp_path / "csizes")], check=True)
+    csz = subprocess.run([str(tmp_path / "csizes")], check=True, capture_output=True, text=True).stdout.split()
+    assert fsz == csz, (fsz, csz)

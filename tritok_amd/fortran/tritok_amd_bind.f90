@@ -1,0 +1,205 @@
+! tritok_amd_bind.f90 -- ISO_C_BINDING interface of libtritok_amd.so
+! (include/tritok_bem.h) for the Tritok Fortran host.
+!
+! The derived types mirror the C structs field for field; arrays are passed
+! as the reference holds them (column-major, 1-based index arrays, INTEGER =
+! C int32, REAL(8) = C double, REAL(4) = C float).  The adapter subroutines at
+! the end keep the argument lists of the reference routines they replace, so
+! a call site changes only its subroutine name (and keeps a device handle).
+MODULE tritok_amd_bind
+  USE, INTRINSIC :: ISO_C_BINDING
+  IMPLICIT NONE
+
+  TYPE, BIND(C) :: tt_mesh_view
+    INTEGER(C_INT32_T) :: nnodes, nbnodes, nbelem, nicell, nq
+    TYPE(C_PTR) :: x, ibc, idc, ibf, xq, flpm, gbn, lbn, nx, ny, nz
+  END TYPE
+
+  TYPE, BIND(C) :: tt_quad
+    INTEGER(C_INT32_T) :: kmBr, kmBs, kmDr, kmDs, iiDiv
+    TYPE(C_PTR) :: gi, ome, ng1, ng2
+  END TYPE
+
+  TYPE, BIND(C) :: tt_case_view
+    TYPE(tt_mesh_view) :: mesh
+    TYPE(tt_quad) :: quad
+    INTEGER(C_INT32_T) :: copy
+    TYPE(C_PTR) :: kode, kobc, wkode, wqkode, Tkode, Tqkode
+    INTEGER(C_INT32_T) :: nunk(3), nbk(3), Tnunk, Tnb
+    TYPE(C_PTR) :: kmu
+    INTEGER(C_INT32_T) :: nsmcol, nrhscol, iDT
+    REAL(C_DOUBLE) :: ReN, RaN, PrN, gx, gy, gz, tstep
+    REAL(C_DOUBLE) :: urBw(3), urDv(3), urDw(3), urDT, eps
+    INTEGER(C_INT32_T) :: nnlit, lsqs_maxit, iDLSE
+    REAL(C_FLOAT) :: lsqs_eps, dlse_r, dlse_epsth
+    INTEGER(C_INT32_T) :: iBw
+  END TYPE
+
+  INTEGER(C_INT32_T), PARAMETER :: TT_PREC_FP64 = 0, TT_PREC_FP32 = 1
+
+  INTERFACE
+    FUNCTION tt_last_error() BIND(C, NAME="tt_last_error")
+      IMPORT :: C_PTR
+      TYPE(C_PTR) :: tt_last_error
+    END FUNCTION
+
+    ! FMATkmdcSteady / corMin_kmdcSteady (lib_fmat.f:3, lib_int.f:1254)
+    FUNCTION tt_macro_integrals(m, q, copy, H, G, B, Abdx, Abdy, Abdz, Htx, Hty, Htz, Dx, Dy, Dz, ont) &
+        BIND(C, NAME="tt_macro_integrals")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, tt_mesh_view, tt_quad
+      TYPE(tt_mesh_view), INTENT(IN) :: m
+      TYPE(tt_quad), INTENT(IN) :: q
+      INTEGER(C_INT32_T), VALUE :: copy
+      REAL(C_DOUBLE) :: H(*), G(*), B(*), Abdx(*), Abdy(*), Abdz(*), Htx(*), Hty(*), Htz(*)
+      REAL(C_DOUBLE) :: Dx(*), Dy(*), Dz(*), ont(11)
+      INTEGER(C_INT) :: tt_macro_integrals
+    END FUNCTION
+
+    ! FMATsdkm / sdKm_corInt rows zac:kon (lib_sdkm.f:1286, :2377)
+    FUNCTION tt_sd_integrals(m, q, row0, row1, H, Htx, Hty, Htz, Dx, Dy, Dz, ont) &
+        BIND(C, NAME="tt_sd_integrals")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, tt_mesh_view, tt_quad
+      TYPE(tt_mesh_view), INTENT(IN) :: m
+      TYPE(tt_quad), INTENT(IN) :: q
+      INTEGER(C_INT32_T), VALUE :: row0, row1
+      REAL(C_DOUBLE) :: H(*), Htx(*), Hty(*), Htz(*), Dx(*), Dy(*), Dz(*), ont(10)
+      INTEGER(C_INT) :: tt_sd_integrals
+    END FUNCTION
+
+    ! prelsqr2 / slvlsqr2 (lib_Solve_v11.f:4501, :4401)
+    FUNCTION tt_lsqr_setup(m, n, nnz, ia, ja, a, q, scale_host, h) BIND(C, NAME="tt_lsqr_setup")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      INTEGER(C_INT32_T), VALUE :: m, n, nnz, scale_host
+      INTEGER(C_INT32_T), INTENT(IN) :: ia(*), ja(*)
+      REAL(C_DOUBLE) :: a(*), q(*)
+      TYPE(C_PTR) :: h
+      INTEGER(C_INT) :: tt_lsqr_setup
+    END FUNCTION
+    FUNCTION tt_lsqr_solve(h, b, x, maxit, eps4, nit, istop) BIND(C, NAME="tt_lsqr_solve")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_FLOAT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(IN) :: b(*)
+      REAL(C_DOUBLE) :: x(*)
+      INTEGER(C_INT32_T), VALUE :: maxit
+      REAL(C_FLOAT), VALUE :: eps4
+      INTEGER(C_INT32_T) :: nit, istop
+      INTEGER(C_INT) :: tt_lsqr_solve
+    END FUNCTION
+    SUBROUTINE tt_lsqr_free(h) BIND(C, NAME="tt_lsqr_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+
+    ! CRSxV (lib_util.f:1351)
+    FUNCTION tt_csr_mv(m, n, nnz, ia, ja, a, x, y) BIND(C, NAME="tt_csr_mv")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: m, n, nnz
+      INTEGER(C_INT32_T), INTENT(IN) :: ia(*), ja(*)
+      REAL(C_DOUBLE), INTENT(IN) :: a(*), x(*)
+      REAL(C_DOUBLE) :: y(*)
+      INTEGER(C_INT) :: tt_csr_mv
+    END FUNCTION
+
+    ! dgefa / dgesl (lib_Solve_v11.f:787, :905)
+    FUNCTION tt_dense_lu(n, a, ipvt) BIND(C, NAME="tt_dense_lu")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: n
+      REAL(C_DOUBLE) :: a(*), ipvt(*)
+      INTEGER(C_INT) :: tt_dense_lu
+    END FUNCTION
+    FUNCTION tt_dense_lu_solve(n, a, ipvt, b) BIND(C, NAME="tt_dense_lu_solve")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE
+      INTEGER(C_INT32_T), VALUE :: n
+      REAL(C_DOUBLE), INTENT(IN) :: a(*), ipvt(*)
+      REAL(C_DOUBLE) :: b(*)
+      INTEGER(C_INT) :: tt_dense_lu_solve
+    END FUNCTION
+
+    ! sdKm_corInt + sdKm_sysm + FormPREfm, then sdkm_solve x, y, z
+    FUNCTION tt_sd_create(m, q, kode, h) BIND(C, NAME="tt_sd_create")
+      IMPORT :: C_INT, C_INT32_T, C_PTR, tt_mesh_view, tt_quad
+      TYPE(tt_mesh_view), INTENT(IN) :: m
+      TYPE(tt_quad), INTENT(IN) :: q
+      INTEGER(C_INT32_T), INTENT(IN) :: kode(*)
+      TYPE(C_PTR) :: h
+      INTEGER(C_INT) :: tt_sd_create
+    END FUNCTION
+    FUNCTION tt_sd_solve(h, vel, vort, urBw) BIND(C, NAME="tt_sd_solve")
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(IN) :: vel(*), urBw(3)
+      REAL(C_DOUBLE) :: vort(*)
+      INTEGER(C_INT) :: tt_sd_solve
+    END FUNCTION
+    SUBROUTINE tt_sd_free(h) BIND(C, NAME="tt_sd_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+
+    ! main.f90 setup (:237-1084) and the steady Picard loop (:1200-1589)
+    FUNCTION tt_case_create(v, vel, vort, qvort, temp, qtemp, prec, refine_tol, h) BIND(C, NAME="tt_case_create")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR, tt_case_view
+      TYPE(tt_case_view), INTENT(IN) :: v
+      REAL(C_DOUBLE), INTENT(IN) :: vel(*), vort(*), qvort(*), temp(*), qtemp(*)
+      INTEGER(C_INT32_T), VALUE :: prec
+      REAL(C_DOUBLE), VALUE :: refine_tol
+      TYPE(C_PTR) :: h
+      INTEGER(C_INT) :: tt_case_create
+    END FUNCTION
+    FUNCTION tt_case_picard_step(h, err, nit) BIND(C, NAME="tt_case_picard_step")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE) :: err(9)
+      INTEGER(C_INT32_T) :: nit(12)
+      INTEGER(C_INT) :: tt_case_picard_step
+    END FUNCTION
+    FUNCTION tt_case_fields(h, vel, vort, qvort, temp, qtemp) BIND(C, NAME="tt_case_fields")
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE) :: vel(*), vort(*), qvort(*), temp(*), qtemp(*)
+      INTEGER(C_INT) :: tt_case_fields
+    END FUNCTION
+    SUBROUTINE tt_case_free(h) BIND(C, NAME="tt_case_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+  END INTERFACE
+
+CONTAINS
+
+  ! prelsqr2 with the reference's arguments; h receives the device handle
+  ! that slvlsqr2_gpu takes (a is scaled in place, as prelsqr2 does)
+  SUBROUTINE prelsqr2_gpu(m, n, nnz, q, ia, ja, a, h)
+    INTEGER :: m, n, nnz, ia(m + 1), ja(nnz)
+    REAL(8) :: q(n), a(nnz)
+    TYPE(C_PTR) :: h
+    IF (tt_lsqr_setup(m, n, nnz, ia, ja, a, q, 1, h) < 0) CALL tt_fail('prelsqr2_gpu')
+  END SUBROUTINE
+
+  ! slvlsqr2 with the reference's arguments (ierr = istop when istop > 3)
+  SUBROUTINE slvlsqr2_gpu(h, m, n, nnz, maxit, eps4, nit, ierr, q, ia, ja, a, b, x)
+    TYPE(C_PTR) :: h
+    INTEGER :: m, n, nnz, maxit, nit, ierr, ia(m + 1), ja(nnz)
+    REAL(4) :: eps4
+    REAL(8) :: q(n), a(nnz), b(m), x(n)
+    INTEGER(C_INT32_T) :: istop, nit4
+    IF (tt_lsqr_solve(h, b, x, maxit, eps4, nit4, istop) < 0) CALL tt_fail('slvlsqr2_gpu')
+    nit = nit4
+    ierr = 0
+    IF (istop > 3) ierr = istop
+  END SUBROUTINE
+
+  SUBROUTINE tt_fail(where)
+    CHARACTER(*) :: where
+    CHARACTER(KIND=C_CHAR), POINTER :: msg(:)
+    INTEGER :: k
+    CALL C_F_POINTER(tt_last_error(), msg, [512])
+    k = 0
+    DO WHILE (k < 512)
+      IF (msg(k + 1) == C_NULL_CHAR) EXIT
+      k = k + 1
+    END DO
+    WRITE (*, '(A,": ",512A)') where, msg(1:k)
+    STOP 4
+  END SUBROUTINE
+END MODULE tritok_amd_bind
