@@ -90,22 +90,22 @@ def read_traffic(kernel):
     return None
 
 
-def accuracy_leg(n):
+def accuracy_leg(n, mixed):
     """Converged fields of the heated cube at tight tolerances (RELA 1e-10,
     LSQR eps 1e-12, fixed tolerance), FP64 and mixed, from the same start."""
     from tritok_amd import workloads
     from tritok_amd.case import Case
     host = workloads.natural_convection(n, eps=1e-10, lsqs_eps=1e-12, dlse=(0, 10.0, 0.001), nnlit=20000)
     out = {}
-    with Case(host, prec="fp32") as c:
-        for prec in ("fp64", "fp32"):
+    with Case(host, prec=mixed) as c:
+        for prec in ("fp64", mixed):
             c.set_precision(prec)
             c.reset()
             t0 = time.perf_counter()
             sta, ite = c.run()
             out[prec] = (c.fields(), len(sta), time.perf_counter() - t0)
             log("accuracy %d^3 %s: %d Picard iterations to 1e-10 in %.1f s" % (n, prec, len(sta), out[prec][2]))
-    diff = {k: rel_l2(out["fp32"][0][k], out["fp64"][0][k]) for k in FIELDS}
+    diff = {k: rel_l2(out[mixed][0][k], out["fp64"][0][k]) for k in FIELDS}
     return {"workload": "heated cube %d^3, Ra=1e5, Picard to 1e-10, LSQR eps 1e-12" % n,
             "rel_l2": diff, "max_rel_l2": max(diff.values()), "iterations": {p: out[p][1] for p in out},
             "seconds": {p: out[p][2] for p in out}}
@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=20)
     ap.add_argument("--probe-stride", type=int, default=8)
+    ap.add_argument("--mixed", default="fp16", choices=["fp32", "fp16", "bf16"],
+                    help="reduced precision of the mixed leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--accuracy-n", type=int, default=6,
@@ -154,7 +156,7 @@ def main():
     log("host preprocessing %.2fs  nnodes=%d nbnodes=%d cells=%d" % (time.perf_counter() - t0, view.mesh.nnodes,
                                                                      view.mesh.nbnodes, view.mesh.nicell))
     t0 = time.perf_counter()
-    case = Case(host, prec="fp32")
+    case = Case(host, prec=args.mixed)
     setup_s = time.perf_counter() - t0
     tm = case.timings()
     log("setup %.2fs (macro %.0f ms, sdkm %.0f ms, LU %.0f ms, systems %.0f ms), device bytes %.2f GB" % (
@@ -162,16 +164,16 @@ def main():
         case.device_bytes() / 1e9))
 
     legs = {}
-    for prec in ("fp64", "fp32"):
+    for prec in ("fp64", args.mixed):
         legs[prec] = run_leg(case, prec, args.steps, args.warmup, args.probe_stride)
         L = legs[prec]
         log("%s: %.3f s for %d steps (%.1f ms/step), LSQR iterations per step %s" % (
             prec, L["total_s"], args.steps, 1e3 * L["total_s"] / args.steps, L["nits"].sum(0)[3:10] / args.steps))
-    f64, f32 = legs["fp64"]["fields"], legs["fp32"]["fields"]
+    f64, f32 = legs["fp64"]["fields"], legs[args.mixed]["fields"]
     diff = {k: rel_l2(f32[k], f64[k]) for k in FIELDS}
     log("mixed vs fp64 rel L2 after %d iterations: %s" % (args.warmup + args.steps, diff))
 
-    mixed = legs["fp32"]
+    mixed = legs[args.mixed]
     value = args.steps / mixed["total_s"]
     fp64_rate = args.steps / legs["fp64"]["total_s"]
     lat = np.sort(mixed["lat"]) * 1e3
@@ -201,16 +203,19 @@ def main():
         import cpu_sample  # the oracle: CPU baseline leg only
         cpu = cpu_sample.picard_rate(host, case, legs["fp64"]["nits"], budget_s=args.cpu_seconds, log=log)
 
-    acc = accuracy_leg(args.accuracy_n) if args.accuracy_n > 0 else None
+    acc = accuracy_leg(args.accuracy_n, args.mixed) if args.accuracy_n > 0 else None
 
     ph = mixed["phases"]
     out = {
         "metric": METRIC, "value": value, "unit": "Picard iterations/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * mixed["total_s"] / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "%s/f32/f64" % {"fp16": "f16", "bf16": "bf16", "fp32": "f32"}[args.mixed],
+        "data": "synthetic",
         "config": {"workload": "natural convection, differentially heated cube, %d^3 subdomains (%d nodes, %d "
                                "boundary nodes), Ra=1e5, Pr=0.71" % (n, view.mesh.nnodes, view.mesh.nbnodes),
-                   "subdomains": n ** 3, "precision": "FP32 least-squares blocks, FP64 residual anchors"},
+                   "subdomains": n ** 3,
+                   "precision": "FP32 least-squares blocks with FP64 residual anchors; kinematics products in %s "
+                                "with per-row 2^k scales, FP64 anchor step every 8 iterations" % args.mixed.upper()},
         "fp64_iters_per_s": fp64_rate, "mixed_over_fp64": value / fp64_rate,
         "mixed_vs_fp64_rel_l2_converged": acc["max_rel_l2"] if acc else None, "accuracy": acc,
         "mixed_vs_fp64_rel_l2_after_steps": max(diff.values()),

@@ -157,18 +157,18 @@ typedef struct tt_case_view {
   int32_t iBw;           /* EQNS iBw: 1 single-domain kinematics, 0 none (main.f90:410,1207) */
 } tt_case_view;
 
-/* storage precision of the iteration matrices */
+/* precision of the iteration matrices (see tt_case_set_precision) */
 enum { TT_PREC_FP64 = 0, TT_PREC_FP32 = 1, TT_PREC_BF16 = 2, TT_PREC_FP16 = 3 };
 
 /* Builds everything the Picard loop needs on the device: macro and sdkm
  * integrals, the three factorizations, all least-squares systems, and the
  * fields (initial values vel(nn,3), vort(nn,3), qvort(nq,3), temp(nn),
- * qtemp(nq)).  prec = TT_PREC_FP64 runs every least-squares iteration on the
- * FP64 blocks; TT_PREC_FP32 runs the bidiagonalisation products on FP32
- * copies of the scaled blocks while the initial residual of every solve
- * (b - A x0, the hot start), the LSQR scalars and norms and the solution
- * update stay FP64, so the Picard loop refines the FP32 corrections against
- * the FP64 equations.  refine_tol is reserved (pass 0). */
+ * qtemp(nq)).  prec as tt_case_set_precision: with anything but TT_PREC_FP64
+ * the least-squares bidiagonalisation products run on FP32 copies of the
+ * scaled blocks while the initial residual of every solve (b - A x0, the hot
+ * start), the LSQR scalars and norms and the solution update stay FP64, so
+ * the Picard loop refines the reduced-precision corrections against the FP64
+ * equations.  refine_tol is reserved (pass 0). */
 int tt_case_create(const tt_case_view *c, const double *vel, const double *vort,
                    const double *qvort, const double *temp, const double *qtemp, int32_t prec,
                    double refine_tol, tt_case **out);
@@ -176,9 +176,18 @@ int tt_case_create(const tt_case_view *c, const double *vel, const double *vort,
  * (the state main.f90:1145-1199 sets up before the first Picard step). */
 int tt_case_reset(tt_case *c, const double *vel, const double *vort, const double *qvort,
                   const double *temp, const double *qtemp);
-/* Switches the least-squares storage precision (TT_PREC_FP64 / TT_PREC_FP32)
- * of a resident case; FP32 copies are made on first use. */
+/* Switches the precision of a resident case.  TT_PREC_FP64: FP64 throughout.
+ * TT_PREC_FP32, TT_PREC_FP16, TT_PREC_BF16: FP32 least-squares blocks (FP64
+ * residual anchors, scalars and vectors), and the dense kinematics products
+ * (the seven matrices of sdkm_rhs and the three inverse system matrices) on
+ * copies in that format with a power-of-two scale per row, taken on the change
+ * of the fields since the last FP64 anchor step; an anchor step (FP64 products
+ * and solves) runs every anchor period (default 8) Picard iterations and after
+ * a reset, so the loop converges to the FP64 fixed point.  Copies are made on
+ * first use. */
 int tt_case_set_precision(tt_case *c, int32_t prec);
+/* Anchor period of the reduced-precision kinematics (>= 1; 1 = every step FP64). */
+int tt_case_set_anchor_period(tt_case *c, int32_t period);
 /* Measurement hooks (no reference counterpart): time every stride-th launch
  * of the least-squares block kernel with HIP events on the case's stream
  * (stride 0 = off; resets the statistics), and read the average duration,

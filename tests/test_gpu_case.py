@@ -4,7 +4,9 @@ The fixtures' converged fields were written by the reference binary; the
 oracle reproduces them bit for bit (test_oracle.py), so they are the oracle's
 fields too.  Parity bars:
   * tight cases (RELA eps 1e-10, LSQR eps 1e-12, fixed tolerance): converged
-    fields within 1e-8 relative L2, FP64 and mixed FP32/FP64 alike;
+    fields within 1e-8 relative L2, in FP64 and in every reduced-precision
+    mode (FP32 least-squares blocks; kinematics products in FP32, FP16 or BF16
+    anchored by FP64 steps);
   * default cases (RELA eps 1e-6, LSQR eps 1e-7 with the dynamic tolerance):
     the iteration trajectory follows the reference (same iteration count
     within 2, per-iteration changes within 1e-2 relative over the first ten
@@ -41,7 +43,7 @@ def _compare(g, f, tol, tag):
         assert e < tol, (tag, k, e)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16", "bf16"])
 @pytest.mark.parametrize("case", ["dc2t", "nc2t"])
 def test_tight_case_matches_reference(case, prec, gauss_table):
     g = load_golden(case)

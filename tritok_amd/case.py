@@ -23,6 +23,7 @@ _bytes = _lib._sig("tt_case_device_bytes", i64, P)
 _free = _lib._sig("tt_case_free", None, P)
 _reset = _lib._sig("tt_case_reset", i32, P, _lib.PD, _lib.PD, _lib.PD, _lib.PD, _lib.PD)
 _setprec = _lib._sig("tt_case_set_precision", i32, P, i32)
+_setanchor = _lib._sig("tt_case_set_anchor_period", i32, P, i32)
 _probe = _lib._sig("tt_case_probe", i32, P, i32)
 _probe_stats = _lib._sig("tt_case_probe_stats", i32, P, i32, C.POINTER(C.c_double), C.POINTER(i64), C.POINTER(i64))
 _sysinfo = _lib._sig("tt_case_system_info", i32, P, i32, C.POINTER(i64))
@@ -40,9 +41,9 @@ class Case:
     """Device-resident case.  `host` is a tritok_amd.Host with mesh, boundary
     conditions and tri.inp parameters; `quad` optionally replaces the
     quadrature description (a tt_quad, e.g. carrying the host's Gauss table).
-    `prec` selects the storage precision of the subdomain least-squares
-    blocks ("fp64" or "fp32"; fp32 iterations are anchored on an FP64 initial
-    residual)."""
+    `prec`: "fp64" everywhere, or "fp32" / "fp16" / "bf16": FP32
+    least-squares blocks anchored on FP64 residuals, and the dense kinematics
+    products in that format anchored by an FP64 step every few iterations."""
 
     def __init__(self, host, prec="fp64", quad=None, fields=None, refine_tol=1e-12):
         self.host = host
@@ -93,6 +94,9 @@ class Case:
 
     def set_precision(self, prec):
         check(_setprec(self._h, PREC[prec]), "tt_case_set_precision")
+
+    def set_anchor_period(self, period):
+        check(_setanchor(self._h, int(period)), "tt_case_set_anchor_period")
 
     def probe(self, stride):
         check(_probe(self._h, int(stride)), "tt_case_probe")

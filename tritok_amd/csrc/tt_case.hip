@@ -359,9 +359,32 @@ void reset_fields(tt_case &cr, const double *vel, const double *vort, const doub
                        c->temp.p, c->ttime.p);
     TT_LAUNCH_CHECK();
     for (int k = 0; k < 9; ++k) c->dlse[k] = c->cv.lsqs_eps;
+    c->dense.since_anchor = 0;
     c->it = 0;
     c->finished = false;
   }
+}
+}  // namespace
+
+namespace {
+// TT_PREC_FP64: FP64 everywhere.  TT_PREC_FP32 / FP16 / BF16: FP32 least-squares
+// blocks, and the dense kinematics products on copies in that format, anchored
+// by an FP64 step every anchor_period Picard iterations.
+void apply_precision(tt_case &c, int32_t prec) {
+  if (prec < TT_PREC_FP64 || prec > TT_PREC_FP16) throw std::runtime_error("unknown precision");
+  if (prec != TT_PREC_FP64) {
+    for (SubSys *ss : {&c.vte[0], &c.vte[1], &c.vte[2], &c.km}) block_make32(ss->bs, c.s);
+    if (c.cv.iDT == 1) block_make32(c.tsys.bs, c.s);
+  }
+  c.lsqr32 = prec != TT_PREC_FP64;
+  if (c.cv.iBw == 1) {
+    const SdPrec sp = prec == TT_PREC_FP64   ? SdPrec::F64
+                      : prec == TT_PREC_FP32 ? SdPrec::F32
+                      : prec == TT_PREC_BF16 ? SdPrec::BF16
+                                             : SdPrec::F16;
+    dense_set_precision(c.dense, sp, c.s);
+  }
+  TT_HIP(hipStreamSynchronize(c.s));
 }
 }  // namespace
 
@@ -378,7 +401,6 @@ extern "C" int tt_case_create(const tt_case_view *cv, const double *vel, const d
     const int nn = m.nnodes, nb = m.nbnodes, nc = m.nicell, nq = m.nq;
     TT_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     hipStream_t s = c->s;
-    c->lsqr32 = sd_prec != TT_PREC_FP64;
     c->kode_h.assign(cv->kode, cv->kode + (size_t)nb * 3);
     bool need_kobc = false;
     for (int k : c->kode_h) need_kobc |= (k == 2 || k == 3);
@@ -436,6 +458,7 @@ extern "C" int tt_case_create(const tt_case_view *cv, const double *vel, const d
     c->kode.upload(c->kode_h, s);
     if (cv->kobc) c->kobc.upload(cv->kobc, nb, s);
     reset_fields(*c, vel, vort, qvort, temp, qtemp);
+    apply_precision(*c, sd_prec);
     TT_HIP(hipStreamSynchronize(s));
     *out = c.release();
     return 0;
@@ -606,6 +629,8 @@ extern "C" int64_t tt_case_device_bytes(tt_case *c) {
   const SdDev &S = c->dense.mats;
   for (auto *p : {&S.H, &S.Htx, &S.Hty, &S.Htz, &S.Dx, &S.Dy, &S.Dz}) add(*p);
   for (auto &p : c->dense.Ainv) add(p);
+  for (auto &p : c->dense.lo16) add(p);
+  for (auto &p : c->dense.lo32) add(p);
   add(c->Af_vte), add(c->Af_t), add(c->Af_km);
   for (SubSys *ss : {&c->vte[0], &c->vte[1], &c->vte[2], &c->tsys, &c->km}) add(ss->bs.A), add(ss->bs.A32);
   return b;
@@ -643,18 +668,21 @@ extern "C" int tt_case_reset(tt_case *c, const double *vel, const double *vort, 
   });
 }
 
+
 extern "C" int tt_case_set_precision(tt_case *c, int32_t prec) {
   return guarded("tt_case_set_precision", [&] {
     if (!c) throw std::runtime_error("NULL case");
-    if (prec != TT_PREC_FP64 && prec != TT_PREC_FP32)
-      throw std::runtime_error("least-squares blocks support TT_PREC_FP64 and TT_PREC_FP32");
-    if (prec == TT_PREC_FP32) {
-      for (SubSys *ss : {&c->vte[0], &c->vte[1], &c->vte[2], &c->km})
-        block_make32(ss->bs, c->s);
-      if (c->cv.iDT == 1) block_make32(c->tsys.bs, c->s);
-    }
-    c->lsqr32 = prec == TT_PREC_FP32;
-    TT_HIP(hipStreamSynchronize(c->s));
+    apply_precision(*c, prec);
+    return 0;
+  });
+}
+
+extern "C" int tt_case_set_anchor_period(tt_case *c, int32_t period) {
+  return guarded("tt_case_set_anchor_period", [&] {
+    if (!c) throw std::runtime_error("NULL case");
+    if (period < 1) throw std::runtime_error("the anchor period must be at least 1");
+    c->dense.anchor_period = period;
+    c->dense.since_anchor = 0;
     return 0;
   });
 }

@@ -16,27 +16,35 @@ struct DenseKin {
   int nb = 0, nn = 0;
   SdDev mats;                 // H, Htx, Hty, Htz (nb x nb), Dx, Dy, Dz (nb x nn), FP64
   DBuf<double> Ainv[3];       // inverse of the system matrix of each equation
-  // low-precision copies for the product (per-row power-of-two scale)
-  SdPrec prec = SdPrec::F64;
-  DBuf<uint16_t> lowH[4], lowD[3];  // bf16/fp16 payloads
-  DBuf<float> f32H[4], f32D[3];
-  DBuf<float> rscale;               // 7 x nb row scales (2^k)
   // per-iteration work
-  DBuf<double> YH, YD, part, dfi, xs, dw, vold;
+  DBuf<double> YH, YD, part, dfi, xs, dw;
   DBuf<int> kode;             // nb x 3
   DBuf<int> isbnd;            // nn: boundary index + 1 or 0
-  // anchored delta products for reduced precision
-  DBuf<double> vanchor, wanchor;  // fields the current products correspond to
-  bool anchored = false;
-  int since_anchor = 0;
+  // Reduced-precision form: the ten matrices (order H, Htx, Hty, Htz, Dx, Dy,
+  // Dz, Ainv x, y, z) stored with leading dimension ldl (nb rounded up to 8)
+  // and a power-of-two scale per row (10 x ldl floats); products are taken on
+  // the change of the fields since an FP64 anchor step.
+  SdPrec prec = SdPrec::F64;
+  int ldl = 0;
+  DBuf<uint16_t> lo16[10];
+  DBuf<float> lo32[10];
+  DBuf<float> rscale;
+  int anchor_period = 8;      // FP64 step every anchor_period steps
+  int since_anchor = 0;       // 0: the next step is an anchor step
+  DBuf<double> YH_a, YD_a, dfi_a, sol_a, vel_a, vort_a;
 };
 
 // Builds A_e, factorises and inverts it (rocSOLVER getrf/getri).
 void dense_setup(DenseKin &d, const DevMesh &m, const int *kode_host, SdPrec prec, hipStream_t s, double *ms_lu);
 
 // Boundary-vorticity step for eqn 1..3 in order: vel (nn,3) read, vort (nn,3)
-// boundary entries updated with relaxation urBw.
+// boundary entries updated with relaxation urBw.  In a reduced precision the
+// step is an FP64 anchor step every anchor_period calls and a delta step
+// otherwise.
 void dense_bw_step(DenseKin &d, const DevMesh &m, const double *vel, double *vort, const double urBw[3],
                    hipStream_t s);
+
+// Builds the reduced-precision copies (F32, F16 or BF16; F64 releases them).
+void dense_set_precision(DenseKin &d, SdPrec prec, hipStream_t s);
 
 }  // namespace tt

@@ -220,6 +220,270 @@ __global__ void add_slots_kernel(int nb, int nm, int ks_cnt, const double *__res
   }
 }
 
+// ---------------------------------------------------------------------------
+// Reduced-precision storage: value = stored * scale(row), scale a power of two
+// chosen so that the row's largest magnitude lies in [0.5, 1).
+
+enum class Lo { F32, F16, BF16 };
+
+__device__ __forceinline__ float bf16_to_f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
+__device__ __forceinline__ uint16_t f_to_bf16(float f) {  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ uint16_t f_to_f16(float f) {
+  _Float16 h = (_Float16)f;
+  return *reinterpret_cast<uint16_t *>(&h);
+}
+__device__ __forceinline__ float f16_to_f(uint16_t u) { return (float)*reinterpret_cast<_Float16 *>(&u); }
+
+// four consecutive rows of one column
+template <Lo L>
+struct Load4;
+template <>
+struct Load4<Lo::F32> {
+  using T = float;
+  __device__ static float4 at(const float *p) { return *reinterpret_cast<const float4 *>(p); }
+};
+template <>
+struct Load4<Lo::F16> {
+  using T = uint16_t;
+  __device__ static float4 at(const uint16_t *p) {
+    const ushort4 v = *reinterpret_cast<const ushort4 *>(p);
+    return make_float4(f16_to_f(v.x), f16_to_f(v.y), f16_to_f(v.z), f16_to_f(v.w));
+  }
+};
+template <>
+struct Load4<Lo::BF16> {
+  using T = uint16_t;
+  __device__ static float4 at(const uint16_t *p) {
+    const ushort4 v = *reinterpret_cast<const ushort4 *>(p);
+    return make_float4(bf16_to_f(v.x), bf16_to_f(v.y), bf16_to_f(v.z), bf16_to_f(v.w));
+  }
+};
+
+// largest |M(i, j)| over a chunk of columns, per row
+__global__ void rowmax_kernel(int nr, int nc, int ks_cnt, const double *__restrict__ M, float *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, ks = blockIdx.y;
+  if (i >= nr) return;
+  const int chunk = (nc + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(nc, j0 + chunk);
+  double m = 0.0;
+  for (int j = j0; j < j1; ++j) m = fmax(m, fabs(M[i + (size_t)nr * j]));
+  out[(size_t)ks * nr + i] = (float)m;
+}
+__global__ void rowscale_kernel(int nr, int ks_cnt, const float *__restrict__ part, float *__restrict__ scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  float m = 0.f;
+  for (int k = 0; k < ks_cnt; ++k) m = fmaxf(m, part[(size_t)k * nr + i]);
+  int e = 0;
+  if (m > 0.f) frexpf(m, &e);  // m = f * 2^e, f in [0.5, 1)
+  scale[i] = ldexpf(1.f, e);   // stored = value / scale
+}
+template <Lo L>
+__global__ void to_low_kernel(int nr, int nc, int ldl, const double *__restrict__ M, const float *__restrict__ scale,
+                              typename Load4<L>::T *__restrict__ out) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)ldl * nc) return;
+  const int i = (int)(k % ldl);
+  const size_t j = k / ldl;
+  const float v = i < nr ? (float)(M[i + (size_t)nr * j] / (double)scale[i]) : 0.f;
+  if constexpr (L == Lo::F32)
+    out[k] = v;
+  else if constexpr (L == Lo::F16)
+    out[k] = f_to_f16(v);
+  else
+    out[k] = f_to_bf16(v);
+}
+
+constexpr int kLoRows = 4;                    // rows per thread
+constexpr int kLoBlockRows = 64 * kLoRows;    // rows per block
+
+// delta of [H Htx Hty Htz] products: sum_j Hq(i,j) * (vel - vel_a)(gbn(j), c)
+template <Lo L>
+__global__ void __launch_bounds__(64) lprod_h_kernel(int nb, int nn, int ldl, int ks_cnt,
+                                                     const typename Load4<L>::T *__restrict__ H0,
+                                                     const typename Load4<L>::T *__restrict__ H1,
+                                                     const typename Load4<L>::T *__restrict__ H2,
+                                                     const typename Load4<L>::T *__restrict__ H3,
+                                                     const float *__restrict__ rscale, const double *__restrict__ vel,
+                                                     const double *__restrict__ vel_a, const int *__restrict__ gbn,
+                                                     double *__restrict__ part) {
+  const int i0 = blockIdx.x * kLoBlockRows + threadIdx.x * kLoRows, ks = blockIdx.y;
+  const int chunk = (nb + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(nb, j0 + chunk);
+  float acc[4][3][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[q][c][r] = 0.f;
+  if (i0 < nb) {
+    for (int j = j0; j < j1; ++j) {
+      const size_t g = gbn[j] - 1;
+      const float d0 = (float)(vel[g] - vel_a[g]), d1 = (float)(vel[g + nn] - vel_a[g + nn]),
+                  d2 = (float)(vel[g + 2 * (size_t)nn] - vel_a[g + 2 * (size_t)nn]);
+      const size_t o = (size_t)ldl * j + i0;
+      const float4 m[4] = {Load4<L>::at(H0 + o), Load4<L>::at(H1 + o), Load4<L>::at(H2 + o), Load4<L>::at(H3 + o)};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float mv[4] = {m[q].x, m[q].y, m[q].z, m[q].w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          acc[q][0][r] += mv[r] * d0;
+          acc[q][1][r] += mv[r] * d1;
+          acc[q][2][r] += mv[r] * d2;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + r;
+      if (i >= nb) break;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          part[((size_t)ks * 12 + q * 3 + c) * nb + i] = (double)acc[q][c][r] * (double)rscale[(size_t)q * ldl + i];
+    }
+  }
+}
+
+// delta of [Dx Dy Dz] products split into boundary (0) / domain (1) columns:
+// slot m*6 + c*2 + dom
+template <Lo L>
+__global__ void __launch_bounds__(64) lprod_d_kernel(int nb, int nn, int ldl, int ks_cnt,
+                                                     const typename Load4<L>::T *__restrict__ D0,
+                                                     const typename Load4<L>::T *__restrict__ D1,
+                                                     const typename Load4<L>::T *__restrict__ D2,
+                                                     const float *__restrict__ rscale, const double *__restrict__ vort,
+                                                     const double *__restrict__ vort_a, const int *__restrict__ isbnd,
+                                                     double *__restrict__ part) {
+  const int i0 = blockIdx.x * kLoBlockRows + threadIdx.x * kLoRows, ks = blockIdx.y;
+  const int chunk = (nn + ks_cnt - 1) / ks_cnt, k0 = ks * chunk, k1 = min(nn, k0 + chunk);
+  float acc[3][6][4];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int o = 0; o < 6; ++o)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[q][o][r] = 0.f;
+  if (i0 < nb) {
+    for (int k = k0; k < k1; ++k) {
+      const float w0 = (float)(vort[k] - vort_a[k]), w1 = (float)(vort[k + nn] - vort_a[k + nn]),
+                  w2 = (float)(vort[k + 2 * (size_t)nn] - vort_a[k + 2 * (size_t)nn]);
+      const size_t o = (size_t)ldl * k + i0;
+      const float4 m[3] = {Load4<L>::at(D0 + o), Load4<L>::at(D1 + o), Load4<L>::at(D2 + o)};
+      const int dom = isbnd[k] ? 0 : 1;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const float mv[4] = {m[q].x, m[q].y, m[q].z, m[q].w};
+        if (dom == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            acc[q][0][r] += mv[r] * w0;
+            acc[q][2][r] += mv[r] * w1;
+            acc[q][4][r] += mv[r] * w2;
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            acc[q][1][r] += mv[r] * w0;
+            acc[q][3][r] += mv[r] * w1;
+            acc[q][5][r] += mv[r] * w2;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + r;
+      if (i >= nb) break;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double sc = rscale[(size_t)(4 + q) * ldl + i];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) part[((size_t)ks * 18 + q * 6 + o) * nb + i] = (double)acc[q][o][r] * sc;
+      }
+    }
+  }
+}
+
+// partials of M_lo * (x - x_a) for an nb x nb matrix (x_a may be null)
+template <Lo L>
+__global__ void __launch_bounds__(64) lgemv_kernel(int nb, int ldl, int ks_cnt, const typename Load4<L>::T *__restrict__ M,
+                                                   const float *__restrict__ scale, const double *__restrict__ x,
+                                                   const double *__restrict__ x_a, double *__restrict__ part) {
+  const int i0 = blockIdx.x * kLoBlockRows + threadIdx.x * kLoRows, ks = blockIdx.y;
+  const int chunk = (nb + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(nb, j0 + chunk);
+  if (i0 >= nb) return;
+  float a[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int j = j0; j < j1; ++j) {
+    const float d = (float)(x_a ? x[j] - x_a[j] : x[j]);
+    const float4 m = Load4<L>::at(M + (size_t)ldl * j + i0);
+    a[0] += m.x * d, a[1] += m.y * d, a[2] += m.z * d, a[3] += m.w * d;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    if (i0 + r < nb) part[(size_t)ks * nb + i0 + r] = (double)a[r] * (double)scale[i0 + r];
+}
+
+// partials of sum_j D_lo(i, gbn(j)) * dw(j) for one or two D matrices
+template <Lo L>
+__global__ void __launch_bounds__(64) ldcorr_kernel(int nb, int ldl, int ks_cnt, int nm,
+                                                    const typename Load4<L>::T *__restrict__ D0,
+                                                    const typename Load4<L>::T *__restrict__ D1,
+                                                    const float *__restrict__ s0, const float *__restrict__ s1,
+                                                    const int *__restrict__ gbn, const double *__restrict__ dw,
+                                                    double *__restrict__ part) {
+  const int i0 = blockIdx.x * kLoBlockRows + threadIdx.x * kLoRows, ks = blockIdx.y;
+  const int chunk = (nb + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(nb, j0 + chunk);
+  if (i0 >= nb) return;
+  float a0[4] = {0.f, 0.f, 0.f, 0.f}, a1[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int j = j0; j < j1; ++j) {
+    const double dd = dw[j];
+    if (dd == 0.0) continue;  // uniform across the wavefront
+    const float d = (float)dd;
+    const size_t col = (size_t)(gbn[j] - 1) * ldl + i0;
+    const float4 m = Load4<L>::at(D0 + col);
+    a0[0] += m.x * d, a0[1] += m.y * d, a0[2] += m.z * d, a0[3] += m.w * d;
+    if (nm > 1) {
+      const float4 n = Load4<L>::at(D1 + col);
+      a1[0] += n.x * d, a1[1] += n.y * d, a1[2] += n.z * d, a1[3] += n.w * d;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + r;
+    if (i >= nb) break;
+    part[((size_t)ks * nm + 0) * nb + i] = (double)a0[r] * (double)s0[i];
+    if (nm > 1) part[((size_t)ks * nm + 1) * nb + i] = (double)a1[r] * (double)s1[i];
+  }
+}
+
+// out[o*nb+i] = base[o*nb+i] + sum over chunks of part
+__global__ void reduce_base_kernel(int nb, int no, int ks_cnt, const double *__restrict__ part,
+                                   const double *__restrict__ base, double *__restrict__ out) {
+  size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)nb * no) return;
+  int o = (int)(idx / nb), i = (int)(idx % nb);
+  double s = 0.0;
+  for (int k = 0; k < ks_cnt; ++k) s += part[((size_t)k * no + o) * nb + i];
+  out[idx] = base[idx] + s;
+}
+
+__global__ void copy_kernel(size_t n, const double *__restrict__ a, double *__restrict__ b) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) b[k] = a[k];
+}
+
+int lo_split(int nb, int ncols) {
+  int rows = (nb + kLoBlockRows - 1) / kLoBlockRows;
+  int ks = (2048 + rows - 1) / rows;
+  return std::max(1, std::min(ks, std::max(1, ncols / 64)));
+}
+
 int split_for(int nb, int ncols) {
   // enough wavefronts to keep HBM busy (~8 per CU on 256 CUs)
   int rows = (nb + kRowsPerBlock - 1) / kRowsPerBlock;
@@ -277,12 +541,22 @@ void dense_setup(DenseKin &d, const DevMesh &m, const int *kode_host, SdPrec pre
   d.dfi.alloc(nb), d.xs.alloc(nb), d.dw.alloc(nb);
 }
 
-void dense_bw_step(DenseKin &d, const DevMesh &m, const double *vel, double *vort, const double urBw[3],
-                   hipStream_t s) {
+namespace {
+
+// FP64 step; with save = true it also records the anchor (fields, products,
+// right-hand sides and solutions of the three equations)
+void step_fp64(DenseKin &d, const DevMesh &m, const double *vel, double *vort, const double urBw[3], hipStream_t s,
+               bool save) {
   const int nb = d.nb, nn = d.nn;
   const unsigned rb = ceil_div(nb, kRowsPerBlock);
   const int ksh = split_for(nb, nb), ksd = split_for(nb, nn);
   const SdDev &M = d.mats;
+  if (save) {
+    hipLaunchKernelGGL(copy_kernel, dim3(ceil_div((size_t)nn * 3, 256)), dim3(256), 0, s, (size_t)nn * 3, vel,
+                       d.vel_a.p);
+    hipLaunchKernelGGL(copy_kernel, dim3(ceil_div((size_t)nn * 3, 256)), dim3(256), 0, s, (size_t)nn * 3, vort,
+                       d.vort_a.p);
+  }
   hipLaunchKernelGGL(prod_h_kernel, dim3(rb, ksh), dim3(kRowsPerBlock), 0, s, nb, nn, ksh, M.H.p, M.Htx.p, M.Hty.p,
                      M.Htz.p, vel, m.gbn.p, d.part.p);
   TT_LAUNCH_CHECK();
@@ -295,12 +569,24 @@ void dense_bw_step(DenseKin &d, const DevMesh &m, const double *vel, double *vor
   hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div((size_t)18 * nb, 256)), dim3(256), 0, s, nb, 18, ksd,
                      d.part.p, d.YD.p, 0);
   TT_LAUNCH_CHECK();
+  if (save) {
+    hipLaunchKernelGGL(copy_kernel, dim3(ceil_div((size_t)12 * nb, 256)), dim3(256), 0, s, (size_t)12 * nb, d.YH.p,
+                       d.YH_a.p);
+    hipLaunchKernelGGL(copy_kernel, dim3(ceil_div((size_t)18 * nb, 256)), dim3(256), 0, s, (size_t)18 * nb, d.YD.p,
+                       d.YD_a.p);
+  }
   const double *Dm[3] = {M.Dx.p, M.Dy.p, M.Dz.p};
   for (int e = 0; e < 3; ++e) {
     hipLaunchKernelGGL(combine_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, e, d.YH.p, d.YD.p, m.nx.p,
                        m.ny.p, m.nz.p, d.kode.p, m.gbn.p, vort, d.dfi.p);
     hipLaunchKernelGGL(gemv_kernel, dim3(rb, ksh), dim3(kRowsPerBlock), 0, s, nb, ksh, d.Ainv[e].p, d.dfi.p, d.part.p);
     hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, 1, ksh, d.part.p, d.xs.p, 0);
+    if (save) {
+      hipLaunchKernelGGL(copy_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, (size_t)nb, d.dfi.p,
+                         d.dfi_a.p + (size_t)nb * e);
+      hipLaunchKernelGGL(copy_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, (size_t)nb, d.xs.p,
+                         d.sol_a.p + (size_t)nb * e);
+    }
     hipLaunchKernelGGL(relax_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, e, urBw[e], d.xs.p, d.kode.p,
                        m.gbn.p, vort, d.dw.p);
     TT_LAUNCH_CHECK();
@@ -315,6 +601,117 @@ void dense_bw_step(DenseKin &d, const DevMesh &m, const double *vel, double *vor
       TT_LAUNCH_CHECK();
     }
   }
+}
+
+// delta step on the reduced-precision copies, relative to the last anchor
+template <Lo L>
+void step_delta(DenseKin &d, const DevMesh &m, const double *vel, double *vort, const double urBw[3], hipStream_t s) {
+  using T = typename Load4<L>::T;
+  auto lo = [&](int k) -> const T * {
+    if constexpr (L == Lo::F32)
+      return d.lo32[k].p;
+    else
+      return d.lo16[k].p;
+  };
+  const int nb = d.nb, nn = d.nn, ldl = d.ldl;
+  const unsigned rb = ceil_div(nb, kLoBlockRows);
+  const int ksh = lo_split(nb, nb), ksd = lo_split(nb, nn);
+  const float *sc = d.rscale.p;
+  hipLaunchKernelGGL((lprod_h_kernel<L>), dim3(rb, ksh), dim3(64), 0, s, nb, nn, ldl, ksh, lo(0), lo(1), lo(2), lo(3),
+                     sc, vel, d.vel_a.p, m.gbn.p, d.part.p);
+  TT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(reduce_base_kernel, dim3(ceil_div((size_t)12 * nb, 256)), dim3(256), 0, s, nb, 12, ksh, d.part.p,
+                     d.YH_a.p, d.YH.p);
+  hipLaunchKernelGGL((lprod_d_kernel<L>), dim3(rb, ksd), dim3(64), 0, s, nb, nn, ldl, ksd, lo(4), lo(5), lo(6), sc,
+                     vort, d.vort_a.p, d.isbnd.p, d.part.p);
+  TT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(reduce_base_kernel, dim3(ceil_div((size_t)18 * nb, 256)), dim3(256), 0, s, nb, 18, ksd, d.part.p,
+                     d.YD_a.p, d.YD.p);
+  for (int e = 0; e < 3; ++e) {
+    hipLaunchKernelGGL(combine_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, e, d.YH.p, d.YD.p, m.nx.p,
+                       m.ny.p, m.nz.p, d.kode.p, m.gbn.p, vort, d.dfi.p);
+    hipLaunchKernelGGL((lgemv_kernel<L>), dim3(rb, ksh), dim3(64), 0, s, nb, ldl, ksh, lo(7 + e),
+                       sc + (size_t)(7 + e) * ldl, (const double *)d.dfi.p, (const double *)(d.dfi_a.p + (size_t)nb * e),
+                       d.part.p);
+    hipLaunchKernelGGL(reduce_base_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, 1, ksh, d.part.p,
+                       d.sol_a.p + (size_t)nb * e, d.xs.p);
+    hipLaunchKernelGGL(relax_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, e, urBw[e], d.xs.p, d.kode.p,
+                       m.gbn.p, vort, d.dw.p);
+    TT_LAUNCH_CHECK();
+    if (e < 2) {
+      const int nm = 2 - e;
+      hipLaunchKernelGGL((ldcorr_kernel<L>), dim3(rb, ksh), dim3(64), 0, s, nb, ldl, ksh, nm, lo(4 + e + 1),
+                         nm > 1 ? lo(4 + e + 2) : lo(4 + e + 1), sc + (size_t)(4 + e + 1) * ldl,
+                         sc + (size_t)(4 + (nm > 1 ? e + 2 : e + 1)) * ldl, m.gbn.p, (const double *)d.dw.p, d.part.p);
+      hipLaunchKernelGGL(add_slots_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nm, ksh, d.part.p,
+                         (e + 1) * 6 + e * 2, (e + 2) * 6 + e * 2, d.YD.p);
+      TT_LAUNCH_CHECK();
+    }
+  }
+}
+
+}  // namespace
+
+void dense_bw_step(DenseKin &d, const DevMesh &m, const double *vel, double *vort, const double urBw[3],
+                   hipStream_t s) {
+  if (d.prec == SdPrec::F64) {
+    step_fp64(d, m, vel, vort, urBw, s, false);
+    return;
+  }
+  if (d.since_anchor == 0)
+    step_fp64(d, m, vel, vort, urBw, s, true);
+  else if (d.prec == SdPrec::F32)
+    step_delta<Lo::F32>(d, m, vel, vort, urBw, s);
+  else if (d.prec == SdPrec::F16)
+    step_delta<Lo::F16>(d, m, vel, vort, urBw, s);
+  else
+    step_delta<Lo::BF16>(d, m, vel, vort, urBw, s);
+  d.since_anchor = (d.since_anchor + 1) % std::max(1, d.anchor_period);
+}
+
+void dense_set_precision(DenseKin &d, SdPrec prec, hipStream_t s) {
+  d.since_anchor = 0;
+  if (prec == d.prec) return;
+  for (auto &b : d.lo16) b.release();
+  for (auto &b : d.lo32) b.release();
+  d.prec = prec;
+  if (prec == SdPrec::F64) return;
+  const int nb = d.nb, nn = d.nn;
+  d.ldl = (nb + 7) / 8 * 8;
+  d.rscale.alloc((size_t)10 * d.ldl);
+  const SdDev &M = d.mats;
+  const double *mats[10] = {M.H.p, M.Htx.p, M.Hty.p, M.Htz.p, M.Dx.p, M.Dy.p, M.Dz.p, d.Ainv[0].p, d.Ainv[1].p,
+                            d.Ainv[2].p};
+  const int ncol[10] = {nb, nb, nb, nb, nn, nn, nn, nb, nb, nb};
+  const int ks = 64;
+  DBuf<float> mx((size_t)ks * nb);
+  for (int k = 0; k < 10; ++k) {
+    const size_t total = (size_t)d.ldl * ncol[k];
+    hipLaunchKernelGGL(rowmax_kernel, dim3(ceil_div(nb, 256), ks), dim3(256), 0, s, nb, ncol[k], ks, mats[k], mx.p);
+    hipLaunchKernelGGL(rowscale_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, ks, mx.p,
+                       d.rscale.p + (size_t)k * d.ldl);
+    if (prec == SdPrec::F32) {
+      d.lo32[k].alloc(total);
+      hipLaunchKernelGGL(to_low_kernel<Lo::F32>, dim3(ceil_div(total, 256)), dim3(256), 0, s, nb, ncol[k], d.ldl,
+                         mats[k], d.rscale.p + (size_t)k * d.ldl, d.lo32[k].p);
+    } else {
+      d.lo16[k].alloc(total);
+      if (prec == SdPrec::F16)
+        hipLaunchKernelGGL(to_low_kernel<Lo::F16>, dim3(ceil_div(total, 256)), dim3(256), 0, s, nb, ncol[k], d.ldl,
+                           mats[k], d.rscale.p + (size_t)k * d.ldl, d.lo16[k].p);
+      else
+        hipLaunchKernelGGL(to_low_kernel<Lo::BF16>, dim3(ceil_div(total, 256)), dim3(256), 0, s, nb, ncol[k], d.ldl,
+                           mats[k], d.rscale.p + (size_t)k * d.ldl, d.lo16[k].p);
+    }
+    TT_LAUNCH_CHECK();
+  }
+  TT_HIP(hipStreamSynchronize(s));  // mx is released on return
+  d.YH_a.alloc((size_t)12 * nb), d.YD_a.alloc((size_t)18 * nb);
+  d.dfi_a.alloc((size_t)3 * nb), d.sol_a.alloc((size_t)3 * nb);
+  d.vel_a.alloc((size_t)3 * nn), d.vort_a.alloc((size_t)3 * nn);
+  const int need = std::max({lo_split(nb, nn) * 18, lo_split(nb, nb) * 12, split_for(nb, nn) * 18,
+                             split_for(nb, nb) * 12});
+  if ((size_t)need * nb > d.part.n) d.part.alloc((size_t)need * nb);
 }
 
 }  // namespace tt
