@@ -9,7 +9,9 @@
 //           k-1), then u' <- A v - alpha u, ||u'||^2 partial and P <- A^T u'
 //           for its cell.  The block is read once from HBM with 16-byte loads
 //           that are all in flight together, staged in LDS and used for both
-//           products.  Block 0 stores the updated scalars.
+//           products (FP64 arithmetic on FP64 blocks; FP32 arithmetic with two
+//           accumulators on the FP32 blocks of the mixed mode, whose solves are
+//           anchored by an FP64 initial residual).  Block 0 stores the scalars.
 //   lsqr_B (one thread per unknown): reduces the cell partials to beta,
 //           finishes the x/w update of the previous iteration, gathers
 //           A^T u' from the cells of its column (fixed-width index table,
@@ -193,6 +195,8 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
   constexpr int S = (R * L + 3) / 4 * 4;
   __shared__ __align__(16) T blk[S];
   __shared__ double vs[64], uu[64], sh[1];
+  __shared__ float vsf[64], uuf[64];
+  constexpr bool f32 = sizeof(T) == 4;  // FP32 blocks: FP32 products, two accumulators
   const int t = threadIdx.x;
   if (blockIdx.x == 0) {
     const double nvs = reduce_partials<64>(pb1, nb, sh);
@@ -219,7 +223,13 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
   double acc = 0.0;
   for (;;) {
     rb.store(blk);
-    vs[t] = (t < L && j >= 0) ? (alpha > 0.0 ? vj / alpha : vj) : 0.0;
+    {
+      const double vn = (t < L && j >= 0) ? (alpha > 0.0 ? vj / alpha : vj) : 0.0;
+      if constexpr (f32)
+        vsf[t] = (float)vn;
+      else
+        vs[t] = vn;
+    }
     __syncthreads();
     const int cn = c + G;
     int jn = -1;
@@ -233,21 +243,46 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
     {
       double s = 0.0;
       const int row = t < R ? t : R - 1;
+      if constexpr (f32) {
+        float f0 = 0.f, f1 = 0.f;
 #pragma unroll
-      for (int l = 0; l < L; ++l) s += (double)blk[row * L + l] * vs[l];
+        for (int l = 0; l + 1 < L; l += 2) {
+          f0 += (float)blk[row * L + l] * vsf[l];
+          f1 += (float)blk[row * L + l + 1] * vsf[l + 1];
+        }
+        if (L & 1) f0 += (float)blk[row * L + L - 1] * vsf[L - 1];
+        s = (double)f0 + (double)f1;
+      } else {
+#pragma unroll
+        for (int l = 0; l < L; ++l) s += (double)blk[row * L + l] * vs[l];
+      }
       const double uold = beta > 0.0 ? uo / beta : uo;
       if (t < R) {
         un = -alpha * uold + s;
         u[(size_t)c * R + t] = un;
       }
     }
-    uu[t] = un;
+    if constexpr (f32)
+      uuf[t] = (float)un;
+    else
+      uu[t] = un;
     __syncthreads();
     {
       double s = 0.0;
       const int col = t < L ? t : L - 1;
+      if constexpr (f32) {
+        float f0 = 0.f, f1 = 0.f;
 #pragma unroll
-      for (int i = 0; i < R; ++i) s += (double)blk[i * L + col] * uu[i];
+        for (int i = 0; i + 1 < R; i += 2) {
+          f0 += (float)blk[i * L + col] * uuf[i];
+          f1 += (float)blk[(i + 1) * L + col] * uuf[i + 1];
+        }
+        if (R & 1) f0 += (float)blk[(R - 1) * L + col] * uuf[R - 1];
+        s = (double)f0 + (double)f1;
+      } else {
+#pragma unroll
+        for (int i = 0; i < R; ++i) s += (double)blk[i * L + col] * uu[i];
+      }
       if (t < L) P[(size_t)c * L + t] = s;
     }
     acc += un * un;
