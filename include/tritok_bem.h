@@ -68,6 +68,20 @@ int tt_macro_integrals(const tt_mesh_view *m, const tt_quad *q, int32_t copy, do
                        double *B, double *Abdx, double *Abdy, double *Abdz, double *Htx,
                        double *Hty, double *Htz, double *Dx, double *Dy, double *Dz, double ont[11]);
 
+/* Replaces FMATmodHelm (lib_mHelm.f:2-483) as called by corMin_modHelm
+ * (lib_mHelm.f:1400-1502): the same matrices as tt_macro_integrals for the
+ * modified-Helmholtz equation lap u - mu^2 u = f, kernel exp(-mu r)/(4 pi r)
+ * (INTEDc27modHelm :1637-2033, INTEBc9modHelm :2034-2348).  Each source
+ * point has mu = sqrt(beta / diff): diff (nnodes) = mHdiff for node sources,
+ * qdiff (nq) = mHqDiff for flux sources; beta = inp%beta.  The free term is
+ * c = -sum H + mu^2 sum B and the linear-field corrections use H - mu^2 B.
+ * *mumax (may be NULL) = the largest mu, as written to tri.modHelm.macro.int.
+ * Errors: diff or qdiff <= 0 (with beta > 0), beta < 0. */
+int tt_macro_integrals_mh(const tt_mesh_view *m, const tt_quad *q, int32_t copy, const double *diff,
+                          const double *qdiff, double beta, double *H, double *G, double *B, double *Abdx,
+                          double *Abdy, double *Abdz, double *Htx, double *Hty, double *Htz, double *Dx,
+                          double *Dy, double *Dz, double ont[11], double *mumax);
+
 /* Replaces FMATsdkm (lib_sdkm.f:1286-1530) as called by sdKm_corInt
  * (lib_sdkm.f:2377-2449) for boundary rows row0..row1-1 (0-based; the
  * reference's zac:kon).  Host outputs with leading dimension row1-row0:

@@ -10,6 +10,7 @@
  *   idc2ibc          lib_interpol.f:663-700, FindMinEdge lib_int.f:2450-2488
  *   FMATkmdcSteady   lib_fmat.f:3-446
  *   FMATsdkm         lib_sdkm.f:1286-1530
+ *   INTEDc27modHelm, INTEBc9modHelm, FMATmodHelm  lib_mHelm.f:1637-2033, 2034-2348, 2-483
  */
 #include "tt_oracle.h"
 
@@ -144,8 +145,18 @@ static const double k_zet51[51] = {
 static const double k_ksi13[13] = {0.0, 0.5, 1.0, 1.0, 1.0, 0.5, 0.0, 0.0, 0.5, 0.125, 0.875, 0.875, 0.125};
 static const double k_eta13[13] = {0.0, 0.0, 0.0, 0.5, 1.0, 1.0, 1.0, 0.5, 0.5, 0.125, 0.125, 0.875, 0.875};
 
-long or_inted27(const or_quad *q, const double p[3], const double c[8][3], int isrc, double *be,
-                double *dxe, double *dye, double *dze) {
+/* Kernel factors at distance ra: fung = u*, gfac multiplies the gradient.
+ * Laplace (mu = 0): u* = 1/(4 pi r).  Modified Helmholtz (INTEDc27modHelm
+ * lib_mHelm.f:1905-1910, INTEBc9modHelm :2227-2233): u* = exp(-mu r)/(4 pi r)
+ * and the gradient carries (1 + mu r).  With mu = 0 both factors are exactly
+ * 1, so the two kernels coincide bit for bit. */
+static inline double kern_g(double PI, double ra, double mu) {
+  return mu != 0.0 ? 0.25 / (PI * ra) * exp(-mu * ra) : 0.25 / (PI * ra);
+}
+static inline double kern_f(double ra, double mu) { return mu != 0.0 ? 1 + mu * ra : 1.0; }
+
+static long inted27_k(const or_quad *q, const double p[3], const double c[8][3], int isrc, double mu,
+                      double *be, double *dxe, double *dye, double *dze) {
   double gis[64], oms[64], gir[64], omr[64];
   int ns = quad_rule(q, q->kmDs, gis, oms);
   int nr = quad_rule(q, q->kmDr, gir, omr);
@@ -279,10 +290,10 @@ long or_inted27(const or_quad *q, const double p[3], const double c[8][3], int i
                       if (ra == 0.0) continue;
                       if (ra < 1.0e-12) ra = 1.0e-12;
                       double ira2 = 1.0 / (ra * ra);
-                      double fung = 0.25 / (PI * ra);
-                      double funx = (p[0] - xc[0]) * ira2 * fung;
-                      double funy = (p[1] - xc[1]) * ira2 * fung;
-                      double funz = (p[2] - xc[2]) * ira2 * fung;
+                      double fung = kern_g(PI, ra, mu), gf = kern_f(ra, mu);
+                      double funx = (p[0] - xc[0]) * ira2 * fung * gf;
+                      double funy = (p[1] - xc[1]) * ira2 * fung * gf;
+                      double funz = (p[2] - xc[2]) * ira2 * fung * gf;
                       double f27[27];
                       or_shape27(eti, etj, etk, f27);
                       for (int kk = 0; kk < 27; ++kk) {
@@ -342,10 +353,10 @@ long or_inted27(const or_quad *q, const double p[3], const double c[8][3], int i
                 ++npts;
                 if (ra < 1.0e-12) ra = 1.0e-12;
                 double ira2 = 1.0 / (ra * ra);
-                double fung = 0.25 / (PI * ra);
-                double funx = (p[0] - xc[0]) * ira2 * fung;
-                double funy = (p[1] - xc[1]) * ira2 * fung;
-                double funz = (p[2] - xc[2]) * ira2 * fung;
+                double fung = kern_g(PI, ra, mu), gf = kern_f(ra, mu);
+                double funx = (p[0] - xc[0]) * ira2 * fung * gf;
+                double funy = (p[1] - xc[1]) * ira2 * fung * gf;
+                double funz = (p[2] - xc[2]) * ira2 * fung * gf;
                 double f27[27];
                 or_shape27(gii, gij, gik, f27);
                 for (int kk = 0; kk < 27; ++kk) {
@@ -363,11 +374,21 @@ long or_inted27(const or_quad *q, const double p[3], const double c[8][3], int i
   return npts;
 }
 
+long or_inted27(const or_quad *q, const double p[3], const double c[8][3], int isrc, double *be,
+                double *dxe, double *dye, double *dze) {
+  return inted27_k(q, p, c, isrc, 0.0, be, dxe, dye, dze);
+}
+
+long or_inted27_mh(const or_quad *q, const double p[3], const double c[8][3], int isrc, double mu,
+                   double *be, double *dxe, double *dye, double *dze) {
+  return inted27_k(q, p, c, isrc, mu, be, dxe, dye, dze);
+}
+
 /* one boundary-face kernel evaluation (shared by the singular and regular
  * branches of INTEBc9kmdclap; the two only differ in how eti/etj/weight are
  * formed) */
 static inline void bnd_point(int macro, const double p[3], const double c[4][3], double eti, double etj,
-                             double wpre, double *he, double *ge, double *adx, double *ady,
+                             double wpre, double mu, double *he, double *ge, double *adx, double *ady,
                              double *adz, double *heyz, double *hezx, double *hexy) {
   const double PI = 2.0 * asin(1.0);
   double e1m = 1.0 - eti, e1p = 1.0 + eti, e2m = 1.0 - etj, e2p = 1.0 + etj;
@@ -390,10 +411,10 @@ static inline void bnd_point(int macro, const double p[3], const double c[4][3],
   double rx = p[0] - xc0, ry = p[1] - yc0, rz = p[2] - zc0;
   double ra = sqrt(rx * rx + ry * ry + rz * rz);
   double ira2 = 1.0 / (ra * ra);
-  double fung = 0.25 / (PI * ra);
-  double funx = (p[0] - xc0) * ira2 * fung;
-  double funy = (p[1] - yc0) * ira2 * fung;
-  double funz = (p[2] - zc0) * ira2 * fung;
+  double fung = kern_g(PI, ra, mu), gf = kern_f(ra, mu);
+  double funx = (p[0] - xc0) * ira2 * fung * gf;
+  double funy = (p[1] - yc0) * ira2 * fung * gf;
+  double funz = (p[2] - zc0) * ira2 * fung * gf;
   double funh = funx * anx1 + funy * any1 + funz * anz1;
   double f9[9];
   or_shape9(eti, etj, f9);
@@ -419,9 +440,9 @@ static inline void bnd_point(int macro, const double p[3], const double c[4][3],
   }
 }
 
-long or_inteb9(const or_quad *q, int macro, const double p[3], const double c[4][3], int isrc,
-               double minedge, double *he, double *ge, double *adx, double *ady, double *adz,
-               double *heyz, double *hezx, double *hexy) {
+static long inteb9_k(const or_quad *q, int macro, const double p[3], const double c[4][3], int isrc,
+                     double minedge, double mu, double *he, double *ge, double *adx, double *ady, double *adz,
+                     double *heyz, double *hezx, double *hexy) {
   double gis[64], oms[64], gir[64], omr[64];
   int ns = quad_rule(q, q->kmBs, gis, oms);
   int nr = quad_rule(q, q->kmBr, gir, omr);
@@ -501,7 +522,7 @@ long or_inteb9(const or_quad *q, int macro, const double p[3], const double c[4]
               etj = cc + 0.5 * (etj + 1.0) * (d - cc);
               /* AJAC=dex*(TH1-TH0)*ROTH*RO*OME(I)*OME(J)/AJAC (lib_int.f:1645) */
               double wpre = dex * (th1 - th0) * roth * ro * oms[I] * oms[J];
-              bnd_point(macro, p, c, eti, etj, wpre, he, ge, adx, ady, adz, heyz, hezx, hexy);
+              bnd_point(macro, p, c, eti, etj, wpre, mu, he, ge, adx, ady, adz, heyz, hezx, hexy);
               ++npts;
             }
         }
@@ -512,13 +533,25 @@ long or_inteb9(const or_quad *q, int macro, const double p[3], const double c[4]
             double gij = cc + 0.5 * (gir[J] + 1.0) * (d - cc);
             /* AJAC=dex*OME(I)*OME(J)/AJAC (lib_int.f:1733) */
             double wpre = dex * omr[I] * omr[J];
-            bnd_point(macro, p, c, gii, gij, wpre, he, ge, adx, ady, adz, heyz, hezx, hexy);
+            bnd_point(macro, p, c, gii, gij, wpre, mu, he, ge, adx, ady, adz, heyz, hezx, hexy);
             ++npts;
           }
         }
       }
     }
   return npts;
+}
+
+long or_inteb9(const or_quad *q, int macro, const double p[3], const double c[4][3], int isrc,
+               double minedge, double *he, double *ge, double *adx, double *ady, double *adz,
+               double *heyz, double *hezx, double *hexy) {
+  return inteb9_k(q, macro, p, c, isrc, minedge, 0.0, he, ge, adx, ady, adz, heyz, hezx, hexy);
+}
+
+long or_inteb9_mh(const or_quad *q, const double p[3], const double c[4][3], int isrc, double minedge,
+                  double mu, double *he, double *ge, double *adx, double *ady, double *adz, double *heyz,
+                  double *hezx, double *hexy) {
+  return inteb9_k(q, 1, p, c, isrc, minedge, mu, he, ge, adx, ady, adz, heyz, hezx, hexy);
 }
 
 /* idc2ibc local permutations (lib_interpol.f:683-688), 1-based local nodes */
@@ -542,11 +575,18 @@ static int find_col(int nc, const int *idc, int ic, int node) {
   return -1;
 }
 
-void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *x, const int *idc,
-                        const int *ibf, const double *xq, const int *flpm, int copy, double *H,
-                        double *G, double *B, double *Abdx, double *Abdy, double *Abdz, double *Htx,
-                        double *Hty, double *Htz, double *Dx, double *Dy, double *Dz, double ont[11],
-                        int nthreads) {
+/* FMATkmdcSteady (mun == NULL) and FMATmodHelm (mun/muq: mu of every node /
+ * flux node as source).  The two differ in the kernel, in the free term
+ * (c = -sum H + mu^2 sum B, lib_mHelm.f:343-357) and in the linear-field
+ * correction ((H - mu^2 B) x, :421-443); FMATmodHelm applies its corrections
+ * after the domain integrals (:318-458), FMATkmdcSteady right after the
+ * boundary integrals -- the results do not depend on that order. */
+static void macro_k(const or_quad *q, int nn, int nc, int nq, const double *x, const int *idc,
+                    const int *ibf, const double *xq, const int *flpm, int copy, const double *mun,
+                    const double *muq, double *H, double *G, double *B, double *Abdx, double *Abdy,
+                    double *Abdz, double *Htx, double *Hty, double *Htz, double *Dx, double *Dy, double *Dz,
+                    double ont[11], double *mumax, int nthreads) {
+  const int helm = mun != NULL;
   const size_t m1 = (size_t)nc * 51, m2 = (size_t)nc * 27;
 #define M1(A, r, c) A[(r) + m1 * (c)]
 #define M2(A, r, c) A[(r) + m2 * (c)]
@@ -564,12 +604,12 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
   memset(Htz, 0, sizeof(double) * m2 * 27);
   int ncomp = copy ? 1 : nc;
   /* per-cell accuracy estimates: [ontH, ontG, Htx, Hty, Htz, Dx, Dy, Dz, HtyDx, HtzDy, HtxDz] */
-  double *ontc = (double *)calloc((size_t)ncomp * 11, sizeof(double));
+  double *ontc = (double *)calloc((size_t)ncomp * 12, sizeof(double));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
 #endif
   for (int ic = 0; ic < ncomp; ++ic) {
-    double *oc = ontc + (size_t)ic * 11;
+    double *oc = ontc + (size_t)ic * 12;
     /* ibc for this cell and the shortest edge */
     int ibcl[6][9];
     for (int s = 0; s < 6; ++s)
@@ -590,6 +630,8 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
       int dn = IDC(ic, it);
       double p[3] = {XC(dn, 0), XC(dn, 1), XC(dn, 2)};
       size_t i = (size_t)ic * 51 + (it - 1), ii = (size_t)ic * 27 + (it - 1);
+      double mu = helm ? mun[dn - 1] : 0.0;
+      if (mu > oc[11]) oc[11] = mu;
       for (int je = 0; je < 6; ++je) {
         double cf[4][3];
         for (int k = 0; k < 4; ++k)
@@ -597,7 +639,7 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
         int isrc = 0;
         for (int s = 0; s < 9; ++s)
           if (dn == ibcl[je][s]) isrc = s + 1;
-        or_inteb9(q, 1, p, cf, isrc, minedge, he, ge, abx, aby, abz, htx, hty, htz);
+        inteb9_k(q, 1, p, cf, isrc, minedge, mu, he, ge, abx, aby, abz, htx, hty, htz);
         for (int s = 0; s < 9; ++s) {
           int col = find_col(nc, idc, ic, ibcl[je][s]) - 1;
           M1(H, i, col) = M1(H, i, col) + he[s];
@@ -613,6 +655,7 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
           M1(G, i, col) = M1(G, i, col) + ge[s] * (double)flpm[ic + (size_t)nc * je];
         }
       }
+      if (helm) continue; /* free term after the domain integrals */
       double cfree = 0.0;
       for (int j = 0; j < 27; ++j) cfree = cfree - M1(H, i, j);
       int col = find_col(nc, idc, ic, dn) - 1;
@@ -632,12 +675,14 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
         int dn = ibf[ic + (size_t)nc * (its + 6 * (itf - 1))];
         double p[3] = {xq[(dn - 1)], xq[(dn - 1) + (size_t)nq], xq[(dn - 1) + 2 * (size_t)nq]};
         size_t i = (size_t)ic * 51 + 27 + its * 4 + (itf - 1);
+        double mu = helm ? muq[dn - 1] : 0.0;
+        if (mu > oc[11]) oc[11] = mu;
         for (int je = 0; je < 6; ++je) {
           double cf[4][3];
           for (int k = 0; k < 4; ++k)
             for (int d = 0; d < 3; ++d) cf[k][d] = XC(ibcl[je][2 * k], d);
           int isrc = (its == je) ? itf + 9 : 0;
-          or_inteb9(q, 1, p, cf, isrc, minedge, he, ge, abx, aby, abz, htx, hty, htz);
+          inteb9_k(q, 1, p, cf, isrc, minedge, mu, he, ge, abx, aby, abz, htx, hty, htz);
           for (int s = 0; s < 9; ++s) {
             int col = find_col(nc, idc, ic, ibcl[je][s]) - 1;
             M1(H, i, col) = M1(H, i, col) + he[s];
@@ -650,6 +695,7 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
             M1(G, i, col) = M1(G, i, col) + ge[s] * (double)flpm[ic + (size_t)nc * je];
           }
         }
+        if (helm) continue;
         double cfree = 0.0;
         for (int j = 0; j < 27; ++j) cfree = cfree - M1(H, i, j);
         oc[1] += fabs(0.5 - cfree);
@@ -671,13 +717,14 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
       int dn = IDC(ic, it);
       double p[3] = {XC(dn, 0), XC(dn, 1), XC(dn, 2)};
       size_t i = (size_t)ic * 51 + (it - 1), ii = (size_t)ic * 27 + (it - 1);
-      or_inted27(q, p, cc8, it, be, dxe, dye, dze);
+      inted27_k(q, p, cc8, it, helm ? mun[dn - 1] : 0.0, be, dxe, dye, dze);
       for (int s = 0; s < 27; ++s) {
         M1(B, i, s) = be[s];
         M1(Dx, i, s) = dxe[s];
         M1(Dy, i, s) = dye[s];
         M1(Dz, i, s) = dze[s];
       }
+      if (helm) continue; /* corrections after all domain integrals */
       /* linear-field corrections: (coordinate used, D matrix, ont slot) */
       double *Dm[3] = {Dz, Dx, Dy};
       int crd[3] = {2, 0, 1}, slot[3] = {7, 5, 6};
@@ -706,7 +753,7 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
         int dn = ibf[ic + (size_t)nc * (its + 6 * (itf - 1))];
         double p[3] = {xq[(dn - 1)], xq[(dn - 1) + (size_t)nq], xq[(dn - 1) + 2 * (size_t)nq]};
         size_t i = (size_t)ic * 51 + 27 + its * 4 + (itf - 1);
-        or_inted27(q, p, cc8, 27 + its * 4 + itf, be, dxe, dye, dze);
+        inted27_k(q, p, cc8, 27 + its * 4 + itf, helm ? muq[dn - 1] : 0.0, be, dxe, dye, dze);
         for (int s = 0; s < 27; ++s) {
           M1(B, i, s) = be[s];
           M1(Dx, i, s) = dxe[s];
@@ -714,6 +761,74 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
           M1(Dz, i, s) = dze[s];
         }
       }
+    if (helm) {
+      /* ---- free terms and Ht row sums (lib_mHelm.f:318-404) ---- */
+      double cfree = 0.0;
+      for (int it = 1; it <= 27; ++it) {
+        int dn = IDC(ic, it);
+        double mu = mun[dn - 1];
+        size_t i = (size_t)ic * 51 + (it - 1), ii = (size_t)ic * 27 + (it - 1);
+        double vH = 0.0, vB = 0.0;
+        for (int j = 0; j < 27; ++j) vH = vH + M1(H, i, j);
+        for (int j = 0; j < 27; ++j) vB = vB + M1(B, i, j);
+        cfree = -vH + mu * mu * vB;
+        int col = find_col(nc, idc, ic, dn) - 1;
+        M1(H, i, col) = M1(H, i, col) + cfree;
+        double *Ht[3] = {Htx, Hty, Htz};
+        for (int t = 0; t < 3; ++t) {
+          double vs = 0.0;
+          for (int j = 0; j < 27; ++j) vs = vs + M2(Ht[t], ii, j);
+          if (fabs(vs) > oc[2 + t]) oc[2 + t] = fabs(vs);
+          M2(Ht[t], ii, it - 1) = M2(Ht[t], ii, it - 1) - vs;
+        }
+      }
+      oc[0] = fabs(1.0 - cfree);
+      for (int its = 0; its < 6; ++its)
+        for (int itf = 1; itf <= 4; ++itf) {
+          int dn = ibf[ic + (size_t)nc * (its + 6 * (itf - 1))];
+          double mu = muq[dn - 1];
+          size_t i = (size_t)ic * 51 + 27 + its * 4 + (itf - 1);
+          double vH = 0.0, vB = 0.0;
+          for (int j = 0; j < 27; ++j) vH = vH + M1(H, i, j);
+          for (int j = 0; j < 27; ++j) vB = vB + M1(B, i, j);
+          double c = -vH + mu * mu * vB;
+          oc[1] += fabs(0.5 - c);
+          const double tc = 3.0 / 4.0;
+          double fi[9];
+          or_shape9((itf == 1 || itf == 4) ? -tc : tc, (itf <= 2) ? -tc : tc, fi);
+          for (int s = 0; s < 9; ++s) {
+            int col = find_col(nc, idc, ic, ibcl[its][s]) - 1;
+            M1(H, i, col) = M1(H, i, col) + c * fi[s];
+          }
+        }
+      /* ---- linear-field corrections with H - mu^2 B (lib_mHelm.f:409-458) ---- */
+      for (int it = 1; it <= 27; ++it) {
+        int dn = IDC(ic, it);
+        double mu = mun[dn - 1];
+        size_t i = (size_t)ic * 51 + (it - 1), ii = (size_t)ic * 27 + (it - 1);
+        double *Dm[3] = {Dz, Dx, Dy};
+        int crd[3] = {2, 0, 1}, slot[3] = {7, 5, 6};
+        for (int t = 0; t < 3; ++t) {
+          double hv = 0.0, vs = 0.0;
+          for (int j = 0; j < 27; ++j) {
+            hv = hv + (M1(H, i, j) - mu * mu * M1(B, i, j)) * XC(IDC(ic, j + 1), crd[t]);
+            vs = vs + M1(Dm[t], i, j);
+          }
+          if (fabs(hv - vs) > oc[slot[t]]) oc[slot[t]] = fabs(hv - vs);
+          M1(Dm[t], i, it - 1) = M1(Dm[t], i, it - 1) + hv - vs;
+        }
+        double hxy = 0.0, hyz = 0.0, hzx = 0.0;
+        for (int j = 0; j < 27; ++j) {
+          int nj = IDC(ic, j + 1);
+          hxy = hxy + M2(Htx, ii, j) * XC(nj, 1) + M1(Dz, i, j);
+          hyz = hyz + M2(Hty, ii, j) * XC(nj, 2) + M1(Dx, i, j);
+          hzx = hzx + M2(Htz, ii, j) * XC(nj, 0) + M1(Dy, i, j);
+        }
+        if (fabs(hyz) > oc[8]) oc[8] = fabs(hyz);
+        if (fabs(hzx) > oc[9]) oc[9] = fabs(hzx);
+        if (fabs(hxy) > oc[10]) oc[10] = fabs(hxy);
+      }
+    }
   }
   /* copy=1: replicate cell 1 (lib_fmat.f:89-120) */
   if (copy)
@@ -746,12 +861,12 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
       }
     }
   /* reduce the accuracy estimates in cell order (sums for H/G, max else) */
-  double o[11] = {0};
+  double o[12] = {0};
   for (int ic = 0; ic < ncomp; ++ic) {
-    double *oc = ontc + (size_t)ic * 11;
+    double *oc = ontc + (size_t)ic * 12;
     o[0] += oc[0];
     o[1] += oc[1];
-    for (int t = 2; t < 11; ++t)
+    for (int t = 2; t < 12; ++t)
       if (oc[t] > o[t]) o[t] = oc[t];
   }
   if (copy) {
@@ -762,9 +877,35 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
   }
   /* file order: ontH,ontG,ontHtx,ontHty,ontHtz,ontDx,ontDy,ontDz,ontHtyDx,ontHtzDy,ontHtxDz */
   for (int t = 0; t < 11; ++t) ont[t] = o[t];
+  if (mumax) *mumax = o[11];
   free(ontc);
 #undef M1
 #undef M2
+}
+
+void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *x, const int *idc,
+                        const int *ibf, const double *xq, const int *flpm, int copy, double *H,
+                        double *G, double *B, double *Abdx, double *Abdy, double *Abdz, double *Htx,
+                        double *Hty, double *Htz, double *Dx, double *Dy, double *Dz, double ont[11],
+                        int nthreads) {
+  macro_k(q, nn, nc, nq, x, idc, ibf, xq, flpm, copy, NULL, NULL, H, G, B, Abdx, Abdy, Abdz, Htx, Hty, Htz, Dx,
+          Dy, Dz, ont, NULL, nthreads);
+}
+
+void or_macro_integrals_mh(const or_quad *q, int nn, int nc, int nq, const double *x, const int *idc,
+                           const int *ibf, const double *xq, const int *flpm, int copy, const double *diff,
+                           const double *qdiff, double beta, double *H, double *G, double *B, double *Abdx,
+                           double *Abdy, double *Abdz, double *Htx, double *Hty, double *Htz, double *Dx,
+                           double *Dy, double *Dz, double ont[11], double *mumax, int nthreads) {
+  /* mu = SQRT(inp%beta/mHdiff(dn)) per source (lib_mHelm.f:139,197,279,303) */
+  double *mun = (double *)malloc(sizeof(double) * (size_t)nn);
+  double *muq = (double *)malloc(sizeof(double) * (size_t)nq);
+  for (int k = 0; k < nn; ++k) mun[k] = sqrt(beta / diff[k]);
+  for (int k = 0; k < nq; ++k) muq[k] = sqrt(beta / qdiff[k]);
+  macro_k(q, nn, nc, nq, x, idc, ibf, xq, flpm, copy, mun, muq, H, G, B, Abdx, Abdy, Abdz, Htx, Hty, Htz, Dx, Dy,
+          Dz, ont, mumax, nthreads);
+  free(mun);
+  free(muq);
 }
 
 /* -------------------------------------------------------------------------- */

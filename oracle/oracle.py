@@ -55,6 +55,11 @@ _inteb9 = _sig("or_inteb9", C.c_long, C.POINTER(OrQuad), C.c_int, PD, PD, C.c_in
                PD, PD, PD)
 _macro = _sig("or_macro_integrals", None, C.POINTER(OrQuad), C.c_int, C.c_int, C.c_int, PD, PI, PI, PD, PI, C.c_int,
               *([PD] * 12), PD, C.c_int)
+_macro_mh = _sig("or_macro_integrals_mh", None, C.POINTER(OrQuad), C.c_int, C.c_int, C.c_int, PD, PI, PI, PD, PI,
+                 C.c_int, PD, PD, C.c_double, *([PD] * 12), PD, PD, C.c_int)
+_inted27_mh = _sig("or_inted27_mh", C.c_long, C.POINTER(OrQuad), PD, PD, C.c_int, C.c_double, PD, PD, PD, PD)
+_inteb9_mh = _sig("or_inteb9_mh", C.c_long, C.POINTER(OrQuad), PD, PD, C.c_int, C.c_double, C.c_double, PD, PD, PD,
+                  PD, PD, PD, PD, PD)
 _sd = _sig("or_sd_integrals", None, C.POINTER(OrQuad), C.c_int, C.c_int, C.c_int, C.c_int, PD, PI, PI, PI, PI,
            C.c_int, C.c_int, C.c_int, *([PD] * 7), PD, C.c_int)
 _dnrm2 = _sig("or_dnrm2", C.c_double, C.c_int, PD)
@@ -145,6 +150,48 @@ def macro_integrals(q, m, copy=0, nthreads=8):
            _d(ont), nthreads)
     out["ont"] = ont
     return out
+
+
+def macro_integrals_mh(q, m, diff, qdiff, beta, copy=0, nthreads=8):
+    """FMATmodHelm: diff(nn), qdiff(nq) per-node diffusivities, mu = sqrt(beta/diff).
+    Returns the dict of macro_integrals plus 'mumax'."""
+    nn, nc, nq = m["nn"], m["nc"], m["nq"]
+    x = np.asfortranarray(m["x"])
+    idc = np.asfortranarray(m["idc"])
+    ibf = np.asfortranarray(m["ibf"])
+    xq = np.asfortranarray(m["xq"])
+    flpm = np.asfortranarray(m["flpm"])
+    diff = np.ascontiguousarray(diff, np.float64)
+    qdiff = np.ascontiguousarray(qdiff, np.float64)
+    assert diff.shape == (nn,) and qdiff.shape == (nq,)
+    m1, m2 = nc * 51, nc * 27
+    names = ["H", "G", "B", "Abdx", "Abdy", "Abdz", "Htx", "Hty", "Htz", "Dx", "Dy", "Dz"]
+    shp = {"G": (m1, 24), "Htx": (m2, 27), "Hty": (m2, 27), "Htz": (m2, 27)}
+    out = {k: np.zeros(shp.get(k, (m1, 27)), order="F") for k in names}
+    ont = np.zeros(11)
+    mumax = C.c_double()
+    _macro_mh(C.byref(q), nn, nc, nq, _d(x), _i(idc), _i(ibf), _d(xq), _i(flpm), int(copy), _d(diff), _d(qdiff),
+              float(beta), *[_d(out[k]) for k in names], _d(ont), C.byref(mumax), nthreads)
+    out["ont"] = ont
+    out["mumax"] = mumax.value
+    return out
+
+
+def inted27_mh(q, p, corners, isrc, mu):
+    p = np.ascontiguousarray(p, np.float64)
+    c = np.ascontiguousarray(corners, np.float64)
+    out = [np.zeros(27) for _ in range(4)]
+    n = _inted27_mh(C.byref(q), _d(p), _d(c), int(isrc), float(mu), *[_d(o) for o in out])
+    return (*out, n)
+
+
+def inteb9_mh(q, p, corners, isrc, mu, minedge=1.0):
+    p = np.ascontiguousarray(p, np.float64)
+    c = np.ascontiguousarray(corners, np.float64)
+    he, ge, ax, ay, az, hyz, hzx, hxy = [np.zeros(9) for _ in range(8)]
+    n = _inteb9_mh(C.byref(q), _d(p), _d(c), int(isrc), float(minedge), float(mu), _d(he), _d(ge), _d(ax), _d(ay),
+                   _d(az), _d(hyz), _d(hzx), _d(hxy))
+    return dict(he=he, ge=ge[:4], adx=ax, ady=ay, adz=az, heyz=hyz, hezx=hzx, hexy=hxy, n=n)
 
 
 def sd_integrals(q, m, row0=0, row1=None, nthreads=8):

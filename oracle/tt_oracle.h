@@ -28,6 +28,9 @@
  *   or_inted27, or_inteb9    reference binary matrices (tests/golden npz files),
  *                            rigid-body / linear-field identities, brute force.
  *   or_macro_integrals       reference tri.macro.int (dc2, nc2, jit2, dc8).
+ *   or_macro_integrals_mh    reference tri.modHelm.macro.int (uniform and
+ *                            variable mu); mu = 0 equals or_macro_integrals
+ *                            exactly; closed-form modified-Helmholtz fields.
  *   or_sd_integrals          reference tri.sdkm.int.1 (dc2, jit2, dc8 rows).
  *   or_lsqr2, or_dgefa/dgesl scipy lsqr / LAPACK getrf on the same systems; the
  *                            reference's per-solve iteration counts (tri.ite).
@@ -86,6 +89,23 @@ void or_macro_integrals(const or_quad *q, int nn, int nc, int nq, const double *
                         int copy, double *H, double *G, double *B, double *Abdx, double *Abdy,
                         double *Abdz, double *Htx, double *Hty, double *Htz, double *Dx,
                         double *Dy, double *Dz, double ont[11], int nthreads);
+
+/* Modified-Helmholtz variants (lib_mHelm.f): INTEDc27modHelm (:1637-2033)
+ * and INTEBc9modHelm (:2034-2348) -- kernel exp(-mu r)/(4 pi r), gradient
+ * scaled by (1 + mu r); FMATmodHelm (:2-483) with mu = sqrt(beta/diff) of each
+ * source, diff(nn) at nodes and qdiff(nq) at flux nodes; *mumax = the largest
+ * mu used.  mu = 0 (beta = 0) gives the Laplace results bit for bit. */
+long or_inted27_mh(const or_quad *q, const double p[3], const double corners[8][3], int isrc,
+                   double mu, double *be, double *dxe, double *dye, double *dze);
+long or_inteb9_mh(const or_quad *q, const double p[3], const double corners[4][3], int isrc,
+                  double minedge, double mu, double *he, double *ge, double *adx, double *ady,
+                  double *adz, double *heyz, double *hezx, double *hexy);
+void or_macro_integrals_mh(const or_quad *q, int nn, int nc, int nq, const double *x,
+                           const int *idc, const int *ibf, const double *xq, const int *flpm,
+                           int copy, const double *diff, const double *qdiff, double beta,
+                           double *H, double *G, double *B, double *Abdx, double *Abdy,
+                           double *Abdz, double *Htx, double *Hty, double *Htz, double *Dx,
+                           double *Dy, double *Dz, double ont[11], double *mumax, int nthreads);
 
 /* FMATsdkm (lib_sdkm.f:1286-1530) for boundary rows row0..row1-1 (0-based),
  * matrices column-major with leading dimension ld (>= row1-row0):

@@ -13,6 +13,11 @@ the reference, and converts its outputs into ``tests/golden/<case>.npz``:
 * ``tri.rst``        -> converged fields (WrstF, src/lib_IO.f:604-685);
 * ``tri.ite`` / ``tri.sta`` / ``tri.sol`` -> per-Picard-iteration solver
   iteration counts, relative changes and dynamic LSQR tolerances;
+* ``tri.modHelm.macro.int`` -> modified-Helmholtz macro integrals
+  (WritemHMacroIntDisk, src/lib_mHelm.f:1524-1576) of the mH cases, run with
+  EQNS 0 0 0 0 0 1 (imH = 1) and MHEL mu f: f = 1 gives diff = beta/mu^2 at every
+  node, f = 3 diff = beta/(1+xyz)^2 (src/main.f90:532-543); the node and flux-node
+  diffusivities fed to the reference are stored with the matrices;
 * ``gauss_ref.npz``  -> the reference's Gauss-Legendre table as its own
   GaussPosWeights (src/lib_int.f:2673-3173) fills it, written by the small
   Fortran harness dump_gauss.f90 linked against the reference's objects.
@@ -81,7 +86,31 @@ CASES = {
                  phys="REYN 1.0", idt=0, nnlit=10000, tight=True),
     "nc2t": dict(geo="kva2x2x2.geo", bic="NaturalConvection-X.bic", urdw=0.3, urdt=0.1,
                  phys="RAPR 1000.0 0.71\nREYN 1.0", idt=1, nnlit=10000, tight=True),
+    # modified-Helmholtz macro integrals: uniform mu = 20 on the 2^3 cube,
+    # mu = 1 + xyz on the jittered cube, uniform mu = 100 on the jittered cube
+    "mh2u": dict(geo="kva2x2x2.geo", mh=(20.0, 1)),
+    "mh2v": dict(geo="jit2x2x2.geo", mh=(1.0, 3), jitter=(0.1, 20251203)),
+    "mh2h": dict(geo="jit2x2x2.geo", mh=(100.0, 1), jitter=(0.1, 20251203)),
 }
+
+MH_INP = """MDIR ./
+LGEO {geo}
+LBIC 3Dldc-X.bic
+REYN 1.0
+MHEL {mu} {f}
+PARA 0 0
+GRAV 0 0 -1
+EQNS 0 0 0 0 0 1
+RELA 1 1.0 1.0D-6
+TIME 1.5 1 1
+LSQS 5000 1.0E-7
+DLSE 1 10.0 0.001
+GAUS 4 4 4 4
+IIDI 2
+SCRO 0
+REST 0
+"""
+MH_BETA = 1.5 / 1.5  # SetTimeScheme (src/lib_util.f:18): beta = 1.5/tstep
 
 
 # ----------------------------------------------------------------------------
@@ -177,8 +206,9 @@ def doubles(recs):
 
 
 def parse_macro(path, nicell):
-    recs = read_records(path)
-    v = doubles(recs[2:])
+    """path: a tri.macro.int file, or its records after the two header records."""
+    recs = read_records(path)[2:] if isinstance(path, str) else path
+    v = doubles(recs)
     nsp, npoc = 51, 27
     m1, m2 = nicell * nsp, nicell * npoc
     shapes = [("H", m1, 27), ("G", m1, 24), ("B", m1, 27), ("Abdx", m1, 27), ("Abdy", m1, 27),
@@ -229,6 +259,70 @@ def parse_table(path, ncol):
             if len(t) >= ncol:
                 rows.append([float(s.replace("D", "E")) for s in t[:ncol]])
     return np.array(rows)
+
+
+def mh_diffusivities(x, idc, ibf, nq, mu, f, beta):
+    """Node diffusivities as src/main.f90:532-543 sets them, and their values at
+    the flux nodes by the 27-node interpolation of Int2MacroNodes
+    (src/lib_util.f:1377-1475), summed in the same order."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    if f == 1:
+        diff = np.full(len(x), beta / (mu * mu))
+    else:
+        t = 1 + x[:, 0] * x[:, 1] * x[:, 2]
+        diff = beta / (t * t)
+    xsi = [-.75, -.75, .75, .75, -.75, .75, .75, -.75, 1, 1, 1, 1, .75, -.75, -.75, .75, -1, -1, -1, -1, -.75, .75,
+           .75, -.75]
+    eta = [-.75, .75, .75, -.75, -1, -1, -1, -1, -.75, .75, .75, -.75, 1, 1, 1, 1, .75, -.75, -.75, .75, -.75, -.75,
+           .75, .75]
+    ceta = [-1, -1, -1, -1, -.75, -.75, .75, .75, -.75, -.75, .75, .75, -.75, -.75, .75, .75, -.75, -.75, .75, .75, 1,
+            1, 1, 1]
+    qdiff = np.zeros(nq)
+    for ic in range(len(idc)):
+        for ii in range(6):
+            for jj in range(4):
+                ll = ii * 4 + jj
+                fi = O.shape27(xsi[ll], eta[ll], ceta[ll])
+                acc = 0.0
+                for kk in range(27):
+                    acc = acc + fi[kk] * diff[idc[ic, kk] - 1]
+                qdiff[ibf[ic, ii, jj] - 1] = acc
+    return diff, qdiff
+
+
+def run_mh_case(name, spec, keep=None):
+    sys.path.insert(0, REPO)
+    from tritok_amd import Host
+    g = read_geo(os.path.join(REFMESH, spec["geo"].replace("jit", "kva")))
+    work = keep or tempfile.mkdtemp(prefix="tt_golden_%s_" % name)
+    os.makedirs(work, exist_ok=True)
+    if "jitter" in spec:
+        g = jitter_corners(g, *spec["jitter"])
+    write_geo(os.path.join(work, spec["geo"]), g)
+    shutil.copy(os.path.join(REFMESH, "3Dldc-X.bic"), os.path.join(work, "3Dldc-X.bic"))
+    mu, f = spec["mh"]
+    with open(os.path.join(work, "tri.inp"), "w") as fh:
+        fh.write(MH_INP.format(geo=spec["geo"], mu=mu, f=f))
+    subprocess.run([REFBIN], cwd=work, check=True, stdout=subprocess.DEVNULL)
+    # the mesh as the reference holds it (linear-geometry snap applied)
+    m = Host.from_geo(os.path.join(work, spec["geo"])).mesh_arrays()
+    diff, qdiff = mh_diffusivities(m["x"], m["idc"], m["ibf"], m["nq"], mu, f, MH_BETA)
+    recs = read_records(os.path.join(work, "tri.modHelm.macro.int"))
+    beta, ssum, mumax = [np.frombuffer(r, "<f8")[0] for r in recs[2:5]]
+    seq = 0.0
+    for v in diff:  # the reference's sum of diff, in node order (src/lib_mHelm.f:1442-1445)
+        seq = seq + v
+    assert beta == MH_BETA and ssum == seq, (beta, ssum, seq)
+    nc = len(g["idc"])
+    out = dict(mesh_x=g["x"], mesh_ibc=g["ibc"], mesh_idc=g["idc"], mesh_bewn=g["bewn"],
+               mesh_nbnodes=np.int32(g["nbnodes"]), mh_diff=diff, mh_qdiff=qdiff, mh_beta=np.float64(beta),
+               mh_mumax=np.float64(mumax))
+    out.update({k.replace("macro_", "mh_"): v for k, v in parse_macro(recs[5:], nc).items()})
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+    print(name, "->", os.path.join(HERE, name + ".npz"), "mumax", mumax)
+    if keep is None:
+        shutil.rmtree(work)
 
 
 def dump_gauss():
@@ -312,4 +406,5 @@ if __name__ == "__main__":
         dump_gauss()
         names = [n for n in names if n != "gauss"]
     for n in names:
-        run_case(n, CASES[n], keep=os.path.join(keep, n) if keep else None)
+        run = run_mh_case if "mh" in CASES[n] else run_case
+        run(n, CASES[n], keep=os.path.join(keep, n) if keep else None)

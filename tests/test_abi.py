@@ -44,6 +44,20 @@ def test_last_error_reports_bad_arguments():
     assert b"NULL" in _lib.tt_last_error()
 
 
+def test_macro_mh_rejects_nonpositive_diffusivity():
+    """mu = sqrt(beta/diff) must be finite: checked on the host before any
+    device call."""
+    from tritok_amd import integrals as I
+    from tritok_amd import Host
+    from tritok_amd._lib import TTError
+    h = Host.cube(1)
+    m = h.mesh_arrays()
+    with pytest.raises(TTError, match="diff must be positive"):
+        I.macro_integrals_mh(h.view(), I.quad(), np.zeros(m["nn"]), np.ones(m["nq"]), 1.0)
+    with pytest.raises(TTError, match="beta"):
+        I.macro_integrals_mh(h.view(), I.quad(), np.ones(m["nn"]), np.ones(m["nq"]), -1.0)
+
+
 def _codes(h):
     v = h.view()
     nb, nn, nq = v.mesh.nbnodes, v.mesh.nnodes, v.mesh.nq

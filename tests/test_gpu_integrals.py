@@ -2,7 +2,8 @@
 
 Parity bar: FP64 integral matrices within 1e-12 relative (Frobenius per matrix)
 of the oracle (which equals the reference bit for bit with the same Gauss
-table), on uniform, jittered and 8^3 meshes."""
+table), on uniform, jittered and 8^3 meshes; for the modified-Helmholtz
+matrices also per cell, since mu spans three decades across a batch."""
 import numpy as np
 import pytest
 
@@ -66,3 +67,51 @@ def test_macro_gpu_vs_oracle_copy_and_computed_gauss():
         o = O.macro_integrals(O.quad(4, 4, 4, 4, 2), m, copy=copy)
         for k in I.MACRO:
             assert rel_l2(r[k], o[k]) < TOL, (copy, k)
+
+
+@pytest.mark.parametrize("case", ["mh2u", "mh2v", "mh2h"])
+def test_macro_mh_gpu_vs_reference(case, gauss_table):
+    """FMATmodHelm on the device against the reference's tri.modHelm.macro.int."""
+    from tritok_amd import integrals as I
+    g = load_golden(case)
+    h = _host(g)
+    r = I.macro_integrals_mh(h.view(), I.quad(4, 4, 4, 4, 2, table=gauss_table), g["mh_diff"], g["mh_qdiff"],
+                             float(g["mh_beta"]))
+    for k in I.MACRO:
+        e = rel_l2(r[k], g["mh_" + k])
+        print(case, k, "rel", e, flush=True)
+        assert e < TOL, k
+    assert np.allclose(r["ont"], g["mh_ont"], rtol=1e-3, atol=1e-12)
+    assert r["mumax"] == g["mh_mumax"]
+
+
+def test_macro_mh_gpu_batch_vs_oracle():
+    """Independent jittered cells with mu log-uniform in [0.1, 100] per cell
+    (the batched-integrals workload, 27 cells): every cell block of every
+    matrix within 1e-12 of the oracle."""
+    from tritok_amd import integrals as I
+    from tritok_amd import workloads as W
+    b = W.cell_batch(3, jitter=0.1, seed=7, mu_range=(0.1, 100.0))
+    am = I.ArrayMesh(b["arrays"])
+    r = I.macro_integrals_mh(am, I.quad(4, 4, 4, 4, 2), b["diff"], b["qdiff"], b["beta"])
+    a = dict(b["arrays"], nn=am.mesh.nnodes, nc=am.mesh.nicell, nq=am.mesh.nq)
+    o = O.macro_integrals_mh(O.quad(4, 4, 4, 4, 2), a, b["diff"], b["qdiff"], b["beta"])
+    nc = am.mesh.nicell
+    for k in I.MACRO:
+        per = r[k].shape[0] // nc
+        worst = max(rel_l2(r[k][c * per:(c + 1) * per], o[k][c * per:(c + 1) * per]) for c in range(nc))
+        print("batch", k, "worst cell rel", worst, flush=True)
+        assert worst < TOL, k
+    assert r["mumax"] == o["mumax"]
+
+
+def test_macro_mh_gpu_mu_zero_is_laplace(gauss_table):
+    """beta = 0 (mu = 0): the modified-Helmholtz kernel reduces to Laplace."""
+    from tritok_amd import integrals as I
+    g = load_golden("jit2")
+    h = _host(g)
+    m = h.mesh_arrays()
+    r = I.macro_integrals_mh(h.view(), I.quad(4, 4, 4, 4, 2, table=gauss_table), np.ones(m["nn"]),
+                             np.ones(m["nq"]), 0.0)
+    for k in I.MACRO:
+        assert rel_l2(r[k], g["macro_" + k]) < TOL, k

@@ -175,6 +175,102 @@ def test_domain_regular_integral_brute_force():
 
 
 # ----------------------------------------------------------- linear algebra
+# ------------------------------------------------- modified-Helmholtz integrals
+@pytest.mark.parametrize("case", ["mh2u", "mh2v", "mh2h"])
+def test_macro_mh_match_reference(case, gauss_table):
+    """FMATmodHelm against the reference's tri.modHelm.macro.int: uniform mu = 20
+    (2^3 cube), mu = 1 + xyz and uniform mu = 100 (jittered 2^3 cube)."""
+    g = load_golden(case)
+    m = _mesh(g)
+    r = O.macro_integrals_mh(O.quad(4, 4, 4, 4, 2, table=gauss_table), m, g["mh_diff"], g["mh_qdiff"],
+                             float(g["mh_beta"]))
+    for k in MACRO:
+        assert np.array_equal(r[k], g["mh_" + k]), k
+    assert np.array_equal(r["ont"], g["mh_ont"])
+    assert r["mumax"] == g["mh_mumax"]
+
+
+def test_macro_mh_mu_zero_is_laplace(gauss_table):
+    """beta = 0 gives mu = 0 at every source: the modified-Helmholtz assembly
+    (its other correction order included) must reproduce FMATkmdcSteady bit for bit."""
+    g = load_golden("jit2")
+    m = _mesh(g)
+    q = O.quad(4, 4, 4, 4, 2, table=gauss_table)
+    r = O.macro_integrals_mh(q, m, np.ones(m["nn"]), np.ones(m["nq"]), 0.0)
+    for k in MACRO:
+        assert np.array_equal(r[k], g["macro_" + k]), k
+    assert r["mumax"] == 0.0
+
+
+def test_domain_regular_mh_integral_brute_force():
+    """A regular modified-Helmholtz domain integral against a 48^3 tensor Gauss
+    rule written here from the fundamental solution exp(-mu r)/(4 pi r) and its
+    gradient -(1 + mu r) exp(-mu r) r_vec/(4 pi r^3) (r_vec = x - p)."""
+    q = O.quad(4, 4, 4, 4, 2)
+    c = np.array([[0, 0, 0], [1.1, 0, 0], [0, 0.9, 0], [1.0, 1.0, 0.1], [0, 0.1, 1], [1, 0, 1.2], [0.1, 1, 1],
+                  [1, 1.1, 1]], float)
+    p = np.array([1.9, -0.4, 0.4])
+    mu = 1.7
+    be, dx, dy, dz, n = O.inted27_mh(q, p, c, 0, mu)
+    gi, om = O.gauss(8)
+    acc = np.zeros((4, 27))
+    for k, wk in zip(gi, om):
+        for j, wj in zip(gi, om):
+            for i, wi in zip(gi, om):
+                N = np.array([(1 + s0 * i) * (1 + s1 * j) * (1 + s2 * k) / 8 for s2 in (-1, 1) for s1 in (-1, 1)
+                              for s0 in (-1, 1)])
+                dN = np.array([[s0 * (1 + s1 * j) * (1 + s2 * k) / 8, s1 * (1 + s0 * i) * (1 + s2 * k) / 8,
+                                s2 * (1 + s0 * i) * (1 + s1 * j) / 8] for s2 in (-1, 1) for s1 in (-1, 1)
+                               for s0 in (-1, 1)])
+                xc = N @ c
+                rv = xc - p
+                ra = np.linalg.norm(rv)
+                us = np.exp(-mu * ra) / (4 * np.pi * ra)
+                grad_x = -(1 + mu * ra) * np.exp(-mu * ra) * rv / (4 * np.pi * ra ** 3)  # d u*/d x (field point)
+                w = wi * wj * wk * np.linalg.det(dN.T @ c)
+                acc += w * np.outer([us, *grad_x], O.shape27(i, j, k))
+    for got, want in zip((be, dx, dy, dz), acc):
+        assert np.abs(got - want).max() < 1e-7 * np.abs(want).max()
+    # mu enters: the Laplace values differ by far more than that tolerance
+    be0 = O.inted27(q, p, c, 0)[0]
+    assert np.abs(be0 - acc[0]).max() > 1e-2 * np.abs(acc[0]).max()
+
+
+def _mh_field_residual(n, mu_kernel, mu_field=1.0):
+    """Max over macro rows of |H u - G q| for u = exp(mu_field a.x), which solves
+    lap u = mu_field^2 u: with the kernel's mu equal to mu_field the domain terms
+    cancel and the boundary equation holds up to interpolation error."""
+    from tritok_amd import Host
+    m = Host.cube(n).mesh_arrays()
+    x, xq, idc, ibf, flpm = m["x"], m["xq"], m["idc"], m["ibf"], m["flpm"]
+    a = np.array([0.6, 0.0, 0.8])
+    u = lambda pts: np.exp(mu_field * pts @ a)
+    r = O.macro_integrals_mh(O.quad(4, 4, 4, 4, 2), m, np.full(m["nn"], 1.0 / mu_kernel ** 2),
+                             np.full(m["nq"], 1.0 / mu_kernel ** 2), 1.0)
+    worst = 0.0
+    for ic in range(m["nc"]):
+        un = u(x[idc[ic] - 1])
+        cen = x[idc[ic] - 1].mean(0)
+        qf = np.zeros(24)
+        for s in range(6):
+            pts = xq[ibf[ic, s] - 1]
+            nrm = pts.mean(0) - cen
+            nrm /= np.linalg.norm(nrm)
+            qf[4 * s:4 * s + 4] = mu_field * (a @ nrm) * u(pts) * flpm[ic, s]
+        rows = slice(51 * ic, 51 * ic + 51)
+        worst = max(worst, np.abs(r["H"][rows] @ un - r["G"][rows] @ qf).max())
+    return worst
+
+
+def test_macro_mh_closed_form_field():
+    """Closed-form modified-Helmholtz solution on the unit cube: the residual of
+    the macro boundary equation falls at the interpolation order (h^3) under
+    refinement, and a kernel with a 10 % wrong mu leaves a residual >10x larger."""
+    r2, r4 = _mh_field_residual(2, 1.0), _mh_field_residual(4, 1.0)
+    assert r2 < 3e-3 and r4 < 4e-4 and r2 / r4 > 5.0
+    assert _mh_field_residual(4, 1.1) > 10 * r4
+
+
 def test_dgefa_dgesl_vs_lapack():
     rng = np.random.default_rng(3)
     a = rng.standard_normal((40, 40))

@@ -35,8 +35,11 @@ struct MacroDev {
   int nc = 0;
   DBuf<double> H, G, B, Ax, Ay, Az, Htx, Hty, Htz, Dx, Dy, Dz;
 };
+// mun/muq (device, nnodes / nq): modified-Helmholtz mu of every node / flux
+// node as source (FMATmodHelm); NULL for the Laplace kernel (FMATkmdcSteady).
 void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev &out, double ont[11],
-                         hipStream_t s);
+                         hipStream_t s, const double *mun = nullptr, const double *muq = nullptr,
+                         double *mumax = nullptr);
 
 // Single-domain kinematics rows row0..row1-1, column-major with ld = rows.
 struct SdDev {

@@ -107,7 +107,13 @@ struct MacroOut {
   double *ont;  // per row: 11 accuracy terms
 };
 
+// Helm: FMATmodHelm (lib_mHelm.f:2-483) with mu of each node / flux node in
+// mun / muq; otherwise FMATkmdcSteady.  The free term, Ht and linear-field
+// corrections follow all integrals of the row (the modified-Helmholtz free
+// term needs the row's B).
+template <bool Helm>
 __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__restrict__ qp, int ncomp,
+                                                   const double *__restrict__ mun, const double *__restrict__ muq,
                                                    MacroOut o) {
   const int ic = blockIdx.x * 64 + threadIdx.x;
   const int it = blockIdx.y;  // source 0..50
@@ -118,15 +124,17 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
   const bool usrc = it < 27;
   auto X = [&](int node, int d) { return m.x[(node - 1) + (size_t)m.nn * d]; };
   auto IDC = [&](int k) { return m.idc[ic + nc * (k - 1)]; };
-  double p[3];
+  double p[3], mu = 0.0;
   int its = 0, itf = 0;
   if (usrc) {
     int dn = IDC(it + 1);
     p[0] = X(dn, 0), p[1] = X(dn, 1), p[2] = X(dn, 2);
+    if (Helm) mu = mun[dn - 1];
   } else {
     its = (it - 27) / 4, itf = (it - 27) % 4 + 1;
     int dn = m.ibf[ic + nc * (its + 6 * (itf - 1))];
     p[0] = m.xq[dn - 1], p[1] = m.xq[dn - 1 + m.nq], p[2] = m.xq[dn - 1 + 2 * (size_t)m.nq];
+    if (Helm) mu = muq[dn - 1];
   }
   // shortest edge of the cell's faces (FindMinEdge)
   double minedge = 1.0e20;
@@ -136,8 +144,9 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
       double dx = X(a, 0) - X(b, 0), dy = X(a, 1) - X(b, 1), dz = X(a, 2) - X(b, 2);
       minedge = fmin(minedge, sqrt(dx * dx + dy * dy + dz * dz));
     }
-  double *ont = o.ont + i * 11;
+  double *ont = o.ont + i * 12;
   for (int k = 0; k < 11; ++k) ont[k] = 0.0;
+  ont[11] = mu;
   // ---- boundary integrals over the six faces
   for (int je = 0; je < 6; ++je) {
     double c[4][3];
@@ -150,8 +159,9 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
     } else if (its == je) {
       isrc = itf + 9;
     }
-    BndAcc<true> acc;
+    BndAcc<true, Helm> acc;
     acc.zero();
+    acc.mu = mu;
     integrate_face<true>(q, p, c, isrc, minedge, acc);
     for (int s = 0; s < 9; ++s) {
       size_t col = kFace9[je][s] - 1;
@@ -168,9 +178,30 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
     double sg = (double)m.flpm[ic + nc * je];
     for (int s = 0; s < 4; ++s) o.G[i + m1 * (je * 4 + s)] += acc.ge[s] * sg;
   }
-  // ---- free term and Ht row sums
+  // ---- domain integrals
+  double cc[8][3];
+  for (int k = 0; k < 8; ++k)
+    for (int d = 0; d < 3; ++d) cc[k][d] = X(IDC(kGeo8[k]), d);
+  DomAcc<4, Helm> dacc;
+  dacc.zero();
+  dacc.mu = mu;
+  integrate_cell(q, p, cc, it + 1, dacc);
+  for (int s = 0; s < 27; ++s) {
+    o.Dx[i + m1 * s] = dacc.a[0][s];
+    o.Dy[i + m1 * s] = dacc.a[1][s];
+    o.Dz[i + m1 * s] = dacc.a[2][s];
+    o.B[i + m1 * s] = dacc.a[3][s];
+  }
+  // ---- free term (rigid body u = 1; modified Helmholtz: f = -mu^2) and Ht row sums
   double cfree = 0.0;
-  for (int j = 0; j < 27; ++j) cfree -= o.H[i + m1 * j];
+  if (Helm) {
+    double vH = 0.0, vB = 0.0;
+    for (int j = 0; j < 27; ++j) vH += o.H[i + m1 * j];
+    for (int j = 0; j < 27; ++j) vB += dacc.a[3][j];
+    cfree = -vH + mu * mu * vB;
+  } else {
+    for (int j = 0; j < 27; ++j) cfree -= o.H[i + m1 * j];
+  }
   if (usrc) {
     o.H[i + m1 * it] += cfree;
     double *Ht[3] = {o.Htx, o.Hty, o.Htz};
@@ -188,26 +219,15 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
     shape9((itf == 1 || itf == 4) ? -tc : tc, itf <= 2 ? -tc : tc, fi);
     for (int s = 0; s < 9; ++s) o.H[i + m1 * (kFace9[its][s] - 1)] += cfree * fi[s];
   }
-  // ---- domain integrals
-  double cc[8][3];
-  for (int k = 0; k < 8; ++k)
-    for (int d = 0; d < 3; ++d) cc[k][d] = X(IDC(kGeo8[k]), d);
-  DomAcc<4> dacc;
-  dacc.zero();
-  integrate_cell<4>(q, p, cc, it + 1, dacc);
-  for (int s = 0; s < 27; ++s) {
-    o.Dx[i + m1 * s] = dacc.a[0][s];
-    o.Dy[i + m1 * s] = dacc.a[1][s];
-    o.Dz[i + m1 * s] = dacc.a[2][s];
-    o.B[i + m1 * s] = dacc.a[3][s];
-  }
   if (usrc) {
+    // linear fields v = (z,0,0), (0,x,0), (0,0,y): sum_j (H - mu^2 B)_ij x_j = sum_j D_ij
     double *Dm[3] = {o.Dz, o.Dx, o.Dy};
     const int crd[3] = {2, 0, 1}, slot[3] = {7, 5, 6};
     for (int t = 0; t < 3; ++t) {
       double hv = 0.0, vs = 0.0;
       for (int j = 0; j < 27; ++j) {
-        hv += o.H[i + m1 * j] * X(IDC(j + 1), crd[t]);
+        double h = Helm ? o.H[i + m1 * j] - mu * mu * o.B[i + m1 * j] : o.H[i + m1 * j];
+        hv += h * X(IDC(j + 1), crd[t]);
         vs += Dm[t][i + m1 * j];
       }
       ont[slot[t]] = fabs(hv - vs);
@@ -254,7 +274,8 @@ __global__ void macro_copy_kernel(int nc, const int *__restrict__ flpm, MacroOut
   }
 }
 
-void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev &o, double ont[11], hipStream_t s) {
+void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev &o, double ont[11], hipStream_t s,
+                         const double *mun, const double *muq, double *mumax) {
   const size_t nc = m.nc, m1 = nc * 51, m2 = nc * 27;
   o.nc = m.nc;
   DBuf<double> *mats1[] = {&o.H, &o.B, &o.Ax, &o.Ay, &o.Az, &o.Dx, &o.Dy, &o.Dz};
@@ -266,9 +287,14 @@ void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev
   DBuf<QuadDev> dq(1);
   dq.upload(&qh, 1, s);
   int ncomp = copy ? 1 : m.nc;
-  DBuf<double> ontd((size_t)ncomp * 51 * 11);
+  DBuf<double> ontd((size_t)ncomp * 51 * 12);
   MacroOut out{o.H.p, o.G.p, o.B.p, o.Ax.p, o.Ay.p, o.Az.p, o.Htx.p, o.Hty.p, o.Htz.p, o.Dx.p, o.Dy.p, o.Dz.p, ontd.p};
-  hipLaunchKernelGGL(macro_kernel, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), dq.p, ncomp, out);
+  if (mun && muq)
+    hipLaunchKernelGGL(macro_kernel<true>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), dq.p, ncomp, mun,
+                       muq, out);
+  else
+    hipLaunchKernelGGL(macro_kernel<false>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), dq.p, ncomp,
+                       nullptr, nullptr, out);
   TT_LAUNCH_CHECK();
   if (copy && m.nc > 1) {
     hipLaunchKernelGGL(macro_copy_kernel, dim3(ceil_div(m1, 256)), dim3(256), 0, s, m.nc, m.flpm.p, out);
@@ -277,14 +303,15 @@ void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev
   std::vector<double> oh(ontd.n);
   ontd.download(oh.data(), oh.size(), s);
   TT_HIP(hipStreamSynchronize(s));
-  // accumulate as FMATkmdcSteady: sums for ontH/ontG, maxima otherwise
-  double r[11] = {0};
+  // accumulate as FMATkmdcSteady / FMATmodHelm: sums for ontH/ontG, maxima
+  // otherwise (slot 11: the largest mu)
+  double r[12] = {0};
   for (int ic = 0; ic < ncomp; ++ic)
     for (int it = 0; it < 51; ++it) {
-      const double *v = &oh[((size_t)ic * 51 + it) * 11];
+      const double *v = &oh[((size_t)ic * 51 + it) * 12];
       r[0] += v[0];
       r[1] += v[1];
-      for (int t = 2; t < 11; ++t) r[t] = std::max(r[t], v[t]);
+      for (int t = 2; t < 12; ++t) r[t] = std::max(r[t], v[t]);
     }
   if (copy) {
     r[1] /= 24;
@@ -292,7 +319,8 @@ void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev
     r[1] = r[1] / m.nc / 24;
     r[0] /= m.nc;
   }
-  std::memcpy(ont, r, sizeof(r));
+  std::memcpy(ont, r, sizeof(double) * 11);
+  if (mumax) *mumax = r[11];
 }
 
 // ---------------------------------------------------------------------------
@@ -354,7 +382,7 @@ __global__ void __launch_bounds__(64) sd_dom_kernel(MeshPtrs m, const QuadDev *_
     if (!active || sing) continue;  // singular cells: sd_dom_singular_kernel
     DomAcc<3> acc;
     acc.zero();
-    integrate_cell<3>(*qp, p, c, 0, acc);
+    integrate_cell(*qp, p, c, 0, acc);
     for (int s = 0; s < 27; ++s) {
       size_t j = nodes[s] - 1;
       o.Dx[r + ld * j] += acc.a[0][s];
@@ -380,7 +408,7 @@ __global__ void __launch_bounds__(64) sd_dom_singular_kernel(MeshPtrs m, const Q
     for (int d = 0; d < 3; ++d) c[t][d] = m.x[m.idc[ic + nc * (kGeo8[t] - 1)] - 1 + (size_t)m.nn * d];
   DomAcc<3> acc;
   acc.zero();
-  integrate_cell<3>(*qp, p, c, psrc[k], acc);
+  integrate_cell(*qp, p, c, psrc[k], acc);
   for (int t = 0; t < 3; ++t)
     for (int s = 0; s < 27; ++s) staging[(size_t)k * 81 + t * 27 + s] = acc.a[t][s];
 }
@@ -516,6 +544,42 @@ extern "C" int tt_macro_integrals(const tt_mesh_view *mv, const tt_quad *q, int3
     upload_mesh(*mv, dm, 0);
     MacroDev md;
     macro_integrals_dev(dm, qh, copy, md, ont, 0);
+    size_t m1 = (size_t)mv->nicell * 51 * 27, m2 = (size_t)mv->nicell * 27 * 27;
+    md.H.download(H, m1), md.B.download(B, m1), md.Ax.download(Abdx, m1), md.Ay.download(Abdy, m1);
+    md.Az.download(Abdz, m1), md.Dx.download(Dx, m1), md.Dy.download(Dy, m1), md.Dz.download(Dz, m1);
+    md.G.download(G, (size_t)mv->nicell * 51 * 24);
+    md.Htx.download(Htx, m2), md.Hty.download(Hty, m2), md.Htz.download(Htz, m2);
+    TT_HIP(hipDeviceSynchronize());
+    return 0;
+  });
+}
+
+extern "C" int tt_macro_integrals_mh(const tt_mesh_view *mv, const tt_quad *q, int32_t copy, const double *diff,
+                                     const double *qdiff, double beta, double *H, double *G, double *B, double *Abdx,
+                                     double *Abdy, double *Abdz, double *Htx, double *Hty, double *Htz, double *Dx,
+                                     double *Dy, double *Dz, double ont[11], double *mumax) {
+  return guarded("tt_macro_integrals_mh", [&] {
+    if (!mv || !q || !diff || !qdiff) throw std::runtime_error("NULL argument");
+    if (!(beta >= 0.0)) throw std::runtime_error("beta must be >= 0");
+    // mu = SQRT(inp%beta/mHdiff(dn)) of each source (lib_mHelm.f:139,197,279,303)
+    std::vector<double> mun(mv->nnodes), muq(mv->nq);
+    for (int k = 0; k < mv->nnodes; ++k) mun[k] = std::sqrt(beta / diff[k]);
+    for (int k = 0; k < mv->nq; ++k) muq[k] = std::sqrt(beta / qdiff[k]);
+    for (double v : mun)
+      if (!std::isfinite(v)) throw std::runtime_error("diff must be positive (mu = sqrt(beta/diff) not finite)");
+    for (double v : muq)
+      if (!std::isfinite(v)) throw std::runtime_error("qdiff must be positive (mu = sqrt(beta/qdiff) not finite)");
+    QuadDev qh;
+    build_quad(*q, qh);
+    DevMesh dm;
+    upload_mesh(*mv, dm, 0);
+    DBuf<double> dmun, dmuq;
+    dmun.upload(mun.data(), mun.size(), 0);
+    dmuq.upload(muq.data(), muq.size(), 0);
+    MacroDev md;
+    double mm = 0.0;
+    macro_integrals_dev(dm, qh, copy, md, ont, 0, dmun.p, dmuq.p, &mm);
+    if (mumax) *mumax = mm;
     size_t m1 = (size_t)mv->nicell * 51 * 27, m2 = (size_t)mv->nicell * 27 * 27;
     md.H.download(H, m1), md.B.download(B, m1), md.Ax.download(Abdx, m1), md.Ay.download(Abdy, m1);
     md.Az.download(Abdz, m1), md.Dx.download(Dx, m1), md.Dy.download(Dy, m1), md.Dz.download(Dz, m1);

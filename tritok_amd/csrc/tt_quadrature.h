@@ -118,11 +118,28 @@ __device__ __forceinline__ void cell_map(const double (&c)[8][3], double eti, do
            aj[0][0] * aj[1][2] * aj[2][1] - aj[0][1] * aj[1][0] * aj[2][2] - aj[0][2] * aj[1][1] * aj[2][0]);
 }
 
+// Fundamental-solution factors at distance ra: u* and the factor of its
+// gradient.  Laplace: u* = 1/(4 pi r), factor 1.  Modified Helmholtz
+// (INTEDc27modHelm lib_mHelm.f:1905-1910, INTEBc9modHelm :2227-2233):
+// u* = exp(-mu r)/(4 pi r), gradient factor (1 + mu r).
+template <bool Helm>
+__device__ __forceinline__ void kernel_factors(double ra, double mu, double &fung, double &gf) {
+  const double PI = 3.141592653589793;
+  fung = 0.25 / (PI * ra);
+  gf = 1.0;
+  if (Helm) {
+    fung = fung * exp(-mu * ra);
+    gf = 1 + mu * ra;
+  }
+}
+
 // Accumulator of the 27-node domain integrals.  NC = 4 also keeps be (u*),
 // NC = 3 keeps only the gradient components (single-domain kinematics).
-template <int NC>
+// Helm: modified-Helmholtz kernel with the source's mu.
+template <int NC, bool Helm = false>
 struct DomAcc {
   double a[NC][27];
+  double mu = 0.0;
   __device__ void zero() {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -132,17 +149,17 @@ struct DomAcc {
   // add one quadrature point: weight w (incl. Jacobian), kernel terms
   __device__ __forceinline__ void add(const double *p, const double *xc, double ajac, double eti, double etj,
                                       double etk, bool skip_zero) {
-    const double PI = 3.141592653589793;
     double rx = p[0] - xc[0], ry = p[1] - xc[1], rz = p[2] - xc[2];
     double ra = sqrt(rx * rx + ry * ry + rz * rz);
     if (skip_zero && ra == 0.0) return;
     if (ra < 1.0e-12) ra = 1.0e-12;
     double ira2 = 1.0 / (ra * ra);
-    double fung = 0.25 / (PI * ra);
+    double fung, gf;
+    kernel_factors<Helm>(ra, mu, fung, gf);
     double f[4];
-    f[0] = rx * ira2 * fung;
-    f[1] = ry * ira2 * fung;
-    f[2] = rz * ira2 * fung;
+    f[0] = rx * ira2 * fung * gf;
+    f[1] = ry * ira2 * fung * gf;
+    f[2] = rz * ira2 * fung * gf;
     f[3] = fung;
     double f27[27];
     shape27(eti, etj, etk, f27);
@@ -155,9 +172,10 @@ struct DomAcc {
   }
 };
 
-// Domain integral of one (source, cell) pair: INTEDc27dcLap.
-template <int NC>
-__device__ void integrate_cell(const QuadDev &q, const double *p, const double (&c)[8][3], int isrc, DomAcc<NC> &acc) {
+// Domain integral of one (source, cell) pair: INTEDc27dcLap (INTEDc27modHelm
+// with a Helm accumulator).
+template <class Acc>
+__device__ void integrate_cell(const QuadDev &q, const double *p, const double (&c)[8][3], int isrc, Acc &acc) {
   const double PI = 3.141592653589793, PI2 = 2.0 * PI;
   int nd = isrc != 0 ? q.iiDiv : 1;
   for (int ix = 1; ix <= nd; ++ix)
@@ -270,11 +288,12 @@ __device__ void integrate_cell(const QuadDev &q, const double *p, const double (
 }
 
 // Accumulator of the 9-node face integrals; Macro also keeps ge and ad*.
-template <bool Macro>
+template <bool Macro, bool Helm = false>
 struct BndAcc {
   double he[9], hexy[9], heyz[9], hezx[9];
   double ad[Macro ? 3 : 1][Macro ? 9 : 1];
   double ge[Macro ? 4 : 1];
+  double mu = 0.0;
   __device__ void zero() {
 #pragma unroll
     for (int s = 0; s < 9; ++s) he[s] = hexy[s] = heyz[s] = hezx[s] = 0.0;
@@ -289,7 +308,6 @@ struct BndAcc {
   }
   __device__ __forceinline__ void add(const double *p, const double (&c)[4][3], double eti, double etj,
                                       double wpre) {
-    const double PI = 3.141592653589793;
     double e1m = 1.0 - eti, e1p = 1.0 + eti, e2m = 1.0 - etj, e2p = 1.0 + etj;
     double f4[4] = {0.25 * e1m * e2m, 0.25 * e1p * e2m, 0.25 * e1p * e2p, 0.25 * e1m * e2p};
     double xc[3], ks[3], et[3];
@@ -308,8 +326,9 @@ struct BndAcc {
     double rx = p[0] - xc[0], ry = p[1] - xc[1], rz = p[2] - xc[2];
     double ra = sqrt(rx * rx + ry * ry + rz * rz);
     double ira2 = 1.0 / (ra * ra);
-    double fung = 0.25 / (PI * ra);
-    double funx = rx * ira2 * fung, funy = ry * ira2 * fung, funz = rz * ira2 * fung;
+    double fung, gf;
+    kernel_factors<Helm>(ra, mu, fung, gf);
+    double funx = rx * ira2 * fung * gf, funy = ry * ira2 * fung * gf, funz = rz * ira2 * fung * gf;
     double funh = funx * n1 + funy * n2 + funz * n3;
     double cxy = funx * n2 - funy * n1, cyz = funy * n3 - funz * n2, czx = funz * n1 - funx * n3;
     double f9[9];
@@ -336,11 +355,11 @@ struct BndAcc {
   }
 };
 
-// Face integral of one (source, face) pair: INTEBc9kmdclap (Macro) or
-// INTEBc9sdkm (no subdivision).
-template <bool Macro>
+// Face integral of one (source, face) pair: INTEBc9kmdclap (Macro),
+// INTEBc9modHelm (Macro, Helm accumulator) or INTEBc9sdkm (no subdivision).
+template <bool Macro, class Acc>
 __device__ void integrate_face(const QuadDev &q, const double *p, const double (&c)[4][3], int isrc, double minedge,
-                               BndAcc<Macro> &acc) {
+                               Acc &acc) {
   const double PI = 3.141592653589793, PI2 = 2.0 * PI;
   int ndx = 1, ndy = 1;
   if (Macro) {

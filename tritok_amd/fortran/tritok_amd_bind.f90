@@ -55,6 +55,20 @@ MODULE tritok_amd_bind
       INTEGER(C_INT) :: tt_macro_integrals
     END FUNCTION
 
+    ! FMATmodHelm / corMin_modHelm (lib_mHelm.f:2, :1400)
+    FUNCTION tt_macro_integrals_mh(m, q, copy, diff, qdiff, beta, H, G, B, Abdx, Abdy, Abdz, Htx, Hty, Htz, &
+                                   Dx, Dy, Dz, ont, mumax) BIND(C, NAME="tt_macro_integrals_mh")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, tt_mesh_view, tt_quad
+      TYPE(tt_mesh_view), INTENT(IN) :: m
+      TYPE(tt_quad), INTENT(IN) :: q
+      INTEGER(C_INT32_T), VALUE :: copy
+      REAL(C_DOUBLE), INTENT(IN) :: diff(*), qdiff(*)
+      REAL(C_DOUBLE), VALUE :: beta
+      REAL(C_DOUBLE) :: H(*), G(*), B(*), Abdx(*), Abdy(*), Abdz(*), Htx(*), Hty(*), Htz(*)
+      REAL(C_DOUBLE) :: Dx(*), Dy(*), Dz(*), ont(11), mumax
+      INTEGER(C_INT) :: tt_macro_integrals_mh
+    END FUNCTION
+
     ! FMATsdkm / sdKm_corInt rows zac:kon (lib_sdkm.f:1286, :2377)
     FUNCTION tt_sd_integrals(m, q, row0, row1, H, Htx, Hty, Htz, Dx, Dy, Dz, ont) &
         BIND(C, NAME="tt_sd_integrals")

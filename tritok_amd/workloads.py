@@ -86,3 +86,52 @@ def natural_convection(n, ra=1e5, pr=0.71, urdw=0.1, urdt=0.05, **kw):
     from rest at Ra = 1e5 that steady Picard iteration diverges, while
     URDW 0.1, URDT 0.05 converges (205 iterations to 1e-6 at 4^3)."""
     return Host.cube(n).read_bic(bic_heated()).read_inp(inp_text(rapr=(ra, pr), urdw=urdw, urdt=urdt, **kw))
+
+
+def cell_batch(n_side=16, jitter=0.1, seed=20251203, mu_range=(0.1, 100.0), beta=1.0):
+    """Batch of n_side^3 independent 27-node cells for the batched
+    modified-Helmholtz integrals: cell (i, j, k) of an n_side^3 arrangement with
+    edge h = 1/n_side, its eight corners moved by U(-jitter*h, jitter*h) per
+    coordinate, its other nodes and flux points placed trilinearly from the
+    corners (straight-edged cells, as the reference's CheckLinearGeometry,
+    lib_mesh.f:135-196, leaves them), and mu log-uniform in mu_range per cell,
+    all from one seeded stream.  Cells share no nodes, so each cell's mu is its
+    nodes' and flux points' mu: diff = beta/mu^2 at all of them.
+
+    Numbering is the reference's for one cell (the 1-cell cube), repeated with
+    offsets.  Returns dict(arrays=mesh arrays as Host.mesh_arrays(), diff,
+    qdiff, beta, mu (per cell))."""
+    import numpy as np
+    one = Host.cube(1).mesh_arrays()
+    nc = n_side ** 3
+    h = 1.0 / n_side
+    rng = np.random.default_rng(seed)
+    ijk = np.stack(np.meshgrid(np.arange(n_side), np.arange(n_side), np.arange(n_side), indexing="ij"),
+                   -1).reshape(-1, 3)[:, ::-1].astype(float)  # i fastest
+    # corners of the unit cell in trilinear order (s0 fastest)
+    unit = np.array([[a, b, c] for c in (0, 1) for b in (0, 1) for a in (0, 1)], float)
+    corners = (ijk[:, None, :] + unit[None]) * h + rng.uniform(-jitter * h, jitter * h, size=(nc, 8, 3))
+    mu = np.exp(rng.uniform(np.log(mu_range[0]), np.log(mu_range[1]), size=nc))
+
+    def place(local):  # (p, 3) local coordinates in [0,1]^3 -> (nc, p, 3)
+        u, v, w = local[:, 0], local[:, 1], local[:, 2]
+        N = np.stack([(u if a else 1 - u) * (v if b else 1 - v) * (w if c else 1 - w)
+                      for c in (0, 1) for b in (0, 1) for a in (0, 1)], 1)  # (p, 8)
+        return np.einsum("pk,ckd->cpd", N, corners)
+
+    nn1, nq1, nb1 = len(one["x"]), len(one["xq"]), len(one["gbn"])
+    x = place(one["x"]).reshape(-1, 3)
+    xq = place(one["xq"]).reshape(-1, 3)
+    off = np.arange(nc, dtype=np.int32)
+    arrays = dict(
+        x=x, xq=xq,
+        idc=(one["idc"][0][None, :] + (off * nn1)[:, None]).astype(np.int32),
+        ibf=(one["ibf"][0][None] + (off * nq1)[:, None, None]).astype(np.int32),
+        flpm=np.repeat(one["flpm"], nc, axis=0).astype(np.int32),
+        ibc=(one["ibc"][None] + (off * nn1)[:, None, None]).reshape(-1, 9).astype(np.int32),
+        gbn=(one["gbn"][None, :] + (off * nn1)[:, None]).reshape(-1).astype(np.int32))
+    lbn = np.zeros(nn1 * nc, np.int32)
+    lbn[arrays["gbn"] - 1] = np.arange(1, nb1 * nc + 1, dtype=np.int32)
+    arrays["lbn"] = lbn
+    d = beta / mu ** 2
+    return dict(arrays=arrays, diff=np.repeat(d, nn1), qdiff=np.repeat(d, nq1), beta=float(beta), mu=mu)
