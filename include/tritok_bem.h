@@ -115,6 +115,88 @@ int tt_csr_mv(int32_t m, int32_t n, int32_t nnz, const int32_t *ia, const int32_
               const double *a, const double *x, double *y);
 
 /* ------------------------------------------------------------------------ */
+/* Compressed matrices: cross approximation (lib_ACA.f), stretched-Haar      */
+/* wavelets (lib_WT.f) and block matrices of compressed parts (lib_ctree.f). */
+/* Matrices are host arrays, column-major; results stay on the device behind */
+/* a handle; vectors are host arrays.                                       */
+
+typedef struct tt_aca tt_aca;
+/* Replaces AcaDo (lib_fm.f:2003-2028): type 1 aca_CrossAprox (lib_ACA.f:126-181,
+ * rank INT(eps*min(m,n))), type 2 acaj_CrossAprox / ACA_AB (:182-225, :372-510,
+ * stop when the EPS_ACA_NORM estimate <= eps), type 3 aca_RankCrossAprox
+ * (:24-125, stop when getRMSnorm of A vec against a (b vec) < eps; vec
+ * required -- the reference draws it with RANDOM_NUMBER).  With vec, *norm is
+ * that getRMSnorm for the final rank (NaN without vec).  *rank = acaMat%rank. */
+int tt_aca_compress(int32_t m, int32_t n, const double *a, int32_t type, double eps, const double *vec,
+                    tt_aca **out, int32_t *rank, double *norm);
+/* acaMat%a (m x rank) and acaMat%b (rank x n), column-major. */
+int tt_aca_factors(const tt_aca *h, double *a, double *b);
+/* aca_abxv (lib_ACA.f:226-249): y = a (b x). */
+int tt_aca_mv(tt_aca *h, const double *x, double *y);
+/* aca_abxv on host factors. */
+int tt_aca_abxv(int32_t m, int32_t n, int32_t rank, const double *a, const double *b, const double *x,
+                double *y);
+void tt_aca_free(tt_aca *h);
+
+typedef struct tt_wt tt_wt;
+/* Replaces Che_WcomMatrix (lib_WT.f:148-210): W A W^T (Che_WaWT :428-579),
+ * entries below kappa * mean|.| zeroed (Che_AbsPovp :719-740, Che_Zanemari
+ * :741-777), CSR form (Che_FormCRS :355-401).  *nnz = stored entries. */
+int tt_wt_compress(int32_t m, int32_t n, const double *a, double kappa, tt_wt **out, int32_t *nnz);
+/* crsMatrixType zvr(m+1), c(nnz), v(nnz) (1-based); returns the reference's
+ * count m*n - (zeroed entries). */
+int tt_wt_csr(const tt_wt *h, int32_t *zvr, int32_t *c, double *v);
+/* Che_WTAWx (lib_WT.f:297-329): y = W^T (C (W x)). */
+int tt_wt_mv(tt_wt *h, const double *x, double *y);
+/* Che_WTAWx on a host CSR. */
+int tt_wt_wtawx(int32_t nrow, int32_t ncol, int32_t nnz, const int32_t *zvr, const int32_t *c, const double *v,
+                const double *x, double *y);
+void tt_wt_free(tt_wt *h);
+
+typedef struct tt_hmat tt_hmat;
+/* Replaces cTree_FillMatrices (lib_ctree.f:2-96) for a partition given as the
+ * reference's HmatrixType parts (inc_types.f90:494-520): part p has rows
+ * rnodes[rptr[p]..rptr[p+1]) and columns cnodes[cptr[p]..cptr[p+1]) (1-based,
+ * rptr[0] = cptr[0] = 0); admissible parts (admiss[p] = 1) are compressed
+ * with ctype 2 (wavelet, wt_kappa) or 3 (cross approximation of aca_type with
+ * aca_eps; type 3 draws its test vector from a fixed hash), the others kept
+ * dense.  a is nrow x ncol with leading dimension lda. */
+int tt_hmat_create_parts(int32_t nrow, int32_t ncol, const double *a, int64_t lda, int32_t nparts,
+                         const int32_t *rptr, const int32_t *rnodes, const int32_t *cptr, const int32_t *cnodes,
+                         const int32_t *admiss, int32_t ctype, int32_t aca_type, double aca_eps, double wt_kappa,
+                         tt_hmat **out);
+/* The same with the library's own partition (the reference reads its cluster
+ * trees from a .ctree file that no input here provides): row and column
+ * points xr (nrow x 3), xc (ncol x 3) are split by geometric bisection into
+ * clusters of at most `leaf` points, and a block is admissible when
+ * min(diam_r, diam_c) < eta * dist (cTree_Admis, lib_ctree.f:575-605, with
+ * bounding-box diameters and distances). */
+int tt_hmat_create(int32_t nrow, int32_t ncol, const double *a, int64_t lda, const double *xr, const double *xc,
+                   int32_t leaf, double eta, int32_t ctype, int32_t aca_type, double aca_eps, double wt_kappa,
+                   tt_hmat **out);
+/* The partition tt_hmat_create uses, computed on the host (no device call):
+ * call with NULL arrays for the sizes, then with arrays of those sizes
+ * (rptr/cptr nparts+1, node lists, admiss nparts) for the parts, which
+ * tt_hmat_create_parts accepts. */
+int tt_hmat_partition(int32_t nrow, const double *xr, int32_t ncol, const double *xc, int32_t leaf, double eta,
+                      int32_t *nparts, int64_t *rnodes_len, int64_t *cnodes_len, int32_t *rptr, int32_t *rnodes,
+                      int32_t *cptr, int32_t *cnodes, int32_t *admiss);
+/* Sizes: parts, lengths of the node lists, doubles stored, largest rank. */
+int tt_hmat_info(const tt_hmat *h, int32_t *nparts, int64_t *rnodes_len, int64_t *cnodes_len, int64_t *stored,
+                 int32_t *max_rank);
+/* The partition: rptr/cptr (nparts+1), node lists, ctype (1 dense, 2, 3), rank. */
+int tt_hmat_parts(const tt_hmat *h, int32_t *rptr, int32_t *rnodes, int32_t *cptr, int32_t *cnodes,
+                  int32_t *ctype, int32_t *rank);
+/* Stored data of one part: dense nr x nc; ACA a (nr x rank) then b (rank x nc);
+ * wavelet CSR values into d and zvr(nr+1) then c into iv.  Returns the count
+ * of doubles. */
+int tt_hmat_part_data(const tt_hmat *h, int32_t part, double *d, int32_t *iv);
+/* cTree_MxV (lib_ctree.f:99-164) for nrhs = 1..4 columns: y(nrow x nrhs) =
+ * H x(ncol x nrhs), parts summed in part order. */
+int tt_hmat_mv(tt_hmat *h, int32_t nrhs, const double *x, double *y);
+void tt_hmat_free(tt_hmat *h);
+
+/* ------------------------------------------------------------------------ */
 /* Dense single-domain kinematics.                                          */
 
 /* Replaces dgefa (lib_Solve_v11.f:787-904) as used by FormPREfm pret=2:

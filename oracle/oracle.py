@@ -279,3 +279,120 @@ def picard(case, fields, maxhist=2000, nthreads=8):
                   _d(f["qtemp"]), maxhist, _d(sta), _i(ite), nthreads, C.byref(spi))
     n = min(its, maxhist)
     return f, sta[:n], ite[:n], spi.value
+
+
+# ---------------------------------------------------------------- compressed
+_aca = _sig("or_aca", C.c_int, C.c_int, C.c_int, C.c_int, PD, C.c_double, PD, PD, PD, PD)
+_aca_abxv = _sig("or_aca_abxv", None, C.c_int, C.c_int, C.c_int, PD, PD, PD, PD)
+_dvanak = _sig("or_dvanak", C.c_int, C.c_int)
+_wxv = _sig("or_wt_wxv", None, C.c_int, PD)
+_wtxv = _sig("or_wt_wtxv", None, C.c_int, PD)
+_wawt = _sig("or_wt_wawt", None, C.c_int, C.c_int, PD)
+_wthr = _sig("or_wt_threshold", C.c_double, C.c_int, C.c_int, PD, C.c_double, PI)
+_wcsr = _sig("or_wt_form_csr", C.c_int, C.c_int, C.c_int, PD, PI, PI, PD)
+_wtawx = _sig("or_wt_wtawx", None, C.c_int, C.c_int, PI, PI, PD, PD, PD)
+PL = C.POINTER(C.c_long)
+_hmxv = _sig("or_hmat_mxv", None, C.c_int, C.c_int, PI, PI, PI, PI, PI, PI, PL, PD, PL, PI, PD, PD)
+
+
+def aca(type_, A, eps, vec=None):
+    """Cross approximation of A (copied).  Returns (a, b, rank, norm, residual)."""
+    R = np.array(A, np.float64, order="F")
+    m, n = R.shape
+    mn = min(m, n)
+    a = np.zeros((m, mn), order="F")
+    b = np.zeros(mn * n)
+    norm = C.c_double(np.nan)
+    v = None if vec is None else np.ascontiguousarray(vec, np.float64)
+    r = _aca(int(type_), m, n, _d(R), float(eps), None if v is None else _d(v), _d(a), _d(b), C.byref(norm))
+    return (np.asfortranarray(a[:, :r]), b[:r * n].reshape((n, r)).T.copy(order="F"), r, norm.value, R)
+
+
+def aca_abxv(a, b, x):
+    a = np.asfortranarray(a, np.float64)
+    b = np.asfortranarray(b, np.float64)
+    y = np.zeros(a.shape[0])
+    _aca_abxv(a.shape[0], b.shape[1], a.shape[1], _d(a), _d(b), _d(np.ascontiguousarray(x, np.float64)), _d(y))
+    return y
+
+
+def dvanak(n):
+    return _dvanak(int(n))
+
+
+def wt_wxv(x):
+    r = np.array(x, np.float64)
+    _wxv(len(r), _d(r))
+    return r
+
+
+def wt_wtxv(x):
+    r = np.array(x, np.float64)
+    _wtxv(len(r), _d(r))
+    return r
+
+
+def wt_wawt(A):
+    W = np.array(A, np.float64, order="F")
+    _wawt(W.shape[0], W.shape[1], _d(W))
+    return W
+
+
+def wt_compress(A, kappa):
+    """Che_WcomMatrix without the random check: returns dict(zvr, c, v, nnz_ref,
+    tresh, T) with T the thresholded transformed matrix."""
+    T = wt_wawt(A)
+    m, n = T.shape
+    nnz = C.c_int()
+    tresh = _wthr(m, n, _d(T), float(kappa), C.byref(nnz))
+    cnt = int(np.count_nonzero(T))
+    zvr = np.zeros(m + 1, np.int32)
+    c = np.zeros(max(cnt, 1), np.int32)
+    v = np.zeros(max(cnt, 1))
+    st = _wcsr(m, n, _d(T), _i(zvr), _i(c), _d(v))
+    return dict(zvr=zvr, c=c[:st].copy(), v=v[:st].copy(), nnz_ref=nnz.value, tresh=tresh, T=T)
+
+
+def wt_wtawx(w, x, nrow):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros(nrow)
+    _wtawx(nrow, len(x), _i(w["zvr"]), _i(w["c"]), _d(w["v"]), _d(x), _d(y))
+    return y
+
+
+def hmat_mxv(nrow, parts, x):
+    """parts: list of dict(rows, cols (1-based arrays), kind in 'dense'/'wt'/'aca',
+    dense: (nr, nc) array; wt: wt_compress dict; aca: (a, b))."""
+    rptr, cptr, rn, cn, ct, rk, doff, ioff, dp, ip = [0], [0], [], [], [], [], [], [], [], []
+    dpos = ipos = 0
+    for p in parts:
+        rn.append(np.asarray(p["rows"], np.int32))
+        cn.append(np.asarray(p["cols"], np.int32))
+        rptr.append(rptr[-1] + len(p["rows"]))
+        cptr.append(cptr[-1] + len(p["cols"]))
+        doff.append(dpos)
+        ioff.append(ipos)
+        if p["kind"] == "dense":
+            d = np.asfortranarray(p["dense"], np.float64).ravel(order="F")
+            ct.append(1), rk.append(0), dp.append(d)
+            dpos += len(d)
+        elif p["kind"] == "wt":
+            w = p["wt"]
+            ct.append(2), rk.append(0), dp.append(w["v"]), ip.append(w["zvr"]), ip.append(w["c"])
+            dpos += len(w["v"])
+            ipos += len(w["zvr"]) + len(w["c"])
+        else:
+            a, b = p["aca"]
+            d = np.concatenate([np.asfortranarray(a).ravel(order="F"), np.asfortranarray(b).ravel(order="F")])
+            ct.append(3), rk.append(a.shape[1]), dp.append(d)
+            dpos += len(d)
+    arr = lambda v, t: np.ascontiguousarray(np.concatenate(v) if v else np.zeros(1), t)
+    rptr, cptr = np.asarray(rptr, np.int32), np.asarray(cptr, np.int32)
+    rn, cn, ct, rk = arr(rn, np.int32), arr(cn, np.int32), np.asarray(ct, np.int32), np.asarray(rk, np.int32)
+    doff, ioff = np.asarray(doff, np.int64), np.asarray(ioff, np.int64)
+    dp, ip = arr(dp, np.float64), arr(ip, np.int32)
+    y = np.zeros(nrow)
+    x = np.ascontiguousarray(x, np.float64)
+    _hmxv(nrow, len(parts), _i(rptr), _i(rn), _i(cptr), _i(cn), _i(ct), _i(rk), doff.ctypes.data_as(PL), _d(dp),
+          ioff.ctypes.data_as(PL), _i(ip), _d(x), _d(y))
+    return y

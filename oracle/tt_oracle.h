@@ -32,6 +32,13 @@
  *                            variable mu); mu = 0 equals or_macro_integrals
  *                            exactly; closed-form modified-Helmholtz fields.
  *   or_sd_integrals          reference tri.sdkm.int.1 (dc2, jit2, dc8 rows).
+ *   or_aca                   brute-force cross approximation written in the
+ *                            tests; exact recovery of rank-r matrices; the
+ *                            Frobenius error bound at the stopping rank.
+ *   or_wt_*                  the stretched-Haar matrix built explicitly in the
+ *                            tests (orthonormal rows at n = 2^k, exact
+ *                            round trip at kappa = 0 for any size).
+ *   or_hmat_mxv              the dense product of the assembled matrix.
  *   or_lsqr2, or_dgefa/dgesl scipy lsqr / LAPACK getrf on the same systems; the
  *                            reference's per-solve iteration counts (tri.ite).
  *   or_picard                reference converged fields (tri.rst) and per-
@@ -114,6 +121,42 @@ void or_sd_integrals(const or_quad *q, int nn, int nb, int ne, int nc, const dou
                      const int *ibc, const int *idc, const int *gbn, const int *lbn, int row0,
                      int row1, int ld, double *H, double *Htx, double *Hty, double *Htz,
                      double *Dx, double *Dy, double *Dz, double ont[10], int nthreads);
+
+/* ---- compressed matrices (lib_ACA.f, lib_WT.f, lib_ctree.f) ------------ */
+
+/* Cross approximation of R (m x n, column-major, overwritten by the residual):
+ * type 1 aca_CrossAprox (rank INT(eps*min(m,n))), type 2 ACA_AB (stop when the
+ * EPS_ACA_NORM estimate <= eps), type 3 aca_RankCrossAprox (stop when the
+ * getRMSnorm test with vec < eps; vec required).  a (m x rank) and b (rank x n)
+ * need room for rank = min(m,n).  *norm (if vec) = getRMSnorm of A vec against
+ * a(b vec).  Returns the rank. */
+int or_aca(int type, int m, int n, double *R, double eps, const double *vec, double *a, double *b,
+           double *norm);
+/* aca_abxv: y = a (b x) */
+void or_aca_abxv(int m, int n, int rank, const double *a, const double *b, const double *x, double *y);
+
+/* Stretched Haar wavelets: DvaNaK; Che_WxV and Che_WTxV in place on a vector
+ * of length n; Che_WaWT in place on A (m x n); Che_AbsPovp + Che_Zanemari with
+ * threshold kappa*mean|A| (returns the threshold, *nnz as the reference
+ * counts it); Che_FormCRS (1-based zvr[m+1], c, v; returns the entries
+ * written); Che_WTAWx: y = W^T (C (W x)). */
+int or_dvanak(int n);
+void or_wt_wxv(int n, double *r);
+void or_wt_wtxv(int n, double *v);
+void or_wt_wawt(int m, int n, double *A);
+double or_wt_threshold(int m, int n, double *A, double kappa, int *nnz);
+int or_wt_form_csr(int m, int n, const double *A, int *zvr, int *c, double *v);
+void or_wt_wtawx(int nrow, int ncol, const int *zvr, const int *c, const double *v, const double *x,
+                 double *y);
+
+/* cTree_MxV: y = sum over parts of scatter(op_p gather(x)).  Part p has rows
+ * rnodes[rptr[p]..rptr[p+1]) and columns cnodes[cptr[p]..) (1-based), ctype 1
+ * (dense, nr x nc column-major at dpool+doff[p]), 2 (wavelet CSR: zvr and c at
+ * ipool+ioff[p], values at dpool+doff[p]) or 3 (ACA: a then b at
+ * dpool+doff[p], rank[p]).  Parts are summed in order. */
+void or_hmat_mxv(int nrow, int nparts, const int *rptr, const int *rnodes, const int *cptr, const int *cnodes,
+                 const int *ctype, const int *rank, const long *doff, const double *dpool, const long *ioff,
+                 const int *ipool, const double *x, double *y);
 
 /* ---- linear algebra (lib_Solve_v11.f, lib_Blas.f) ---------------------- */
 

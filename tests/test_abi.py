@@ -58,6 +58,19 @@ def test_macro_mh_rejects_nonpositive_diffusivity():
         I.macro_integrals_mh(h.view(), I.quad(), np.ones(m["nn"]), np.ones(m["nq"]), -1.0)
 
 
+def test_compressed_entry_points_reject_bad_arguments():
+    """Argument checks of the compressed-matrix entry points run before any
+    device call."""
+    from tritok_amd import compress as K
+    from tritok_amd._lib import TTError
+    with pytest.raises(TTError, match="kappa"):
+        K.Wavelet(np.ones((3, 3)), kappa=-1.0)
+    with pytest.raises(TTError, match="bad sizes"):
+        K.HMatrix(np.ones((4, 4)), np.zeros((4, 3)), np.zeros((4, 3)), leaf=0)
+    with pytest.raises(TTError, match="zvr"):
+        K.wt_wtawx(np.array([1, 2], np.int32), np.array([1, 1], np.int32), np.ones(2), np.ones(2), 1)
+
+
 def _codes(h):
     v = h.view()
     nb, nn, nq = v.mesh.nbnodes, v.mesh.nnodes, v.mesh.nq

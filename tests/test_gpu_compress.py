@@ -1,0 +1,134 @@
+"""Compressed matrices on the device against the oracle: cross approximation,
+wavelet compression and block (H-) matrix products.
+
+Bar: the factors, CSR structure and products agree with the oracle to 1e-13
+relative (the device keeps the reference's summation order for every
+coefficient and row; only the stopping-norm reductions are parallel); block
+products agree with the dense product to the compression tolerance."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _kernel_block(m, n, sep=3.0, seed=0):
+    rng = np.random.default_rng(seed)
+    x = rng.random((m, 3))
+    y = rng.random((n, 3)) + np.array([sep, 0.0, 0.0])
+    return 1.0 / np.linalg.norm(x[:, None, :] - y[None, :, :], axis=2)
+
+
+def _rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("t,eps", [(1, 0.2), (2, 1e-9), (3, 1e-9)])
+def test_aca_matches_oracle(t, eps):
+    from tritok_amd import compress as K
+    A = _kernel_block(150, 120, sep=1.2)
+    vec = np.random.default_rng(1).random(120)
+    with K.Aca(A, t, eps, vec=vec) as h:
+        a, b = h.factors()
+        oa, ob, orank, onorm, _ = O.aca(t, A, eps, vec=vec)
+        print("type", t, "rank", h.rank, orank, "norm", h.norm, onorm, flush=True)
+        assert h.rank == orank
+        assert _rel(a, oa) < 1e-13 and _rel(b, ob) < 1e-13
+        x = np.random.default_rng(2).standard_normal(120)
+        assert _rel(h.mv(x), O.aca_abxv(oa, ob, x)) < 1e-13
+        assert _rel(K.aca_abxv(oa, ob, x), O.aca_abxv(oa, ob, x)) < 1e-14
+        assert abs(h.norm - onorm) <= 1e-6 * max(onorm, 1e-300) + 1e-15
+
+
+@pytest.mark.parametrize("m,n,kappa", [(150, 120, 1e-3), (64, 64, 0.0), (37, 90, 0.1), (1, 5, 1e-2)])
+def test_wavelet_matches_oracle(m, n, kappa):
+    from tritok_amd import compress as K
+    A = _kernel_block(m, n, sep=1.0, seed=m)
+    w = O.wt_compress(A, kappa)
+    with K.Wavelet(A, kappa) as h:
+        c = h.csr()
+        print(m, n, kappa, "nnz", h.nnz, len(w["v"]), flush=True)
+        assert np.array_equal(c["zvr"], w["zvr"]) and np.array_equal(c["c"], w["c"])
+        assert _rel(c["v"], w["v"]) < 1e-13
+        assert c["nnz_ref"] == w["nnz_ref"]
+        x = np.random.default_rng(3).standard_normal(n)
+        ref = O.wt_wtawx(w, x, m)
+        assert _rel(h.mv(x), ref) < 1e-13
+        assert _rel(K.wt_wtawx(w["zvr"], w["c"], w["v"], x, m), ref) < 1e-13
+
+
+def _sphere_problem(n=2000, seed=5):
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal((n, 3))
+    p /= np.linalg.norm(p, axis=1)[:, None]
+    d = np.linalg.norm(p[:, None, :] - p[None, :, :], axis=2)
+    np.fill_diagonal(d, 1.0)
+    A = 1.0 / (4 * np.pi * d)
+    np.fill_diagonal(A, 1.0)
+    return p, A
+
+
+def _oracle_parts(h):
+    info = h.parts()
+    parts = []
+    for p in range(h.nparts):
+        rows = info["rnodes"][info["rptr"][p]:info["rptr"][p + 1]]
+        cols = info["cnodes"][info["cptr"][p]:info["cptr"][p + 1]]
+        d, iv = h.part_data(p)
+        nr, nc = len(rows), len(cols)
+        if info["ctype"][p] == 1:
+            parts.append(dict(rows=rows, cols=cols, kind="dense", dense=d.reshape((nc, nr)).T))
+        elif info["ctype"][p] == 3:
+            r = info["rank"][p]
+            a = d[:nr * r].reshape((r, nr)).T
+            b = d[nr * r:].reshape((nc, r)).T
+            parts.append(dict(rows=rows, cols=cols, kind="aca", aca=(a, b)))
+        else:
+            parts.append(dict(rows=rows, cols=cols, kind="wt", wt=dict(zvr=iv[:nr + 1], c=iv[nr + 1:], v=d)))
+    return info, parts
+
+
+@pytest.mark.parametrize("ctype,tol", [(3, 1e-6), (2, 1e-4)])
+def test_hmatrix_product(ctype, tol):
+    from tritok_amd import compress as K
+    p, A = _sphere_problem()
+    n = len(p)
+    with K.HMatrix(A, p, p, leaf=32, eta=2.0, ctype=ctype, aca_type=2, aca_eps=1e-8, wt_kappa=1e-6) as h:
+        info, parts = _oracle_parts(h)
+        # the parts tile the matrix exactly once
+        cover = np.zeros((n, n), np.int32)
+        for q in parts:
+            cover[np.ix_(q["rows"] - 1, q["cols"] - 1)] += 1
+        assert np.all(cover == 1)
+        assert (info["ctype"] != 1).any() and (info["ctype"] == 1).any()
+        print("parts", h.nparts, "stored/full", h.stored / n / n, "max rank", h.max_rank, flush=True)
+        assert h.stored < (0.8 if ctype == 3 else 1.0) * n * n
+        X = np.random.default_rng(7).standard_normal((n, 4))
+        Y = h.mv(X)
+        for r in range(4):
+            assert np.linalg.norm(Y[:, r] - A @ X[:, r]) / np.linalg.norm(A @ X[:, r]) < tol
+            assert _rel(Y[:, r], O.hmat_mxv(n, parts, X[:, r])) < 1e-13
+            assert _rel(h.mv(X[:, r]), Y[:, r]) == 0.0
+
+
+def test_hmatrix_given_partition_and_kinematics_matrix(gauss_table):
+    """The single-domain kinematics H matrix of a 3^3 cube (boundary-node
+    rows and columns), compressed on the library's own partition, then rebuilt
+    from that partition through the parts entry point: identical products."""
+    from tritok_amd import Host
+    from tritok_amd import compress as K
+    from tritok_amd import integrals as I
+    h = Host.cube(3)
+    m = h.mesh_arrays()
+    H = I.sd_integrals(h.view(), I.quad(4, 4, 4, 4, 2, table=gauss_table))["H"]
+    pts = m["x"][m["gbn"] - 1]
+    x = np.random.default_rng(11).standard_normal(H.shape[1])
+    with K.HMatrix(H, pts, pts, leaf=16, eta=1.0, ctype=3, aca_type=2, aca_eps=1e-6) as hm:
+        y = hm.mv(x)
+        assert np.linalg.norm(y - H @ x) / np.linalg.norm(H @ x) < 1e-5
+        info = hm.parts()
+        adm = (info["ctype"] != 1).astype(np.int32)
+        parts = (info["rptr"], info["rnodes"], info["cptr"], info["cnodes"], adm)
+        with K.HMatrix(H, parts=parts, ctype=3, aca_type=2, aca_eps=1e-6) as h2:
+            assert np.array_equal(h2.mv(x), y)
