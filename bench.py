@@ -111,6 +111,84 @@ def accuracy_leg(n, mixed):
             "seconds": {p: out[p][2] for p in out}}
 
 
+def serving_leg(requests, slots, max_iter, precs, seed=20251203):
+    """The serving workload's request stream (Poisson arrivals at 20/s, cubes
+    of 6^3..16^3 subdomains, half convection / half cavity, each to a Picard
+    change of 1e-6), first `requests` arrivals submitted in real time to the
+    request loop, once per precision.  Latency = finish - submit."""
+    from tritok_amd import workloads
+    from tritok_amd.serve import Server
+    stream = workloads.serving_stream(count=requests, rate=20.0, seed=seed)
+    hosts = [workloads.serving_case(s) for s in stream]  # the host side prepares inputs before they arrive
+    out = {"workload": "first %d requests of the seeded 2000-request stream: Poisson 20/s, n in {6,8,12,16}, "
+                       "50%% convection Ra log-U[1e3,1e6] / 50%% cavity Re U[10,400], Picard to 1e-6 "
+                       "(cap %d iterations)" % (requests, max_iter),
+           "requests": requests, "slots": slots, "arrival_rate_per_s": 20.0}
+    for prec in precs:
+        with Server(slots=slots, max_iter=max_iter) as srv:
+            t0 = time.perf_counter()
+            ids = []
+            for spec, h in zip(stream, hosts):
+                dt = spec["t"] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids.append(srv.submit(h, prec))
+            infos = [srv.wait(i) for i in ids]
+            wall = time.perf_counter() - t0
+        lat = np.array([i["t_end"] - i["t_submit"] for i in infos])
+        run = np.array([i["t_end"] - i["t_start"] for i in infos])
+        ok = [i["status"] == 0 for i in infos]
+        log("serving %s: %d requests in %.1f s, latency p50 %.2f s p99 %.2f s, %d converged, %d capped, %d failed" % (
+            prec, requests, wall, np.percentile(lat, 50), np.percentile(lat, 99), sum(ok),
+            sum(i["status"] == 1 for i in infos), sum(i["status"] < 0 for i in infos)))
+        out[prec] = {"latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                                   "mean": float(lat.mean()), "max": float(lat.max())},
+                     "service_s": {"p50": float(np.percentile(run, 50)), "p99": float(np.percentile(run, 99))},
+                     "solves_per_s": requests / wall, "wall_s": wall, "converged": int(sum(ok)),
+                     "capped": int(sum(i["status"] == 1 for i in infos)),
+                     "failed": int(sum(i["status"] < 0 for i in infos)),
+                     "mean_picard_iterations": float(np.mean([i["picard_iterations"] for i in infos])),
+                     "per_request": [{"n": s["n"], "kind": s["kind"], "its": i["picard_iterations"],
+                                      "status": i["status"], "latency_s": round(float(l), 3)}
+                                     for s, i, l in zip(stream, infos, lat)]}
+    return out
+
+
+FP64_PEAK_TFLOPS = 78.6
+# algorithmic flops per fundamental-solution evaluation, modified-Helmholtz
+# kernel: 27 shapes x 4 outputs x 2 + 15 (+3 for mu) over cells, (7 x 9 + 4) x 2
+# + 25 (+3) over faces; square roots, divisions, exp and the singular
+# transforms' trigonometry are not credited
+FLOP_DOMAIN_MH, FLOP_BOUNDARY_MH = 234.0, 162.0
+
+
+def batched_integrals_leg(n_side, seed=20251203):
+    """Modified-Helmholtz macro integrals of n_side^3 independent jittered
+    cells (mu log-uniform in [0.1, 100] per cell) in one launch."""
+    from tritok_amd import integrals as I
+    from tritok_amd import workloads
+    b = workloads.cell_batch(n_side, jitter=0.1, seed=seed, mu_range=(0.1, 100.0))
+    am = I.ArrayMesh(b["arrays"])
+    q = I.quad(4, 4, 4, 4, 2)
+    I.macro_integrals_mh(am, q, b["diff"], b["qdiff"], b["beta"])  # warm-up (module load)
+    t0 = time.perf_counter()
+    I.macro_integrals_mh(am, q, b["diff"], b["qdiff"], b["beta"])
+    wall = time.perf_counter() - t0
+    st = I.last_integral_stats()
+    flops = st["domain_points"] * FLOP_DOMAIN_MH + st["boundary_points"] * FLOP_BOUNDARY_MH
+    tflops = flops / (st["ms"] * 1e-3) / 1e12
+    nc = am.mesh.nicell
+    log("batched integrals: %d cells in %.1f ms on the device (%.2f s with transfers), %.3g domain + %.3g boundary "
+        "evaluations, %.2f TFLOP/s" % (nc, st["ms"], wall, st["domain_points"], st["boundary_points"], tflops))
+    return {"workload": "%d independent 27-node cells (%d^3 arrangement, corners jittered +-10%% of the edge), "
+                        "mu log-uniform in [0.1, 100] per cell, modified-Helmholtz macro matrices" % (nc, n_side),
+            "cells": nc, "kernel_ms": st["ms"], "cells_per_s": nc / (st["ms"] * 1e-3), "wall_s_with_transfers": wall,
+            "domain_points": st["domain_points"], "boundary_points": st["boundary_points"],
+            "roofline": {"bound": "fp64", "achieved": tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tflops / FP64_PEAK_TFLOPS,
+                         "flop_per_point": {"domain": FLOP_DOMAIN_MH, "boundary": FLOP_BOUNDARY_MH}}}
+
+
 def lsqr_matrix_bytes(info, value_bytes):
     """Algorithmic bytes of the two sparse products of one LSQR iteration for a
     CSR system (values + int32 column indices, row pointers), per the
@@ -133,6 +211,12 @@ def main():
                     help="cube size of the converged mixed-vs-FP64 comparison (0: skip)")
     ap.add_argument("--no-kinematics", action="store_true",
                     help="EQNS iBw = 0: skip the single-domain kinematics (least-squares systems only)")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving stream replayed in real time (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-max-iter", type=int, default=400)
+    ap.add_argument("--batch-cells", type=int, default=16,
+                    help="n: batched modified-Helmholtz integrals of n^3 independent cells (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB of a device-to-device copy issued first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -203,7 +287,11 @@ def main():
         import cpu_sample  # the oracle: CPU baseline leg only
         cpu = cpu_sample.picard_rate(host, case, legs["fp64"]["nits"], budget_s=args.cpu_seconds, log=log)
 
+    case.close()  # free the 20^3 case before the other legs
     acc = accuracy_leg(args.accuracy_n, args.mixed) if args.accuracy_n > 0 else None
+    batched = batched_integrals_leg(args.batch_cells) if args.batch_cells > 0 else None
+    serving = (serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, ("fp64", args.mixed))
+               if args.serve_requests > 0 else None)
 
     ph = mixed["phases"]
     out = {
@@ -226,9 +314,10 @@ def main():
         "lsqr_iters_per_step": {"fp64": (legs["fp64"]["nits"].sum(0)[3:10] / args.steps).tolist(),
                                 "mixed": (nit[3:10] / args.steps).tolist()},
         "setup_s": setup_s, "device": dev, "roofline": roofline, "cpu_baseline": cpu,
+        "p99_solve_latency_s": serving[args.mixed]["latency_s"]["p99"] if serving else None,
+        "serving": serving, "batched_integrals": batched,
     }
     print(json.dumps(out), flush=True)
-    case.close()
 
 
 if __name__ == "__main__":

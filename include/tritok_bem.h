@@ -82,6 +82,12 @@ int tt_macro_integrals_mh(const tt_mesh_view *m, const tt_quad *q, int32_t copy,
                           double *Abdy, double *Abdz, double *Htx, double *Hty, double *Htz, double *Dx,
                           double *Dy, double *Dz, double ont[11], double *mumax);
 
+/* Measurement hook (no reference counterpart): device time (ms) of the
+ * calling thread's last macro assembly kernel (tt_macro_integrals,
+ * tt_macro_integrals_mh, tt_case_create) and its fundamental-solution
+ * evaluations over cells (domain) and faces (boundary). */
+int tt_last_integral_stats(double *ms, int64_t *domain_points, int64_t *boundary_points);
+
 /* Replaces FMATsdkm (lib_sdkm.f:1286-1530) as called by sdKm_corInt
  * (lib_sdkm.f:2377-2449) for boundary rows row0..row1-1 (0-based; the
  * reference's zac:kon).  Host outputs with leading dimension row1-row0:
@@ -344,6 +350,46 @@ int tt_host_fields(tt_host *h, const double **vel, const double **vort, const do
 /* Writes the mesh as a GiD .geo file the reference can read. */
 int tt_host_write_geo(tt_host *h, const char *path);
 void tt_host_free(tt_host *h);
+
+/* ------------------------------------------------------------------------ */
+/* Request loop over whole cases (no reference counterpart: the reference runs
+ * one case per process).  Worker threads, one per slot, admit submitted cases
+ * in arrival order as slots and device memory free up (first fit against the
+ * budget), build each on its own stream, iterate it to convergence, keep the
+ * converged fields on the host and release the device memory.  Cases in
+ * different slots run concurrently on the device.                           */
+
+typedef struct tt_server tt_server;
+typedef struct tt_server_info {
+  int32_t status;            /* 0 converged, 1 stopped at max_iter, -1 failed (error) */
+  int32_t picard_iterations;
+  int64_t lsqr_iterations;   /* all least-squares iterations of the case */
+  double final_change;       /* largest relative change of the last Picard step */
+  double t_submit, t_start, t_setup, t_end; /* s since tt_server_create: queued,
+                                               admitted, built, finished */
+  int64_t estimated_bytes, device_bytes;
+  char error[256];
+} tt_server_info;
+
+/* slots concurrent cases; mem_budget device bytes for admission (0: 80 % of
+ * the free memory); anchor_period of reduced-precision cases (0: 8);
+ * max_iter caps the Picard steps per case (0: the case's own RELA limit). */
+int tt_server_create(int32_t slots, int64_t mem_budget, int32_t anchor_period, int32_t max_iter,
+                     tt_server **out);
+/* Queues the case held by h (which must live until the request finishes) in
+ * precision prec (TT_PREC_*); *id identifies the request. */
+int tt_server_submit(tt_server *s, tt_host *h, int32_t prec, int64_t *id);
+/* Blocks until request id finishes; fills info. */
+int tt_server_wait(tt_server *s, int64_t id, tt_server_info *info);
+/* Converged fields of a finished request (arrays sized as tt_case_fields). */
+int tt_server_fields(tt_server *s, int64_t id, double *vel, double *vort, double *qvort, double *temp,
+                     double *qtemp);
+/* Forgets a finished request. */
+int tt_server_release(tt_server *s, int64_t id);
+/* Queue length, device bytes admitted, budget. */
+int tt_server_load(tt_server *s, int32_t *queued, int64_t *bytes_in_use, int64_t *budget);
+/* Finishes the queued requests, stops the workers, frees the server. */
+void tt_server_free(tt_server *s);
 
 #ifdef __cplusplus
 }

@@ -575,6 +575,14 @@ static int find_col(int nc, const int *idc, int ic, int node) {
   return -1;
 }
 
+/* kernel evaluations of the last macro assembly: domain, boundary */
+static long g_points[2];
+
+void or_last_points(long *domain, long *boundary) {
+  *domain = g_points[0];
+  *boundary = g_points[1];
+}
+
 /* FMATkmdcSteady (mun == NULL) and FMATmodHelm (mun/muq: mu of every node /
  * flux node as source).  The two differ in the kernel, in the free term
  * (c = -sum H + mu^2 sum B, lib_mHelm.f:343-357) and in the linear-field
@@ -605,6 +613,7 @@ static void macro_k(const or_quad *q, int nn, int nc, int nq, const double *x, c
   int ncomp = copy ? 1 : nc;
   /* per-cell accuracy estimates: [ontH, ontG, Htx, Hty, Htz, Dx, Dy, Dz, HtyDx, HtzDy, HtxDz] */
   double *ontc = (double *)calloc((size_t)ncomp * 12, sizeof(double));
+  long *cpts = (long *)calloc((size_t)ncomp * 2, sizeof(long));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
 #endif
@@ -639,7 +648,7 @@ static void macro_k(const or_quad *q, int nn, int nc, int nq, const double *x, c
         int isrc = 0;
         for (int s = 0; s < 9; ++s)
           if (dn == ibcl[je][s]) isrc = s + 1;
-        inteb9_k(q, 1, p, cf, isrc, minedge, mu, he, ge, abx, aby, abz, htx, hty, htz);
+        cpts[2 * ic + 1] += inteb9_k(q, 1, p, cf, isrc, minedge, mu, he, ge, abx, aby, abz, htx, hty, htz);
         for (int s = 0; s < 9; ++s) {
           int col = find_col(nc, idc, ic, ibcl[je][s]) - 1;
           M1(H, i, col) = M1(H, i, col) + he[s];
@@ -682,7 +691,7 @@ static void macro_k(const or_quad *q, int nn, int nc, int nq, const double *x, c
           for (int k = 0; k < 4; ++k)
             for (int d = 0; d < 3; ++d) cf[k][d] = XC(ibcl[je][2 * k], d);
           int isrc = (its == je) ? itf + 9 : 0;
-          inteb9_k(q, 1, p, cf, isrc, minedge, mu, he, ge, abx, aby, abz, htx, hty, htz);
+          cpts[2 * ic + 1] += inteb9_k(q, 1, p, cf, isrc, minedge, mu, he, ge, abx, aby, abz, htx, hty, htz);
           for (int s = 0; s < 9; ++s) {
             int col = find_col(nc, idc, ic, ibcl[je][s]) - 1;
             M1(H, i, col) = M1(H, i, col) + he[s];
@@ -717,7 +726,7 @@ static void macro_k(const or_quad *q, int nn, int nc, int nq, const double *x, c
       int dn = IDC(ic, it);
       double p[3] = {XC(dn, 0), XC(dn, 1), XC(dn, 2)};
       size_t i = (size_t)ic * 51 + (it - 1), ii = (size_t)ic * 27 + (it - 1);
-      inted27_k(q, p, cc8, it, helm ? mun[dn - 1] : 0.0, be, dxe, dye, dze);
+      cpts[2 * ic] += inted27_k(q, p, cc8, it, helm ? mun[dn - 1] : 0.0, be, dxe, dye, dze);
       for (int s = 0; s < 27; ++s) {
         M1(B, i, s) = be[s];
         M1(Dx, i, s) = dxe[s];
@@ -753,7 +762,7 @@ static void macro_k(const or_quad *q, int nn, int nc, int nq, const double *x, c
         int dn = ibf[ic + (size_t)nc * (its + 6 * (itf - 1))];
         double p[3] = {xq[(dn - 1)], xq[(dn - 1) + (size_t)nq], xq[(dn - 1) + 2 * (size_t)nq]};
         size_t i = (size_t)ic * 51 + 27 + its * 4 + (itf - 1);
-        inted27_k(q, p, cc8, 27 + its * 4 + itf, helm ? muq[dn - 1] : 0.0, be, dxe, dye, dze);
+        cpts[2 * ic] += inted27_k(q, p, cc8, 27 + its * 4 + itf, helm ? muq[dn - 1] : 0.0, be, dxe, dye, dze);
         for (int s = 0; s < 27; ++s) {
           M1(B, i, s) = be[s];
           M1(Dx, i, s) = dxe[s];
@@ -878,6 +887,9 @@ static void macro_k(const or_quad *q, int nn, int nc, int nq, const double *x, c
   /* file order: ontH,ontG,ontHtx,ontHty,ontHtz,ontDx,ontDy,ontDz,ontHtyDx,ontHtzDy,ontHtxDz */
   for (int t = 0; t < 11; ++t) ont[t] = o[t];
   if (mumax) *mumax = o[11];
+  g_points[0] = g_points[1] = 0;
+  for (int ic = 0; ic < ncomp; ++ic) g_points[0] += cpts[2 * ic], g_points[1] += cpts[2 * ic + 1];
+  free(cpts);
   free(ontc);
 #undef M1
 #undef M2

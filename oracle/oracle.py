@@ -60,6 +60,7 @@ _macro_mh = _sig("or_macro_integrals_mh", None, C.POINTER(OrQuad), C.c_int, C.c_
 _inted27_mh = _sig("or_inted27_mh", C.c_long, C.POINTER(OrQuad), PD, PD, C.c_int, C.c_double, PD, PD, PD, PD)
 _inteb9_mh = _sig("or_inteb9_mh", C.c_long, C.POINTER(OrQuad), PD, PD, C.c_int, C.c_double, C.c_double, PD, PD, PD,
                   PD, PD, PD, PD, PD)
+_lastpts = _sig("or_last_points", None, C.POINTER(C.c_long), C.POINTER(C.c_long))
 _sd = _sig("or_sd_integrals", None, C.POINTER(OrQuad), C.c_int, C.c_int, C.c_int, C.c_int, PD, PI, PI, PI, PI,
            C.c_int, C.c_int, C.c_int, *([PD] * 7), PD, C.c_int)
 _dnrm2 = _sig("or_dnrm2", C.c_double, C.c_int, PD)
@@ -396,3 +397,10 @@ def hmat_mxv(nrow, parts, x):
     _hmxv(nrow, len(parts), _i(rptr), _i(rn), _i(cptr), _i(cn), _i(ct), _i(rk), doff.ctypes.data_as(PL), _d(dp),
           ioff.ctypes.data_as(PL), _i(ip), _d(x), _d(y))
     return y
+
+
+def last_points():
+    """(domain, boundary) kernel evaluations of the last macro assembly."""
+    d, b = C.c_long(), C.c_long()
+    _lastpts(C.byref(d), C.byref(b))
+    return d.value, b.value

@@ -114,6 +114,10 @@ void or_macro_integrals_mh(const or_quad *q, int nn, int nc, int nq, const doubl
                            double *Abdz, double *Htx, double *Hty, double *Htz, double *Dx,
                            double *Dy, double *Dz, double ont[11], double *mumax, int nthreads);
 
+/* Kernel evaluations (quadrature points) of the last or_macro_integrals /
+ * or_macro_integrals_mh call: over cells and over faces. */
+void or_last_points(long *domain, long *boundary);
+
 /* FMATsdkm (lib_sdkm.f:1286-1530) for boundary rows row0..row1-1 (0-based),
  * matrices column-major with leading dimension ld (>= row1-row0):
  * H,Htx,Hty,Htz (ld x nb), Dx,Dy,Dz (ld x nn).  ont[10] as the cache file. */

@@ -115,3 +115,19 @@ def test_macro_mh_gpu_mu_zero_is_laplace(gauss_table):
                              np.ones(m["nq"]), 0.0)
     for k in I.MACRO:
         assert rel_l2(r[k], g["macro_" + k]) < TOL, k
+
+
+def test_macro_kernel_evaluation_counts_match_oracle():
+    """The device's count of fundamental-solution evaluations (the unit of the
+    integral roofline) equals the oracle's replay of the subdivision rules."""
+    from tritok_amd import integrals as I
+    from tritok_amd import workloads as W
+    b = W.cell_batch(2, jitter=0.1, seed=3, mu_range=(0.5, 50.0))
+    am = I.ArrayMesh(b["arrays"])
+    I.macro_integrals_mh(am, I.quad(4, 4, 4, 4, 2), b["diff"], b["qdiff"], b["beta"])
+    st = I.last_integral_stats()
+    a = dict(b["arrays"], nn=am.mesh.nnodes, nc=am.mesh.nicell, nq=am.mesh.nq)
+    O.macro_integrals_mh(O.quad(4, 4, 4, 4, 2), a, b["diff"], b["qdiff"], b["beta"])
+    print("device", st, "oracle", O.last_points(), flush=True)
+    assert (st["domain_points"], st["boundary_points"]) == O.last_points()
+    assert st["ms"] > 0

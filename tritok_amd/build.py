@@ -54,7 +54,7 @@ def build_library(force=True, jobs=8):
                 flags = [f for f in HIPFLAGS if not f.startswith("-ffp-contract")] + ["-ffp-contract=off"]
             cmd = [HIPCC, "-x", "hip"] + COMMON + flags + ["-c", s, "-o", o]
         else:
-            cmd = [HIPCC, "-x", "c++", "-D__HIP_PLATFORM_AMD__"] + COMMON + ["-c", s, "-o", o]
+            cmd = [HIPCC, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"] + COMMON + ["-c", s, "-o", o]
         print(" ".join(cmd), flush=True)
         procs.append(subprocess.Popen(cmd))
         if len(procs) >= jobs:

@@ -102,6 +102,15 @@ MeshPtrs ptrs(const DevMesh &d) {
 
 // ---------------------------------------------------------------------------
 // macro matrices
+
+// device time and kernel evaluations of the calling thread's last macro
+// assembly (measurement hook)
+struct MacroStats {
+  float ms;
+  long long dom_pts, bnd_pts;
+};
+static thread_local MacroStats g_last_macro{0.f, 0, 0};
+
 struct MacroOut {
   double *H, *G, *B, *Ax, *Ay, *Az, *Htx, *Hty, *Htz, *Dx, *Dy, *Dz;
   double *ont;  // per row: 11 accuracy terms
@@ -114,7 +123,7 @@ struct MacroOut {
 template <bool Helm>
 __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__restrict__ qp, int ncomp,
                                                    const double *__restrict__ mun, const double *__restrict__ muq,
-                                                   MacroOut o) {
+                                                   MacroOut o, unsigned long long *__restrict__ npts) {
   const int ic = blockIdx.x * 64 + threadIdx.x;
   const int it = blockIdx.y;  // source 0..50
   if (ic >= ncomp) return;
@@ -148,6 +157,7 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
   for (int k = 0; k < 11; ++k) ont[k] = 0.0;
   ont[11] = mu;
   // ---- boundary integrals over the six faces
+  int pts_b = 0;
   for (int je = 0; je < 6; ++je) {
     double c[4][3];
     for (int k = 0; k < 4; ++k)
@@ -163,6 +173,7 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
     acc.zero();
     acc.mu = mu;
     integrate_face<true>(q, p, c, isrc, minedge, acc);
+    pts_b += acc.npts;
     for (int s = 0; s < 9; ++s) {
       size_t col = kFace9[je][s] - 1;
       o.H[i + m1 * col] += acc.he[s];
@@ -186,6 +197,10 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
   dacc.zero();
   dacc.mu = mu;
   integrate_cell(q, p, cc, it + 1, dacc);
+  if (npts) {
+    atomicAdd(&npts[0], (unsigned long long)dacc.npts);
+    atomicAdd(&npts[1], (unsigned long long)pts_b);
+  }
   for (int s = 0; s < 27; ++s) {
     o.Dx[i + m1 * s] = dacc.a[0][s];
     o.Dy[i + m1 * s] = dacc.a[1][s];
@@ -289,20 +304,33 @@ void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev
   int ncomp = copy ? 1 : m.nc;
   DBuf<double> ontd((size_t)ncomp * 51 * 12);
   MacroOut out{o.H.p, o.G.p, o.B.p, o.Ax.p, o.Ay.p, o.Az.p, o.Htx.p, o.Hty.p, o.Htz.p, o.Dx.p, o.Dy.p, o.Dz.p, ontd.p};
+  DBuf<unsigned long long> cnt(2);
+  cnt.zero(s);
+  hipEvent_t e0, e1;
+  TT_HIP(hipEventCreate(&e0));
+  TT_HIP(hipEventCreate(&e1));
+  TT_HIP(hipEventRecord(e0, s));
   if (mun && muq)
     hipLaunchKernelGGL(macro_kernel<true>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), dq.p, ncomp, mun,
-                       muq, out);
+                       muq, out, cnt.p);
   else
     hipLaunchKernelGGL(macro_kernel<false>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), dq.p, ncomp,
-                       nullptr, nullptr, out);
+                       nullptr, nullptr, out, cnt.p);
   TT_LAUNCH_CHECK();
+  TT_HIP(hipEventRecord(e1, s));
   if (copy && m.nc > 1) {
     hipLaunchKernelGGL(macro_copy_kernel, dim3(ceil_div(m1, 256)), dim3(256), 0, s, m.nc, m.flpm.p, out);
     TT_LAUNCH_CHECK();
   }
   std::vector<double> oh(ontd.n);
   ontd.download(oh.data(), oh.size(), s);
+  unsigned long long hc[2] = {0, 0};
+  cnt.download(hc, 2, s);
   TT_HIP(hipStreamSynchronize(s));
+  float ms = 0.f;
+  TT_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+  g_last_macro = MacroStats{ms, (long long)hc[0], (long long)hc[1]};
   // accumulate as FMATkmdcSteady / FMATmodHelm: sums for ontH/ontG, maxima
   // otherwise (slot 11: the largest mu)
   double r[12] = {0};
@@ -588,6 +616,13 @@ extern "C" int tt_macro_integrals_mh(const tt_mesh_view *mv, const tt_quad *q, i
     TT_HIP(hipDeviceSynchronize());
     return 0;
   });
+}
+
+extern "C" int tt_last_integral_stats(double *ms, int64_t *domain_points, int64_t *boundary_points) {
+  if (ms) *ms = g_last_macro.ms;
+  if (domain_points) *domain_points = g_last_macro.dom_pts;
+  if (boundary_points) *boundary_points = g_last_macro.bnd_pts;
+  return 0;
 }
 
 extern "C" int tt_sd_integrals(const tt_mesh_view *mv, const tt_quad *q, int32_t row0, int32_t row1, double *H,

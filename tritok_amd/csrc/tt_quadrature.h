@@ -140,6 +140,7 @@ template <int NC, bool Helm = false>
 struct DomAcc {
   double a[NC][27];
   double mu = 0.0;
+  int npts = 0;  // kernel evaluations (measurement)
   __device__ void zero() {
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -149,6 +150,7 @@ struct DomAcc {
   // add one quadrature point: weight w (incl. Jacobian), kernel terms
   __device__ __forceinline__ void add(const double *p, const double *xc, double ajac, double eti, double etj,
                                       double etk, bool skip_zero) {
+    ++npts;
     double rx = p[0] - xc[0], ry = p[1] - xc[1], rz = p[2] - xc[2];
     double ra = sqrt(rx * rx + ry * ry + rz * rz);
     if (skip_zero && ra == 0.0) return;
@@ -294,6 +296,7 @@ struct BndAcc {
   double ad[Macro ? 3 : 1][Macro ? 9 : 1];
   double ge[Macro ? 4 : 1];
   double mu = 0.0;
+  int npts = 0;  // kernel evaluations (measurement)
   __device__ void zero() {
 #pragma unroll
     for (int s = 0; s < 9; ++s) he[s] = hexy[s] = heyz[s] = hezx[s] = 0.0;
@@ -308,6 +311,7 @@ struct BndAcc {
   }
   __device__ __forceinline__ void add(const double *p, const double (&c)[4][3], double eti, double etj,
                                       double wpre) {
+    ++npts;
     double e1m = 1.0 - eti, e1p = 1.0 + eti, e2m = 1.0 - etj, e2p = 1.0 + etj;
     double f4[4] = {0.25 * e1m * e2m, 0.25 * e1p * e2m, 0.25 * e1p * e2p, 0.25 * e1m * e2p};
     double xc[3], ks[3], et[3];

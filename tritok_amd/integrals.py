@@ -11,6 +11,7 @@ _macro = _lib._sig("tt_macro_integrals", i32, C.POINTER(MeshView), C.POINTER(Qua
                    _lib.PD)
 _macro_mh = _lib._sig("tt_macro_integrals_mh", i32, C.POINTER(MeshView), C.POINTER(Quad), i32, _lib.PD, _lib.PD,
                       C.c_double, *([_lib.PD] * 12), _lib.PD, _lib.PD)
+_stats = _lib._sig("tt_last_integral_stats", i32, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_int64))
 _sd = _lib._sig("tt_sd_integrals", i32, C.POINTER(MeshView), C.POINTER(Quad), i32, i32, *([_lib.PD] * 7), _lib.PD)
 
 MACRO = ["H", "G", "B", "Abdx", "Abdy", "Abdz", "Htx", "Hty", "Htz", "Dx", "Dy", "Dz"]
@@ -96,3 +97,11 @@ def sd_integrals(view, q, row0=0, row1=None):
               ont.ctypes.data_as(_lib.PD)), "tt_sd_integrals")
     out["ont"] = ont
     return out
+
+
+def last_integral_stats():
+    """Device time (ms) and kernel evaluations (domain, boundary) of this
+    thread's last macro assembly."""
+    ms, d, b = C.c_double(), C.c_int64(), C.c_int64()
+    _stats(C.byref(ms), C.byref(d), C.byref(b))
+    return dict(ms=ms.value, domain_points=d.value, boundary_points=b.value)

@@ -135,3 +135,32 @@ def cell_batch(n_side=16, jitter=0.1, seed=20251203, mu_range=(0.1, 100.0), beta
     arrays["lbn"] = lbn
     d = beta / mu ** 2
     return dict(arrays=arrays, diff=np.repeat(d, nn1), qdiff=np.repeat(d, nq1), beta=float(beta), mu=mu)
+
+
+def serving_stream(count=2000, rate=20.0, seed=20251203, sizes=(6, 8, 12, 16)):
+    """Seeded request stream of the serving workload: Poisson arrivals at
+    `rate` per second, cube size uniform in `sizes`, half natural convection
+    (Ra log-uniform in [1e3, 1e6], Pr = 0.71) and half lid-driven cavity (Re
+    uniform in [10, 400]), each to a Picard change of 1e-6."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    out = []
+    for k in range(count):
+        n = int(rng.choice(sizes))
+        if rng.random() < 0.5:
+            out.append(dict(t=float(t[k]), kind="convection", n=n, ra=float(10 ** rng.uniform(3.0, 6.0))))
+        else:
+            out.append(dict(t=float(t[k]), kind="cavity", n=n, re=float(rng.uniform(10.0, 400.0))))
+    return out
+
+
+def serving_case(spec, **kw):
+    """Host inputs of one request of serving_stream().  Convection above
+    Ra = 1e5 gets half the vorticity and temperature relaxation of the
+    Ra = 1e5 setting (0.05 / 0.025), cavities the relaxation 0.1."""
+    if spec["kind"] == "convection":
+        strong = spec["ra"] > 1e5
+        return natural_convection(spec["n"], ra=spec["ra"], pr=0.71, urdw=0.05 if strong else 0.1,
+                                  urdt=0.025 if strong else 0.05, **kw)
+    return cavity(spec["n"], re=spec["re"], urdw=0.1, **kw)
