@@ -189,6 +189,51 @@ def batched_integrals_leg(n_side, seed=20251203):
                          "flop_per_point": {"domain": FLOP_DOMAIN_MH, "boundary": FLOP_BOUNDARY_MH}}}
 
 
+def batched_solves_leg(count, seed=20251203):
+    """`count` independent lid-driven-cavity vorticity systems (n^3 cells, n
+    uniform in 2..10, Re uniform in [10, 400], seeded; right-hand sides
+    synthetic), solved to the reference's LSQR tolerance 1e-7 in one launch."""
+    from tritok_amd import integrals as I
+    from tritok_amd import solvers as S
+    from tritok_amd import workloads
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(2, 11, size=count)
+    res = rng.uniform(10.0, 400.0, size=count)
+    parts = {}
+    systems, bs = [], []
+    for n, re in zip(ns, res):
+        n = int(n)
+        if n not in parts:
+            host = workloads.cavity(n, re=100.0)
+            parts[n] = workloads.vte_csr_parts(host, I.macro_integrals(host.view(), I.quad()))
+        ia, ja, ah, ab, nu, beta = parts[n]
+        systems.append((ia, ja, ah + ab * beta * re, nu))
+        bs.append(rng.standard_normal(len(ia) - 1))
+    x0 = [np.zeros(s_[3]) for s_ in systems]
+    t0 = time.perf_counter()
+    B = S.LsqrBatch(systems)
+    setup = time.perf_counter() - t0
+    B.solve(bs, x0, 5000, 1e-7)  # warm-up
+    xs, istop, nit, ms = B.solve(bs, x0, 5000, 1e-7)
+    B.close()
+    m = np.array([len(s_[0]) - 1 for s_ in systems], float)
+    nnz = np.array([len(s_[1]) for s_ in systems], float)
+    nunk = np.array([s_[3] for s_ in systems], float)
+    alg = float(np.sum(nit * (2 * (12 * nnz + 4 * (m + 1)) + 8 * (3 * m + 6 * nunk))))
+    gbs = alg / (ms * 1e-3) / 1e9
+    log("batched solves: %d systems (%.3g nonzeros) in %.1f ms, %d..%d iterations, %.0f GB/s algorithmic" % (
+        count, nnz.sum(), ms, nit.min(), nit.max(), gbs))
+    return {"workload": "%d lid-driven-cavity vorticity systems, n^3 cells with n uniform in 2..10, Re uniform in "
+                        "[10, 400], synthetic right-hand sides, LSQR eps 1e-7" % count,
+            "systems": count, "nonzeros": float(nnz.sum()), "kernel_ms": ms, "setup_s": setup,
+            "systems_per_s": count / (ms * 1e-3), "iterations": {"min": int(nit.min()), "mean": float(nit.mean()),
+                                                                  "max": int(nit.max())},
+            "istop": {int(k): int(v) for k, v in zip(*np.unique(istop, return_counts=True))},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS,
+                         "bytes": "per system iteration 2*((8+4)*nnz + 4*(m+1)) + 8*(3m + 6n)"}}
+
+
 def lsqr_matrix_bytes(info, value_bytes):
     """Algorithmic bytes of the two sparse products of one LSQR iteration for a
     CSR system (values + int32 column indices, row pointers), per the
@@ -217,6 +262,8 @@ def main():
     ap.add_argument("--serve-max-iter", type=int, default=400)
     ap.add_argument("--batch-cells", type=int, default=16,
                     help="n: batched modified-Helmholtz integrals of n^3 independent cells (0: skip)")
+    ap.add_argument("--batch-solves", type=int, default=256,
+                    help="independent cavity systems solved in one launch (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB of a device-to-device copy issued first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -290,6 +337,7 @@ def main():
     case.close()  # free the 20^3 case before the other legs
     acc = accuracy_leg(args.accuracy_n, args.mixed) if args.accuracy_n > 0 else None
     batched = batched_integrals_leg(args.batch_cells) if args.batch_cells > 0 else None
+    bsolves = batched_solves_leg(args.batch_solves) if args.batch_solves > 0 else None
     serving = (serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, ("fp64", args.mixed))
                if args.serve_requests > 0 else None)
 
@@ -315,7 +363,7 @@ def main():
                                 "mixed": (nit[3:10] / args.steps).tolist()},
         "setup_s": setup_s, "device": dev, "roofline": roofline, "cpu_baseline": cpu,
         "p99_solve_latency_s": serving[args.mixed]["latency_s"]["p99"] if serving else None,
-        "serving": serving, "batched_integrals": batched,
+        "serving": serving, "batched_integrals": batched, "batched_solves": bsolves,
     }
     print(json.dumps(out), flush=True)
 

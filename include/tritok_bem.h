@@ -115,6 +115,24 @@ int tt_lsqr_solve(tt_lsqr *s, const double *b, double *x, int32_t maxit, float e
                   int32_t *istop);
 void tt_lsqr_free(tt_lsqr *s);
 
+/* Many independent systems in one persistent launch: prelsqr2 for each at
+ * setup, slvlsqr2 for all of them per solve call (no reference counterpart
+ * for the batching: the reference solves its systems one at a time).  System
+ * k has m[k] rows, n[k] unknowns and nnz[k] entries; ia holds the systems'
+ * 1-based row pointers one after another (m[k]+1 each), ja/a their entries,
+ * q receives the column scalings (sum of n), and with scale_host the scaled
+ * values are written back to a.  A solve takes b (sum of m) and the hot start
+ * x (sum of n, overwritten by the solutions), the reference's maxit and REAL(4)
+ * tolerance for every system, and returns each system's iteration count and
+ * istop; *ms (may be NULL) = device time of the launch. */
+typedef struct tt_lsqr_batch tt_lsqr_batch;
+int tt_lsqr_batch_setup(int32_t nsys, const int32_t *m, const int32_t *n, const int64_t *nnz,
+                        const int32_t *ia, const int32_t *ja, double *a, double *q, int32_t scale_host,
+                        tt_lsqr_batch **out);
+int tt_lsqr_batch_solve(tt_lsqr_batch *h, const double *b, double *x, int32_t maxit, float eps4, int32_t *nit,
+                        int32_t *istop, double *ms);
+void tt_lsqr_batch_free(tt_lsqr_batch *h);
+
 /* Replaces CRSxV (lib_util.f:1351-1372): y = A x on the device for a CSR
  * matrix given by host arrays (m rows, n columns). */
 int tt_csr_mv(int32_t m, int32_t n, int32_t nnz, const int32_t *ia, const int32_t *ja,

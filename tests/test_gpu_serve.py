@@ -28,7 +28,8 @@ def test_results_alone_equal_results_in_the_mix():
             alone.append((s.fields(rid), info))
     with Server(slots=4) as s:
         ids = [s.submit(host, prec) for host, prec in cases]
-        mixed = [(s.fields(i), s.wait(i)) for i in ids]
+        infos = [s.wait(i) for i in ids]
+        mixed = [(s.fields(i), info) for i, info in zip(ids, infos)]
         starts = sorted(s.wait(i)["t_start"] for i in ids)
         ends = sorted(s.wait(i)["t_end"] for i in ids)
     assert starts[-1] < ends[0]  # all four were admitted before the first finished: they overlapped

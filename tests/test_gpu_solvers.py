@@ -158,3 +158,33 @@ def test_sd_kinematics_step_matches_oracle(case, gauss_table):
     print(case, "bw step rel", e, flush=True)
     assert e < 1e-10
     sd.close()
+
+
+def test_lsqr_batch_matches_oracle_per_system():
+    """Systems of different sizes and kinds in one launch: each agrees with
+    the oracle's slvlsqr2 as the single-system solver does."""
+    from tritok_amd import solvers as S
+    systems = [_vte_system(load_golden("dc2")), _vte_system(load_golden("jit2"), e=1)]
+    for m, n, k, seed in [(3000, 1000, 11, 3), (400, 120, 7, 4), (2500, 2000, 5, 6)]:
+        ia, ja, a = _random_csr(m, n, k, seed)
+        systems.append((ia, ja, a, n))
+    rng = np.random.default_rng(9)
+    bs = [rng.standard_normal(len(s[0]) - 1) for s in systems]
+    x0s = [0.1 * rng.standard_normal(s[3]) for s in systems]
+    B = S.LsqrBatch(systems)
+    for eps in (1e-7, 1e-10):
+        xs, istop, nit, ms = B.solve(bs, x0s, 5000, eps)
+        print("batch eps", eps, "device ms", ms, flush=True)
+        for k, (ia, ja, a, n) in enumerate(systems):
+            m = len(ia) - 1
+            ao = np.array(a, np.float64)
+            q = O.prelsqr2(m, n, ia, ja, ao)
+            assert np.max(np.abs(B.q[B.noff[k]:B.noff[k + 1]] - q) / q) <= 5e-16
+            x_ref, istop_ref, nit_ref = O.slvlsqr2(m, n, ia, ja, ao, q, bs[k], x0s[k], 5000, eps)
+            print(k, "nit", nit[k], nit_ref, "istop", istop[k], istop_ref, "rel", rel_l2(xs[k], x_ref), flush=True)
+            assert istop[k] == istop_ref and abs(nit[k] - nit_ref) <= 2
+            assert rel_l2(xs[k], x_ref) < (1e-8 if eps < 1e-9 else 1e-5)
+    # repeatable
+    xs2, _, nit2, _ = B.solve(bs, x0s, 5000, 1e-10)
+    assert all(np.array_equal(u, w) for u, w in zip(xs, xs2)) and np.array_equal(nit, nit2)
+    B.close()

@@ -73,6 +73,57 @@ class Lsqr:
             pass
 
 
+_bsetup = _lib._sig("tt_lsqr_batch_setup", i32, i32, PI, PI, C.POINTER(C.c_int64), PI, PI, PD, PD, i32,
+                    C.POINTER(P))
+_bsolve = _lib._sig("tt_lsqr_batch_solve", i32, P, PD, PD, i32, C.c_float, PI, PI, C.POINTER(C.c_double))
+_bfree = _lib._sig("tt_lsqr_batch_free", None, P)
+
+
+class LsqrBatch:
+    """Independent CSR systems [(ia, ja, a, n), ...] (1-based) solved together
+    in one launch; prelsqr2 of each at construction (self.q, and the scaled
+    values in self.a when scale_host)."""
+
+    def __init__(self, systems, scale_host=True):
+        self.m = np.array([len(s[0]) - 1 for s in systems], np.int32)
+        self.n = np.array([int(s[3]) for s in systems], np.int32)
+        self.nnz = np.array([len(s[1]) for s in systems], np.int64)
+        ia = np.concatenate([_i32(s[0]) for s in systems])
+        ja = np.concatenate([_i32(s[1]) for s in systems])
+        self.a = np.concatenate([np.asarray(s[2], np.float64) for s in systems])
+        self.q = np.zeros(int(self.n.sum()))
+        h = P()
+        check(_bsetup(len(systems), self.m.ctypes.data_as(PI), self.n.ctypes.data_as(PI),
+                      self.nnz.ctypes.data_as(C.POINTER(C.c_int64)), ia.ctypes.data_as(PI), ja.ctypes.data_as(PI),
+                      self.a.ctypes.data_as(PD), self.q.ctypes.data_as(PD), int(scale_host), C.byref(h)),
+              "tt_lsqr_batch_setup")
+        self._h = h
+        self.moff = np.concatenate([[0], np.cumsum(self.m)])
+        self.noff = np.concatenate([[0], np.cumsum(self.n)])
+
+    def solve(self, b, x0, maxit, eps4):
+        """b, x0: lists (or concatenated arrays).  Returns (xs, istop, nit, ms)."""
+        b = _f64(np.concatenate(b) if isinstance(b, (list, tuple)) else b)
+        x = np.array(np.concatenate(x0) if isinstance(x0, (list, tuple)) else x0, dtype=np.float64)
+        k = len(self.m)
+        nit, istop, ms = np.zeros(k, np.int32), np.zeros(k, np.int32), C.c_double()
+        check(_bsolve(self._h, b.ctypes.data_as(PD), x.ctypes.data_as(PD), int(maxit), float(eps4),
+                      nit.ctypes.data_as(PI), istop.ctypes.data_as(PI), C.byref(ms)), "tt_lsqr_batch_solve")
+        xs = [x[self.noff[i]:self.noff[i + 1]] for i in range(k)]
+        return xs, istop, nit, ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _bfree(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def dense_lu(a):
     """dgefa: returns (factor in LINPACK storage, pivots as float64, info)."""
     a = np.array(a, dtype=np.float64, order="F")

@@ -164,3 +164,42 @@ def serving_case(spec, **kw):
         return natural_convection(spec["n"], ra=spec["ra"], pr=0.71, urdw=0.05 if strong else 0.1,
                                   urdt=0.025 if strong else 0.05, **kw)
     return cavity(spec["n"], re=spec["re"], urdw=0.1, **kw)
+
+
+def vte_csr(host, macro, re, eq=0):
+    """See vte_csr_parts: the values for Reynolds number re."""
+    ia, ja, ah, ab, n, beta = vte_csr_parts(host, macro, eq)
+    return ia, ja, ah + ab * beta * re, n
+
+
+def vte_csr_parts(host, macro, eq=0):
+    """The vorticity-transport system of component eq (0, 1, 2) as the
+    reference assembles it (sMat2crsSysRhsB_w_fill, lib_solvFDwTE.f:593-752):
+    per cell row, H + B*beta*Re on the unknown node columns and -G on the
+    unknown flux columns, columns in increasing unknown order, 1-based CSR.
+    macro: the dict of integrals.macro_integrals for the host's mesh.
+    Returns (ia, ja, a_H, a_B, n, beta): the values are a_H + a_B * beta * Re
+    (a_B zero on flux columns), in the reference's order of operations."""
+    import numpy as np
+    v = host.view()
+    m = v.mesh
+    nc, nn, nq = m.nicell, m.nnodes, m.nq
+    arr = lambda p, k: np.ctypeslib.as_array(p, shape=(k,)).copy()
+    idc = arr(m.idc, nc * 27).reshape(27, nc).T - 1
+    ibf = arr(m.ibf, nc * 24).reshape(4, 6, nc).transpose(2, 1, 0).reshape(nc, 24) - 1
+    wk = arr(v.wkode, nn * 3).reshape(3, nn)[eq]
+    wq = arr(v.wqkode, nq * 3).reshape(3, nq)[eq]
+    beta = 1.5 / v.tstep
+    codes = np.concatenate([wk[idc], wq[ibf]], axis=1)  # (nc, 51)
+    vh = np.concatenate([macro["H"], -macro["G"]], axis=1).reshape(nc, 51, 51)
+    vb = np.concatenate([macro["B"], np.zeros_like(macro["G"])], axis=1).reshape(nc, 51, 51)
+    ja, ah, ab, counts = [], [], [], []
+    for c in range(nc):
+        unk = np.nonzero(codes[c] < 0)[0]
+        order = unk[np.argsort(-codes[c][unk], kind="stable")]
+        ja.append(np.tile(-codes[c][order], 51))
+        ah.append(vh[c][:, order].ravel())
+        ab.append(vb[c][:, order].ravel())
+        counts.append(np.full(51, len(order)))
+    ia = np.concatenate([[1], 1 + np.cumsum(np.concatenate(counts))]).astype(np.int32)
+    return (ia, np.concatenate(ja).astype(np.int32), np.concatenate(ah), np.concatenate(ab), int(v.nunk[eq]), beta)
