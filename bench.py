@@ -11,10 +11,13 @@ workload (oracle/cpu_sample.py).
 
 A step is one Picard iteration (main.f90:1200-1589): boundary vorticities,
 kinematics x,y,z, energy, vorticity transport x,y,z, convergence test and
-dynamic LSQR tolerance.  The FP64 leg and the mixed leg (FP32 least-squares
-blocks, FP64 hot-start residuals and scalars) start from the same initial
-fields; `value` is the mixed leg's rate, the FP64 leg's is reported beside
-it together with the relative L2 difference of the two legs' fields.
+dynamic LSQR tolerance.  The FP64 leg and the mixed leg (FP16 least-squares
+blocks with FP32 accumulation and FP64 hot-start residuals, scalars and
+vectors; FP16 kinematics products with FP64 anchor steps) start from the same
+initial fields; `value` is the mixed leg's rate, the FP64 leg's is reported
+beside it, with the converged mixed-vs-FP64 difference of a tight-tolerance
+run (accuracy leg).  Further legs: the batched modified-Helmholtz integrals,
+the batched least-squares solves and the serving request stream.
 
     python bench.py [--steps K] [--warmup W] [--n 20] [--no-cpu]
 """
@@ -310,15 +313,18 @@ def main():
     lat = np.sort(mixed["lat"]) * 1e3
 
     # roofline of the dominant kernel: lsqr_A on the 51-row systems (VTE x, y, z and T)
+    half = args.mixed == "fp16"
+    kname = "lsqr_A<_Float16,51,51>" if half else "lsqr_A<float,51,51>"
+    vbytes = 2 if half else 4
     infos = [case.system_info(s) for s in range(4)]
     nit = mixed["nits"].sum(0)
     w = np.array([nit[6], nit[7], nit[8], nit[9]], float)
-    alg = np.array([lsqr_matrix_bytes(i, 4) for i in infos])
+    alg = np.array([lsqr_matrix_bytes(i, vbytes) for i in infos])
     alg_bytes = float((w * alg).sum() / max(w.sum(), 1.0))
     avg_ms = mixed["probe51"]["avg_ms"]
-    traffic = read_traffic("lsqr_A<float,51,51>")  # measured HBM bytes per launch (profiles/)
+    traffic = read_traffic(kname)  # measured HBM bytes per launch (profiles/)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
-    roofline = {"kernel": "lsqr_A<float> (51-row blocks)", "bound": "hbm",
+    roofline = {"kernel": "%s (51-row blocks)" % kname, "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None,
                 "traffic": traffic,
@@ -326,7 +332,7 @@ def main():
                 "traffic_frac": traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic and avg_ms > 0 else None,
                 "avg_launch_ms": avg_ms, "launches_timed": mixed["probe51"]["sampled"],
                 "launches": mixed["probe51"]["launches"], "algorithmic_bytes_per_launch": alg_bytes,
-                "block_bytes_per_launch": float(infos[0]["cells"] * 51 * 51 * 4)}
+                "block_bytes_per_launch": float(infos[0]["cells"] * (2608 * 2 if half else 2604 * 4))}
 
     cpu = None
     if not args.no_cpu:
@@ -350,8 +356,9 @@ def main():
         "config": {"workload": "natural convection, differentially heated cube, %d^3 subdomains (%d nodes, %d "
                                "boundary nodes), Ra=1e5, Pr=0.71" % (n, view.mesh.nnodes, view.mesh.nbnodes),
                    "subdomains": n ** 3,
-                   "precision": "FP32 least-squares blocks with FP64 residual anchors; kinematics products in %s "
-                                "with per-row 2^k scales, FP64 anchor step every 8 iterations" % args.mixed.upper()},
+                   "precision": "%s least-squares blocks (FP32 accumulation) with FP64 residual anchors, "
+                                "scalars and vectors; kinematics products in %s with per-row 2^k scales, FP64 anchor "
+                                "step every 8 iterations" % ("FP16" if half else "FP32", args.mixed.upper())},
         "fp64_iters_per_s": fp64_rate, "mixed_over_fp64": value / fp64_rate,
         "mixed_vs_fp64_rel_l2_converged": acc["max_rel_l2"] if acc else None, "accuracy": acc,
         "mixed_vs_fp64_rel_l2_after_steps": max(diff.values()),

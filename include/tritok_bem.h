@@ -131,6 +131,12 @@ int tt_lsqr_batch_setup(int32_t nsys, const int32_t *m, const int32_t *n, const 
                         tt_lsqr_batch **out);
 int tt_lsqr_batch_solve(tt_lsqr_batch *h, const double *b, double *x, int32_t maxit, float eps4, int32_t *nit,
                         int32_t *istop, double *ms);
+/* Gang width of every system (no reference counterpart; a tuning and test
+ * hook): 0 = automatic (a system's share of the batch's work in units of one
+ * workgroup's fair share, 1..64), else that width for all.  *max_width (may
+ * be NULL) = the widest gang of the new schedule.  Results depend on the
+ * width only through the order of the parallel sums. */
+int tt_lsqr_batch_set_width(tt_lsqr_batch *h, int32_t width, int32_t *max_width);
 void tt_lsqr_batch_free(tt_lsqr_batch *h);
 
 /* Replaces CRSxV (lib_util.f:1351-1372): y = A x on the device for a CSR
@@ -284,11 +290,12 @@ enum { TT_PREC_FP64 = 0, TT_PREC_FP32 = 1, TT_PREC_BF16 = 2, TT_PREC_FP16 = 3 };
  * integrals, the three factorizations, all least-squares systems, and the
  * fields (initial values vel(nn,3), vort(nn,3), qvort(nq,3), temp(nn),
  * qtemp(nq)).  prec as tt_case_set_precision: with anything but TT_PREC_FP64
- * the least-squares bidiagonalisation products run on FP32 copies of the
- * scaled blocks while the initial residual of every solve (b - A x0, the hot
- * start), the LSQR scalars and norms and the solution update stay FP64, so
- * the Picard loop refines the reduced-precision corrections against the FP64
- * equations.  refine_tol is reserved (pass 0). */
+ * the least-squares bidiagonalisation products run on reduced-precision
+ * copies of the scaled blocks (FP16 for TT_PREC_FP16, FP32 otherwise; FP32
+ * arithmetic) while the initial residual of every solve (b - A x0, the hot
+ * start), the LSQR scalars and norms, all vectors and the solution update
+ * stay FP64, so the Picard loop refines the reduced-precision corrections
+ * against the FP64 equations.  refine_tol is reserved (pass 0). */
 int tt_case_create(const tt_case_view *c, const double *vel, const double *vort,
                    const double *qvort, const double *temp, const double *qtemp, int32_t prec,
                    double refine_tol, tt_case **out);
@@ -297,11 +304,13 @@ int tt_case_create(const tt_case_view *c, const double *vel, const double *vort,
 int tt_case_reset(tt_case *c, const double *vel, const double *vort, const double *qvort,
                   const double *temp, const double *qtemp);
 /* Switches the precision of a resident case.  TT_PREC_FP64: FP64 throughout.
- * TT_PREC_FP32, TT_PREC_FP16, TT_PREC_BF16: FP32 least-squares blocks (FP64
- * residual anchors, scalars and vectors), and the dense kinematics products
- * (the seven matrices of sdkm_rhs and the three inverse system matrices) on
- * copies in that format with a power-of-two scale per row, taken on the change
- * of the fields since the last FP64 anchor step; an anchor step (FP64 products
+ * TT_PREC_FP32, TT_PREC_BF16: FP32 least-squares blocks; TT_PREC_FP16: FP16
+ * least-squares blocks (the scaled blocks' columns have unit norm); in all
+ * three the products accumulate in FP32 while the residual anchors, scalars and
+ * vectors stay FP64.  The dense kinematics products (the seven matrices of
+ * sdkm_rhs and the three inverse system matrices) run on copies in that
+ * format with a power-of-two scale per row, taken on the change of the
+ * fields since the last FP64 anchor step; an anchor step (FP64 products
  * and solves) runs every anchor period (default 8) Picard iterations and after
  * a reset, so the loop converges to the FP64 fixed point.  Copies are made on
  * first use. */
@@ -322,7 +331,9 @@ int tt_case_probe_stats(tt_case *c, int32_t rows, double *avg_ms, int64_t *sampl
 int tt_case_system_info(tt_case *c, int32_t sys, int64_t info[6]);
 /* One Picard iteration (main.f90:1200-1589): err[9] = the nine relative
  * changes (tri.sta), nit[12] = solver iterations (tri.ite).  Returns 1 while
- * the loop should continue, 0 when converged or at nnlit, < 0 on error. */
+ * the loop should continue, 0 when converged or at nnlit, < 0 on error --
+ * including a diverged iteration (a non-finite relative change, where the
+ * reference would stop as if converged). */
 int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]);
 /* Runs Picard iterations until convergence; returns the iteration count. */
 int tt_case_run(tt_case *c, int32_t max_iter, double *sta, int32_t *ite);

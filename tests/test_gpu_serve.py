@@ -13,7 +13,7 @@ FIELDS = ("velocity", "vorticity", "qvorticity", "temp", "qtemp")
 def _cases():
     from tritok_amd import workloads as W
     return [(W.cavity(2, re=100.0), "fp64"), (W.natural_convection(2, ra=1e4), "fp64"),
-            (W.cavity(3, re=50.0), "fp16"), (W.natural_convection(3, ra=1e5), "fp32")]
+            (W.cavity(3, re=50.0), "fp16"), (W.natural_convection(3, ra=2e4), "fp32")]
 
 
 def test_results_alone_equal_results_in_the_mix():
@@ -36,6 +36,7 @@ def test_results_alone_equal_results_in_the_mix():
     for (fa, ia), (fm, im), (_, prec) in zip(alone, mixed, cases):
         assert im["status"] == 0 and im["picard_iterations"] == ia["picard_iterations"], (ia, im)
         for k in FIELDS:
+            assert np.isfinite(fa[k]).all() and np.isfinite(fm[k]).all(), (prec, k)
             scale = max(np.abs(fa[k]).max(), 1e-300)
             d = np.abs(fm[k] - fa[k]).max() / scale
             print(prec, k, "alone vs mix max rel", d, flush=True)

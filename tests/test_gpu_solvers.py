@@ -165,9 +165,11 @@ def test_lsqr_batch_matches_oracle_per_system():
     the oracle's slvlsqr2 as the single-system solver does."""
     from tritok_amd import solvers as S
     systems = [_vte_system(load_golden("dc2")), _vte_system(load_golden("jit2"), e=1)]
-    for m, n, k, seed in [(3000, 1000, 11, 3), (400, 120, 7, 4), (2500, 2000, 5, 6)]:
+    for m, n, k, seed in [(3000, 1000, 11, 3), (400, 120, 7, 4), (2500, 1200, 9, 6)]:
         ia, ja, a = _random_csr(m, n, k, seed)
         systems.append((ia, ja, a, n))
+    for ia, ja, a, n in systems:
+        assert len(np.unique(ja)) == n  # every unknown in some equation
     rng = np.random.default_rng(9)
     bs = [rng.standard_normal(len(s[0]) - 1) for s in systems]
     x0s = [0.1 * rng.standard_normal(s[3]) for s in systems]
@@ -187,4 +189,16 @@ def test_lsqr_batch_matches_oracle_per_system():
     # repeatable
     xs2, _, nit2, _ = B.solve(bs, x0s, 5000, 1e-10)
     assert all(np.array_equal(u, w) for u, w in zip(xs, xs2)) and np.array_equal(nit, nit2)
+    # every system on a gang of several workgroups (barriers and sums across
+    # workgroups): the same solutions to the same bar, repeatable
+    for width in (3, 8):
+        assert B.set_width(width) == width
+        xg, istop_g, nit_g, ms = B.solve(bs, x0s, 5000, 1e-10)
+        print("gang width", width, "device ms", ms, "nit", nit_g, flush=True)
+        for k, (ia, ja, a, n) in enumerate(systems):
+            assert istop_g[k] == istop[k] and abs(nit_g[k] - nit[k]) <= 2
+            assert rel_l2(xg[k], xs[k]) < 1e-8
+        xg2, _, nit_g2, _ = B.solve(bs, x0s, 5000, 1e-10)
+        assert all(np.array_equal(u, w) for u, w in zip(xg, xg2)) and np.array_equal(nit_g, nit_g2)
+    assert B.set_width(0) >= 1
     B.close()

@@ -24,6 +24,10 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "
 # sources whose products and sums must round separately, as the reference's
 # (pivot searches and thresholds compare values with the CPU restatement)
 NO_CONTRACT = {"tt_compress.hip"}
+# sources whose matrix-core loops keep their accumulators in VGPRs (without
+# the option the compiler copies loop-carried accumulators between VGPRs and
+# AGPRs around every product chain)
+MFMA_VGPR = {"tt_integrals.hip"}
 
 
 def _run(cmd):
@@ -52,6 +56,8 @@ def build_library(force=True, jobs=8):
             flags = HIPFLAGS
             if os.path.basename(s) in NO_CONTRACT:
                 flags = [f for f in HIPFLAGS if not f.startswith("-ffp-contract")] + ["-ffp-contract=off"]
+            if os.path.basename(s) in MFMA_VGPR:
+                flags = flags + ["-mllvm", "-amdgpu-mfma-vgpr-form"]
             cmd = [HIPCC, "-x", "hip"] + COMMON + flags + ["-c", s, "-o", o]
         else:
             cmd = [HIPCC, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"] + COMMON + ["-c", s, "-o", o]

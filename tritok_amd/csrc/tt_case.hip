@@ -248,7 +248,7 @@ struct tt_case {
   LsqrWork work;
   DBuf<double> vel, vort, qvort, temp, qtemp, pv, pw, pt, rhsv, ttime, b, x, cpart;
   DBuf<int> kode, kobc;
-  bool lsqr32 = false;
+  int lsqr_prec = kBlocks64;  // blocks the least-squares iterations read (BlockPrec)
   float dlse[9];
   LsqrProbe probe51, probe27;  // lsqr_A samples on 51-row (VTE, T) and 27-row (KM) systems
   double beta = 0, beta2 = 0, beta3 = 0, ren = 0;
@@ -323,7 +323,7 @@ int solve_subsys(tt_case &c, SubSys &S, const double *fnode, const double *fflux
   hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, c.s, n, c.cv.mesh.nnodes, S.unk_field.p,
                      fnode, fflux, c.x.p);
   TT_LAUNCH_CHECK();
-  LsqrResult r = block_lsqr(S.bs, c.work, c.b.p, c.x.p, c.cv.lsqs_maxit, eps4, c.lsqr32, c.s,
+  LsqrResult r = block_lsqr(S.bs, c.work, c.b.p, c.x.p, c.cv.lsqs_maxit, eps4, c.lsqr_prec, c.s,
                             S.R == 51 ? &c.probe51 : &c.probe27);
   if (r.istop > 3) throw std::runtime_error("LSQR stopped with istop > 3 (ill-conditioned or iteration limit)");
   return r.nit;
@@ -372,11 +372,14 @@ namespace {
 // by an FP64 step every anchor_period Picard iterations.
 void apply_precision(tt_case &c, int32_t prec) {
   if (prec < TT_PREC_FP64 || prec > TT_PREC_FP16) throw std::runtime_error("unknown precision");
-  if (prec != TT_PREC_FP64) {
-    for (SubSys *ss : {&c.vte[0], &c.vte[1], &c.vte[2], &c.km}) block_make32(ss->bs, c.s);
-    if (c.cv.iDT == 1) block_make32(c.tsys.bs, c.s);
+  // FP16 mode: half-precision blocks; FP32 and BF16 modes: single-precision blocks
+  const int bp = prec == TT_PREC_FP64 ? kBlocks64 : prec == TT_PREC_FP16 ? kBlocks16 : kBlocks32;
+  for (SubSys *ss : {&c.vte[0], &c.vte[1], &c.vte[2], &c.km, &c.tsys}) {
+    if (ss == &c.tsys && c.cv.iDT != 1) continue;
+    if (bp == kBlocks32) block_make32(ss->bs, c.s);
+    if (bp == kBlocks16) block_make16(ss->bs, c.s);
   }
-  c.lsqr32 = prec != TT_PREC_FP64;
+  c.lsqr_prec = bp;
   if (c.cv.iBw == 1) {
     const SdPrec sp = prec == TT_PREC_FP64   ? SdPrec::F64
                       : prec == TT_PREC_FP32 ? SdPrec::F32
@@ -437,19 +440,19 @@ extern "C" int tt_case_create(const tt_case_view *cv, const double *vel, const d
     TT_LAUNCH_CHECK();
     for (int e = 0; e < 3; ++e)
       build_subsys(*c, c->vte[e], 51, 51, c->Af_vte.p, cv->wkode + (size_t)nn * e, cv->wqkode + (size_t)nq * e,
-                   cv->nunk[e], c->lsqr32);
+                   cv->nunk[e], c->lsqr_prec == kBlocks32);
     if (cv->iDT == 1) {
       c->Af_t.alloc(tot51);
       hipLaunchKernelGGL(afull_kernel, dim3(ceil_div(tot51, 256)), dim3(256), 0, s, nc, 51, 51, M.H.p, M.G.p, M.B.p,
                          c->beta, c->ren * cv->PrN / 1.0, c->Af_t.p);
       TT_LAUNCH_CHECK();
-      build_subsys(*c, c->tsys, 51, 51, c->Af_t.p, cv->Tkode, cv->Tqkode, cv->Tnunk, c->lsqr32);
+      build_subsys(*c, c->tsys, 51, 51, c->Af_t.p, cv->Tkode, cv->Tqkode, cv->Tnunk, c->lsqr_prec == kBlocks32);
     }
     c->Af_km.alloc((size_t)nc * 729);
     hipLaunchKernelGGL(km_block_kernel, dim3(ceil_div((size_t)nc * 729, 256)), dim3(256), 0, s, nc, M.H.p,
                        c->Af_km.p);
     TT_LAUNCH_CHECK();
-    build_subsys(*c, c->km, 27, 27, c->Af_km.p, cv->kmu, nullptr, cv->nsmcol, c->lsqr32);
+    build_subsys(*c, c->km, 27, 27, c->Af_km.p, cv->kmu, nullptr, cv->nsmcol, c->lsqr_prec == kBlocks32);
     c->t_phase[10] = tsys.stop();
     c->b.alloc((size_t)nc * 51);
     int nmax = std::max({cv->nunk[0], cv->nunk[1], cv->nunk[2], cv->Tnunk, cv->nsmcol});
@@ -579,7 +582,13 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
     if (nit) std::memcpy(nit, its, sizeof(its));
     for (int k = 0; k < 7; ++k) c->t_phase[k] = tp[k];
     bool more = false;
-    for (int j = 0; j < 9; ++j) more |= e9[j] > cv.eps;
+    for (int j = 0; j < 9; ++j) {
+      if (!std::isfinite(e9[j])) {
+        c->finished = true;  // the reference would stop here as if converged (NaN > eps is false)
+        throw std::runtime_error("Picard iteration diverged (non-finite relative change)");
+      }
+      more |= e9[j] > cv.eps;
+    }
     if (!(c->it < cv.nnlit && more)) {
       c->finished = true;
       return 0;
@@ -632,7 +641,7 @@ extern "C" int64_t tt_case_device_bytes(tt_case *c) {
   for (auto &p : c->dense.lo16) add(p);
   for (auto &p : c->dense.lo32) add(p);
   add(c->Af_vte), add(c->Af_t), add(c->Af_km);
-  for (SubSys *ss : {&c->vte[0], &c->vte[1], &c->vte[2], &c->tsys, &c->km}) add(ss->bs.A), add(ss->bs.A32);
+  for (SubSys *ss : {&c->vte[0], &c->vte[1], &c->vte[2], &c->tsys, &c->km}) add(ss->bs.A), add(ss->bs.A32), add(ss->bs.A16);
   return b;
 }
 

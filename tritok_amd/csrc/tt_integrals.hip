@@ -116,12 +116,16 @@ struct MacroOut {
   double *ont;  // per row: 11 accuracy terms
 };
 
+// Macro matrices in three launches, one (cell, source) row at a time:
+//   macro_bnd_kernel  one thread per row: the boundary integrals over the
+//                     cell's six faces (H, G, Abd, Ht before corrections);
+//   macro_dom_kernel  one wavefront per row: the domain integral (B, D);
+//   macro_corr_kernel one thread per row: free term, Ht row sums and
+//                     linear-field corrections, accuracy estimates.
 // Helm: FMATmodHelm (lib_mHelm.f:2-483) with mu of each node / flux node in
-// mun / muq; otherwise FMATkmdcSteady.  The free term, Ht and linear-field
-// corrections follow all integrals of the row (the modified-Helmholtz free
-// term needs the row's B).
+// mun / muq; otherwise FMATkmdcSteady (lib_fmat.f:3-446).
 template <bool Helm>
-__global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__restrict__ qp, int ncomp,
+__global__ void __launch_bounds__(64) macro_bnd_kernel(MeshPtrs m, const QuadDev *__restrict__ qp, int ncomp,
                                                    const double *__restrict__ mun, const double *__restrict__ muq,
                                                    MacroOut o, unsigned long long *__restrict__ npts) {
   const int ic = blockIdx.x * 64 + threadIdx.x;
@@ -153,9 +157,6 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
       double dx = X(a, 0) - X(b, 0), dy = X(a, 1) - X(b, 1), dz = X(a, 2) - X(b, 2);
       minedge = fmin(minedge, sqrt(dx * dx + dy * dy + dz * dz));
     }
-  double *ont = o.ont + i * 12;
-  for (int k = 0; k < 11; ++k) ont[k] = 0.0;
-  ont[11] = mu;
   // ---- boundary integrals over the six faces
   int pts_b = 0;
   for (int je = 0; je < 6; ++je) {
@@ -189,30 +190,80 @@ __global__ void __launch_bounds__(64) macro_kernel(MeshPtrs m, const QuadDev *__
     double sg = (double)m.flpm[ic + nc * je];
     for (int s = 0; s < 4; ++s) o.G[i + m1 * (je * 4 + s)] += acc.ge[s] * sg;
   }
-  // ---- domain integrals
+  if (npts) atomicAdd(&npts[1], (unsigned long long)pts_b);
+}
+
+// the source point of row (ic, it) and its mu
+template <bool Helm>
+__device__ __forceinline__ void macro_source(const MeshPtrs &m, int ic, int it, const double *__restrict__ mun,
+                                             const double *__restrict__ muq, double *p, double &mu) {
+  const size_t nc = m.nc;
+  mu = 0.0;
+  if (it < 27) {
+    const int dn = m.idc[ic + nc * it];
+    p[0] = m.x[dn - 1], p[1] = m.x[dn - 1 + m.nn], p[2] = m.x[dn - 1 + 2 * (size_t)m.nn];
+    if (Helm) mu = mun[dn - 1];
+  } else {
+    const int its = (it - 27) / 4, itf = (it - 27) % 4 + 1;
+    const int dn = m.ibf[ic + nc * (its + 6 * (itf - 1))];
+    p[0] = m.xq[dn - 1], p[1] = m.xq[dn - 1 + m.nq], p[2] = m.xq[dn - 1 + 2 * (size_t)m.nq];
+    if (Helm) mu = muq[dn - 1];
+  }
+}
+
+template <bool Helm>
+__global__ void __launch_bounds__(64) macro_dom_kernel(MeshPtrs m, const QuadDev *__restrict__ qp,
+                                                       const double *__restrict__ mun, const double *__restrict__ muq,
+                                                       MacroOut o, unsigned long long *__restrict__ npts) {
+  __shared__ double lds[wave_cell_lds<4>()];
+  const int ic = blockIdx.x, it = blockIdx.y;
+  const size_t nc = m.nc, m1 = nc * 51, i = (size_t)ic * 51 + it;
+  double p[3], mu;
+  macro_source<Helm>(m, ic, it, mun, muq, p, mu);
   double cc[8][3];
-  for (int k = 0; k < 8; ++k)
-    for (int d = 0; d < 3; ++d) cc[k][d] = X(IDC(kGeo8[k]), d);
-  DomAcc<4, Helm> dacc;
-  dacc.zero();
-  dacc.mu = mu;
-  integrate_cell(q, p, cc, it + 1, dacc);
+  for (int k = 0; k < 8; ++k) {
+    const int node = m.idc[ic + nc * (kGeo8[k] - 1)];
+    for (int d = 0; d < 3; ++d) cc[k][d] = m.x[(node - 1) + (size_t)m.nn * d];
+  }
+  WaveCell<4, Helm> w;
+  w.run(*qp, p, cc, it + 1, mu, lds);
+  const int lane = threadIdx.x;
+  if (lane < 27) {
+    o.Dx[i + m1 * lane] = w.acc[0];
+    o.Dy[i + m1 * lane] = w.acc[1];
+    o.Dz[i + m1 * lane] = w.acc[2];
+    o.B[i + m1 * lane] = w.acc[3];
+  }
   if (npts) {
-    atomicAdd(&npts[0], (unsigned long long)dacc.npts);
-    atomicAdd(&npts[1], (unsigned long long)pts_b);
+    long n = w.npts;
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    if (lane == 0) atomicAdd(&npts[0], (unsigned long long)n);
   }
-  for (int s = 0; s < 27; ++s) {
-    o.Dx[i + m1 * s] = dacc.a[0][s];
-    o.Dy[i + m1 * s] = dacc.a[1][s];
-    o.Dz[i + m1 * s] = dacc.a[2][s];
-    o.B[i + m1 * s] = dacc.a[3][s];
-  }
+}
+
+template <bool Helm>
+__global__ void __launch_bounds__(64) macro_corr_kernel(MeshPtrs m, int ncomp, const double *__restrict__ mun,
+                                                        const double *__restrict__ muq, MacroOut o) {
+  const int ic = blockIdx.x * 64 + threadIdx.x;
+  const int it = blockIdx.y;  // source 0..50
+  if (ic >= ncomp) return;
+  const size_t nc = m.nc, m1 = nc * 51, m2 = nc * 27;
+  const size_t i = (size_t)ic * 51 + it, ii = (size_t)ic * 27 + it;
+  const bool usrc = it < 27;
+  auto X = [&](int node, int d) { return m.x[(node - 1) + (size_t)m.nn * d]; };
+  auto IDC = [&](int k) { return m.idc[ic + nc * (k - 1)]; };
+  double p[3], mu;
+  macro_source<Helm>(m, ic, it, mun, muq, p, mu);
+  const int its = usrc ? 0 : (it - 27) / 4, itf = usrc ? 0 : (it - 27) % 4 + 1;
+  double *ont = o.ont + i * 12;
+  for (int k = 0; k < 11; ++k) ont[k] = 0.0;
+  ont[11] = mu;
   // ---- free term (rigid body u = 1; modified Helmholtz: f = -mu^2) and Ht row sums
   double cfree = 0.0;
   if (Helm) {
     double vH = 0.0, vB = 0.0;
     for (int j = 0; j < 27; ++j) vH += o.H[i + m1 * j];
-    for (int j = 0; j < 27; ++j) vB += dacc.a[3][j];
+    for (int j = 0; j < 27; ++j) vB += o.B[i + m1 * j];
     cfree = -vH + mu * mu * vB;
   } else {
     for (int j = 0; j < 27; ++j) cfree -= o.H[i + m1 * j];
@@ -310,12 +361,24 @@ void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev
   TT_HIP(hipEventCreate(&e0));
   TT_HIP(hipEventCreate(&e1));
   TT_HIP(hipEventRecord(e0, s));
-  if (mun && muq)
-    hipLaunchKernelGGL(macro_kernel<true>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), dq.p, ncomp, mun,
-                       muq, out, cnt.p);
-  else
-    hipLaunchKernelGGL(macro_kernel<false>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), dq.p, ncomp,
+  if (mun && muq) {
+    hipLaunchKernelGGL(macro_bnd_kernel<true>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), dq.p, ncomp,
+                       mun, muq, out, cnt.p);
+    TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(macro_dom_kernel<true>, dim3(ncomp, 51), dim3(64), 0, s, ptrs(m), dq.p, mun, muq, out, cnt.p);
+    TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(macro_corr_kernel<true>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), ncomp, mun,
+                       muq, out);
+  } else {
+    hipLaunchKernelGGL(macro_bnd_kernel<false>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), dq.p, ncomp,
                        nullptr, nullptr, out, cnt.p);
+    TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(macro_dom_kernel<false>, dim3(ncomp, 51), dim3(64), 0, s, ptrs(m), dq.p, nullptr, nullptr, out,
+                       cnt.p);
+    TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(macro_corr_kernel<false>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), ncomp,
+                       nullptr, nullptr, out);
+  }
   TT_LAUNCH_CHECK();
   TT_HIP(hipEventRecord(e1, s));
   if (copy && m.nc > 1) {
@@ -389,34 +452,152 @@ __global__ void __launch_bounds__(64) sd_bnd_kernel(MeshPtrs m, const QuadDev *_
   }
 }
 
-__global__ void __launch_bounds__(64) sd_dom_kernel(MeshPtrs m, const QuadDev *__restrict__ qp, int row0, int nrows,
-                                                    SdOut o) {
-  const int r = blockIdx.x * 64 + threadIdx.x;
-  const bool active = r < nrows;
-  const int i = active ? row0 + r : row0;
-  const int dn = m.gbn[i];
-  const double p[3] = {m.x[dn - 1], m.x[dn - 1 + m.nn], m.x[dn - 1 + 2 * (size_t)m.nn]};
-  const size_t nc = m.nc, ld = o.ld;
-  for (int ic = 0; ic < m.nc; ++ic) {
-    int nodes[27];
-    bool sing = false;
-    for (int k = 0; k < 27; ++k) {
-      nodes[k] = m.idc[ic + nc * k];
-      sing |= nodes[k] == dn;
+// ---- regular domain integrals of the single-domain rows on the FP64 matrix cores.
+// The regular rule (INTEDc27dcLap with the node outside the cell) uses the same
+// reference-cell points in every cell, so the 27 x 3 integrals of a row over a
+// cell are sum_p phi_s(p) K_d(row, p), with phi the 27 shape functions at the
+// points and K_d = r_d / (4 pi r^3) w_p |J(p)| (the kernel of DomAcc<3>).  Per
+// cell that is a (32 x P) * (P x rows) product per component: the shape table
+// is the A operand and the kernel values, one (row, point) pair per lane, are
+// the B operand of v_mfma_f64_16x16x4f64.  Cells of one colour share no node,
+// so the waves of one launch add into disjoint entries of D.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// geo[(c*Pp + p)*4 + {0,1,2,3}] = position and weight w_p |J| of point p in
+// cell c (Pp: the point count padded to a multiple of 16; padding points have
+// weight 0); phi[p*32 + s] = shape function s at
+// point p (0 for s >= 27 and padding points).
+__global__ void sd_geom_kernel(MeshPtrs m, const QuadDev *__restrict__ qp, int Pp, double *__restrict__ geo,
+                               double *__restrict__ phi) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)m.nc * Pp) return;
+  const int c = (int)(t / Pp), p = (int)(t % Pp);
+  const QuadDev &q = *qp;
+  const int n1 = q.ndr, P = n1 * n1 * n1;
+  const size_t nc = m.nc;
+  double cc[8][3];
+  for (int k = 0; k < 8; ++k) {
+    const int node = m.idc[c + nc * (kGeo8[k] - 1)];
+    for (int d = 0; d < 3; ++d) cc[k][d] = m.x[node - 1 + (size_t)m.nn * d];
+  }
+  double *g = geo + t * 4;
+  double f27[27];
+  if (p < P) {
+    const int I = p % n1, J = (p / n1) % n1, K = p / (n1 * n1);
+    // one sub-cell: a = -1, b = 1 in every direction, dex = 1
+    const double gii = -1.0 + 0.5 * (q.gdr[I] + 1.0) * 2.0;
+    const double gij = -1.0 + 0.5 * (q.gdr[J] + 1.0) * 2.0;
+    const double gik = -1.0 + 0.5 * (q.gdr[K] + 1.0) * 2.0;
+    double xc[3], dj;
+    cell_map(cc, gii, gij, gik, xc, &dj);
+    g[0] = xc[0], g[1] = xc[1], g[2] = xc[2];
+    g[3] = 1.953125e-3 * dj * q.odr[I] * (q.odr[J] * q.odr[K]);
+    if (c == 0) shape27(gii, gij, gik, f27);
+  } else {
+    g[0] = cc[0][0], g[1] = cc[0][1], g[2] = cc[0][2], g[3] = 0.0;
+    for (int s = 0; s < 27; ++s) f27[s] = 0.0;
+  }
+  if (c == 0)
+    for (int s = 0; s < 32; ++s) phi[(size_t)p * 32 + s] = s < 27 ? f27[s] : 0.0;
+}
+
+// Block: 4 wavefronts, 16 rows each; blockIdx.y: a chunk of `chunk` cells of
+// the colour list `cells`.  Lane l evaluates row l%16 at point k0 + l/16.  The
+// row's own cells (singular) get zero weight here.
+// NP: 16-point passes per cell (Pp / 16) when known at compile time, so the
+// products of a cell form one unrolled chain with the accumulators in place;
+// NP = 0 loops over Pp / 16 passes.
+template <int NP>
+__global__ void __launch_bounds__(256) sd_dom_mfma_kernel(MeshPtrs m, int row0, int nrows, int Pp,
+                                                          const double *__restrict__ geo,
+                                                          const double *__restrict__ phi,
+                                                          const int *__restrict__ cells, int ncells, int chunk,
+                                                          const unsigned char *__restrict__ touches, SdOut o) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int n = lane & 15, kq = lane >> 4;
+  const int r = blockIdx.x * 64 + w * 16 + n;
+  const bool valid = r < nrows;
+  const int dn = valid ? m.gbn[row0 + r] : 0;
+  const size_t nn = m.nn, nc = m.nc, ld = o.ld;
+  const double px = valid ? m.x[dn - 1] : 0.0, py = valid ? m.x[dn - 1 + nn] : 0.0,
+               pz = valid ? m.x[dn - 1 + 2 * nn] : 0.0;
+  double *D[3] = {o.Dx, o.Dy, o.Dz};
+  const double k4pi = 0.25 / 3.141592653589793;
+  const int c0 = blockIdx.y * chunk, c1 = min(ncells, c0 + chunk);
+  // lane k < 27 holds node k of the cell; the next cell's ids are loaded one
+  // cell ahead
+  int cnext = c0 < c1 ? cells[c0] : 0;
+  int nidnext = (c0 < c1 && lane < 27) ? m.idc[cnext + nc * lane] : 0;
+  for (int ci = c0; ci < c1; ++ci) {
+    const int c = cnext, nid = nidnext;
+    if (ci + 1 < c1) {
+      cnext = cells[ci + 1];
+      nidnext = lane < 27 ? m.idc[cnext + nc * lane] : 0;
     }
-    double c[8][3];
-    for (int k = 0; k < 8; ++k)
-      for (int d = 0; d < 3; ++d) c[k][d] = m.x[nodes[kGeo8[k] - 1] - 1 + (size_t)m.nn * d];
-    if (!active || sing) continue;  // singular cells: sd_dom_singular_kernel
-    DomAcc<3> acc;
-    acc.zero();
-    integrate_cell(*qp, p, c, 0, acc);
-    for (int s = 0; s < 27; ++s) {
-      size_t j = nodes[s] - 1;
-      o.Dx[r + ld * j] += acc.a[0][s];
-      o.Dy[r + ld * j] += acc.a[1][s];
-      o.Dz[r + ld * j] += acc.a[2][s];
+    bool use = valid;
+    if (touches[c])
+      for (int k = 0; k < 27; ++k) use = use && __builtin_amdgcn_readlane(nid, k) != dn;
+    // the entries of D this lane adds to (register i of tile t: shape
+    // function 16t + 4i + kq, row n), read before the products
+    size_t jo[2][4];
+    double dv[3][2][4];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int sf = 16 * t + 4 * i + kq;
+        const int node = __shfl(nid, sf < 27 ? sf : 0, 64);
+        jo[t][i] = (size_t)r + ld * (size_t)(node - 1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dv[d][t][i] = (valid && sf < 27) ? D[d][jo[t][i]] : 0.0;
+      }
+    dbl4 acc[3][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d][0] = acc[d][1] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const double *g = geo + (size_t)c * Pp * 4;
+    // 16 points (four k-steps) per pass: all loads of the pass first, then the
+    // kernel values, then the 24 products
+    const int npass = NP > 0 ? NP : Pp / 16;
+#pragma unroll(NP > 0 ? NP : 1)
+    for (int ps = 0; ps < npass; ++ps) {
+      const int k0 = 16 * ps;
+      double gx[4], gy[4], gz[4], gw[4], a0[4], a1[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int p = k0 + 4 * q + kq;
+        const double4 v = *reinterpret_cast<const double4 *>(g + p * 4);
+        gx[q] = v.x, gy[q] = v.y, gz[q] = v.z, gw[q] = v.w;
+        a0[q] = phi[p * 32 + n], a1[q] = phi[p * 32 + 16 + n];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const double rx = px - gx[q], ry = py - gy[q], rz = pz - gz[q];
+        const double r2 = fmax(rx * rx + ry * ry + rz * rz, 1.0e-24);
+        // 1/r: hardware estimate and two Newton steps
+        double y = __builtin_amdgcn_rsq(r2);
+        y = fma(0.5 * y, fma(-r2 * y, y, 1.0), y);
+        y = fma(0.5 * y, fma(-r2 * y, y, 1.0), y);
+        const double f = use ? gw[q] * k4pi * (y * y * y) : 0.0;
+        const double b0 = rx * f, b1 = ry * f, b2 = rz * f;
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b0, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b1, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b1, acc[1][1], 0, 0, 0);
+        acc[2][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[q], b2, acc[2][0], 0, 0, 0);
+        acc[2][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[q], b2, acc[2][1], 0, 0, 0);
+      }
     }
+    // register i of lane l holds C[4i + l/16][l%16] (the f64 16x16x4 output
+    // layout): shape function 16t + 4i + kq, row n
+    if (!valid) continue;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if (16 * t + 4 * i + kq >= 27) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) D[d][jo[t][i]] = dv[d][t][i] + acc[d][t][i];
+      }
   }
 }
 
@@ -458,43 +639,75 @@ __global__ void sd_dom_singular_apply(MeshPtrs m, int row0, int nrows, const int
   }
 }
 
-// free term, Ht row sums and linear-field corrections (lib_sdkm.f:1387-1522)
-__global__ void sd_corr_kernel(MeshPtrs m, int row0, int nrows, SdOut o, double *__restrict__ ont) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// Free term, Ht row sums and linear-field corrections (lib_sdkm.f:1387-1522)
+// in two passes.  sd_sums_kernel: per row and column slice (blockIdx.y of
+// kSdSplit) the partial sums of the uncorrected rows:
+//   0 sum H, 1..3 sum Ht_t, 4..6 sum H(s) x_c(s) (c = z, x, y),
+//   7..9 sum D_t (t = z, x, y), 10..12 sum Htx y, Hty z, Htz x.
+// sd_apply_kernel: one thread per row adds the slices in slice order and
+// applies the corrections; the sums of the corrected rows that the estimates
+// use follow from the uncorrected sums and the corrections.
+constexpr int kSdSplit = 64, kSdSums = 13;
+
+__global__ void __launch_bounds__(64) sd_sums_kernel(MeshPtrs m, int nrows, SdOut o, double *__restrict__ part) {
+  const int r = blockIdx.x * 64 + threadIdx.x;
   if (r >= nrows) return;
   const size_t ld = o.ld, nb = m.nb, nn = m.nn;
+  const int sp = blockIdx.y;
+  auto X = [&](int node, int d) { return m.x[(node - 1) + nn * d]; };
+  double v[kSdSums];
+  for (int k = 0; k < kSdSums; ++k) v[k] = 0.0;
+  const size_t b0 = nb * sp / kSdSplit, b1 = nb * (sp + 1) / kSdSplit;
+  for (size_t j = b0; j < b1; ++j) {
+    const int g = m.gbn[j];
+    const double h = o.H[r + ld * j], tx = o.Htx[r + ld * j], ty = o.Hty[r + ld * j], tz = o.Htz[r + ld * j];
+    v[0] += h;
+    v[1] += tx, v[2] += ty, v[3] += tz;
+    v[4] += h * X(g, 2), v[5] += h * X(g, 0), v[6] += h * X(g, 1);
+    v[10] += tx * X(g, 1), v[11] += ty * X(g, 2), v[12] += tz * X(g, 0);
+  }
+  const size_t n0 = nn * sp / kSdSplit, n1 = nn * (sp + 1) / kSdSplit;
+  for (size_t j = n0; j < n1; ++j) v[7] += o.Dz[r + ld * j], v[8] += o.Dx[r + ld * j], v[9] += o.Dy[r + ld * j];
+  double *out = part + ((size_t)sp * nrows + r) * kSdSums;
+  for (int k = 0; k < kSdSums; ++k) out[k] = v[k];
+}
+
+__global__ void sd_apply_kernel(MeshPtrs m, int row0, int nrows, SdOut o, const double *__restrict__ part,
+                                double *__restrict__ ont) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const size_t ld = o.ld, nn = m.nn;
   const int i = row0 + r, dn = m.gbn[i];
   auto X = [&](int node, int d) { return m.x[(node - 1) + nn * d]; };
+  double v[kSdSums];
+  for (int k = 0; k < kSdSums; ++k) v[k] = 0.0;
+  for (int sp = 0; sp < kSdSplit; ++sp) {
+    const double *pp = part + ((size_t)sp * nrows + r) * kSdSums;
+    for (int k = 0; k < kSdSums; ++k) v[k] += pp[k];
+  }
   double *on = ont + (size_t)r * 10;
-  double cfree = 0.0;
-  for (size_t j = 0; j < nb; ++j) cfree -= o.H[r + ld * j];
+  // free term: rigid-body field u = 1
+  const double cfree = -v[0];
   on[0] = cfree;
   o.H[r + ld * i] += cfree;
   double *Ht[3] = {o.Htx, o.Hty, o.Htz};
   for (int t = 0; t < 3; ++t) {
-    double vs = 0.0;
-    for (size_t j = 0; j < nb; ++j) vs += Ht[t][r + ld * j];
-    on[1 + t] = fabs(vs);
-    Ht[t][r + ld * i] -= vs;
+    on[1 + t] = fabs(v[1 + t]);
+    Ht[t][r + ld * i] -= v[1 + t];
   }
+  // linear fields (z,0,0), (0,x,0), (0,0,y): sum_s H_is x_c(s) = sum_j D_ij
   double *Dm[3] = {o.Dz, o.Dx, o.Dy};
   const int crd[3] = {2, 0, 1}, slot[3] = {6, 4, 5};
+  double hv[3];
   for (int t = 0; t < 3; ++t) {
-    double hv = 0.0, vs = 0.0;
-    for (size_t s = 0; s < nb; ++s) hv += o.H[r + ld * s] * X(m.gbn[s], crd[t]);
-    for (size_t j = 0; j < nn; ++j) vs += Dm[t][r + ld * j];
-    on[slot[t]] = fabs(hv - vs);
-    Dm[t][r + ld * (dn - 1)] = Dm[t][r + ld * (dn - 1)] + hv - vs;
+    hv[t] = v[4 + t] + cfree * X(dn, crd[t]);
+    const double dlt = hv[t] - v[7 + t];
+    on[slot[t]] = fabs(dlt);
+    Dm[t][r + ld * (dn - 1)] = Dm[t][r + ld * (dn - 1)] + dlt;
   }
-  double v1 = 0, v2 = 0, v3 = 0, hxy = 0, hyz = 0, hzx = 0;
-  for (size_t j = 0; j < nn; ++j) v1 += o.Dz[r + ld * j], v2 += o.Dx[r + ld * j], v3 += o.Dy[r + ld * j];
-  for (size_t j = 0; j < nb; ++j) {
-    int g = m.gbn[j];
-    hxy += o.Htx[r + ld * j] * X(g, 1);
-    hyz += o.Hty[r + ld * j] * X(g, 2);
-    hzx += o.Htz[r + ld * j] * X(g, 0);
-  }
-  on[7] = fabs(hxy + v1), on[8] = fabs(hyz + v2), on[9] = fabs(hzx + v3);
+  // curl estimates on the corrected rows: sum_j D_t,ij is hv[t] after the correction
+  const double hxy = v[10] - v[1] * X(dn, 1), hyz = v[11] - v[2] * X(dn, 2), hzx = v[12] - v[3] * X(dn, 0);
+  on[7] = fabs(hxy + hv[0]), on[8] = fabs(hyz + hv[1]), on[9] = fabs(hzx + hv[2]);
 }
 
 void sd_integrals_dev(const DevMesh &m, const tt_mesh_view &hv, const QuadDev &qh, int row0, int row1, SdDev &o,
@@ -510,8 +723,61 @@ void sd_integrals_dev(const DevMesh &m, const tt_mesh_view &hv, const QuadDev &q
   SdOut out{nr, o.H.p, o.Htx.p, o.Hty.p, o.Htz.p, o.Dx.p, o.Dy.p, o.Dz.p};
   hipLaunchKernelGGL(sd_bnd_kernel, dim3(ceil_div(nr, 64)), dim3(64), 0, s, ptrs(m), dq.p, row0, nr, out);
   TT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(sd_dom_kernel, dim3(ceil_div(nr, 64)), dim3(64), 0, s, ptrs(m), dq.p, row0, nr, out);
-  TT_LAUNCH_CHECK();
+  // regular cells: colour the cells so that no two of one colour share a node
+  // (greedy, first free colour), then one product launch per colour
+  {
+    const int n1 = qh.ndr, Pp = (n1 * n1 * n1 + 15) / 16 * 16;
+    std::vector<uint64_t> used(m.nn, 0);
+    std::vector<int> colour(m.nc);
+    std::vector<unsigned char> touches(m.nc, 0);
+    int ncol = 0;
+    for (int ic = 0; ic < m.nc; ++ic) {
+      uint64_t mask = 0;
+      for (int k = 0; k < 27; ++k) {
+        const int nd = hv.idc[ic + (size_t)m.nc * k] - 1;
+        mask |= used[nd];
+        touches[ic] |= hv.lbn[nd] != 0;
+      }
+      if (~mask == 0) throw std::runtime_error("cell colouring needs more than 64 colours");
+      const int col = __builtin_ctzll(~mask);
+      colour[ic] = col, ncol = std::max(ncol, col + 1);
+      for (int k = 0; k < 27; ++k) used[hv.idc[ic + (size_t)m.nc * k] - 1] |= 1ull << col;
+    }
+    std::vector<int> cstart(ncol + 1, 0), clist(m.nc);
+    for (int ic = 0; ic < m.nc; ++ic) ++cstart[colour[ic] + 1];
+    for (int k = 0; k < ncol; ++k) cstart[k + 1] += cstart[k];
+    {
+      std::vector<int> fill(cstart.begin(), cstart.end() - 1);
+      for (int ic = 0; ic < m.nc; ++ic) clist[fill[colour[ic]]++] = ic;
+    }
+    DBuf<double> geo((size_t)m.nc * Pp * 4), phi((size_t)Pp * 32);
+    DBuf<int> dcl;
+    DBuf<unsigned char> dtouch;
+    dcl.upload(clist, s), dtouch.upload(touches, s);
+    hipLaunchKernelGGL(sd_geom_kernel, dim3(ceil_div((int)((size_t)m.nc * Pp), 256)), dim3(256), 0, s, ptrs(m), dq.p,
+                       Pp, geo.p, phi.p);
+    TT_LAUNCH_CHECK();
+    const int bx = ceil_div(nr, 64);
+    for (int k = 0; k < ncol; ++k) {
+      const int cnt = cstart[k + 1] - cstart[k];
+      const int chunk = std::max(1, (int)(((int64_t)cnt * bx + 4095) / 4096));
+      const dim3 grid(bx, ceil_div(cnt, chunk));
+      auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, ptrs(m), row0, nr, Pp, geo.p, phi.p, dcl.p + cstart[k], cnt,
+                           chunk, dtouch.p, out);
+      };
+      switch (Pp / 16) {
+        case 1: launch(sd_dom_mfma_kernel<1>); break;
+        case 2: launch(sd_dom_mfma_kernel<2>); break;
+        case 4: launch(sd_dom_mfma_kernel<4>); break;
+        case 8: launch(sd_dom_mfma_kernel<8>); break;
+        case 14: launch(sd_dom_mfma_kernel<14>); break;
+        default: launch(sd_dom_mfma_kernel<0>); break;
+      }
+      TT_LAUNCH_CHECK();
+    }
+    TT_HIP(hipStreamSynchronize(s));
+  }
   // singular (row, cell) pairs: cells containing the row's node, in cell order
   std::vector<std::vector<int>> node_cells(m.nn);
   for (int ic = 0; ic < m.nc; ++ic)
@@ -542,8 +808,11 @@ void sd_integrals_dev(const DevMesh &m, const tt_mesh_view &hv, const QuadDev &q
                        dpcell.p, staging.p, out);
     TT_LAUNCH_CHECK();
   }
-  DBuf<double> ontd((size_t)nr * 10);
-  hipLaunchKernelGGL(sd_corr_kernel, dim3(ceil_div(nr, 64)), dim3(64), 0, s, ptrs(m), row0, nr, out, ontd.p);
+  DBuf<double> ontd((size_t)nr * 10), part((size_t)kSdSplit * nr * kSdSums);
+  hipLaunchKernelGGL(sd_sums_kernel, dim3(ceil_div(nr, 64), kSdSplit), dim3(64), 0, s, ptrs(m), nr, out, part.p);
+  TT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(sd_apply_kernel, dim3(ceil_div(nr, 64)), dim3(64), 0, s, ptrs(m), row0, nr, out, part.p,
+                     ontd.p);
   TT_LAUNCH_CHECK();
   std::vector<double> oh(ontd.n);
   ontd.download(oh.data(), oh.size(), s);

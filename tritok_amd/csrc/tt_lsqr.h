@@ -33,17 +33,25 @@ struct LsqrState {
 // elements (a multiple of 4) so that every block starts on a 16-byte boundary
 // in both precisions and is read with 16-byte vector loads.
 inline int block_stride(int R, int L) { return (R * L + 3) / 4 * 4; }
+// 16-bit blocks: rows padded to a multiple of 8 entries (16 bytes), so that a
+// row is read with 16-byte loads; the padding columns hold zeros
+inline int block_row16(int L) { return (L + 7) / 8 * 8; }
+inline int block_stride16(int R, int L) { return R * block_row16(L); }
+
+// precision of the blocks an LSQR solve iterates with
+enum BlockPrec { kBlocks64 = 0, kBlocks32 = 1, kBlocks16 = 2 };
 
 struct BlockSys {
   int ncell = 0, R = 0, L = 0, n = 0, stride = 0;
   DBuf<double> A;         // scaled blocks: ncell*stride
   DBuf<float> A32;        // the same in single precision (mixed-precision solves)
+  DBuf<_Float16> A16;     // the same in half precision, rows padded to block_row16
   DBuf<int> colmap;       // ncell*L
   DBuf<int> tptr, tidx;   // for unknown j: entries tidx[tptr[j]..tptr[j+1]) = cell*L + l
   int gw = 0;             // gather width: max cells per unknown, rounded up to 4
   DBuf<int> gtab;         // n*gw: the entries of tidx per unknown in cell order, -1 padded
   DBuf<double> q;         // column scaling
-  bool has32 = false;
+  bool has32 = false, has16 = false;
   int last_nit = 0;       // iterations of the previous solve (sizes the first launch batch)
   float last_eps = 0.f;   // its tolerance
 };
@@ -84,13 +92,14 @@ struct LsqrResult {
 };
 
 // slvlsqr2 on a block system: b (m = ncell*R) on the device, x holds the hot
-// start on entry and the solution on exit (device).  use32 selects the
-// single-precision blocks for the iterations; the initial residual
-// b - A*x0 always uses the double-precision blocks.
-LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, bool use32,
+// start on entry and the solution on exit (device).  prec (BlockPrec) selects
+// the blocks the iterations read (FP64; FP32 or FP16 with FP32 arithmetic);
+// the initial residual b - A*x0 always uses the double-precision blocks.
+LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
                       hipStream_t st, LsqrProbe *probe = nullptr);
 
-// single-precision copy of the scaled blocks (built on demand)
+// single- and half-precision copies of the scaled blocks (built on demand)
 void block_make32(BlockSys &s, hipStream_t st);
+void block_make16(BlockSys &s, hipStream_t st);
 
 }  // namespace tt

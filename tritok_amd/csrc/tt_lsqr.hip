@@ -57,6 +57,21 @@ template <>
 struct Vec16<float> {
   using type = float4;
 };
+template <>
+struct Vec16<_Float16> {
+  using type = uint4;  // eight halves, moved as bits
+};
+
+// padded block length of an R x L block in element type T (16-byte multiple;
+// 16-bit blocks have rows padded to row_len)
+template <typename T, int L>
+constexpr int row_len() {
+  return sizeof(T) == 2 ? (L + 7) / 8 * 8 : L;
+}
+template <typename T, int R, int L>
+constexpr int block_len() {
+  return sizeof(T) == 2 ? R * row_len<T, L>() : (R * L + 3) / 4 * 4;
+}
 
 // Loads one padded block into LDS: all 16-byte loads of a lane are issued
 // before the first LDS store.
@@ -135,6 +150,17 @@ __global__ void to32_kernel(size_t n, const double *__restrict__ a, float *__res
   if (k < n) b[k] = (float)a[k];
 }
 
+// FP64 blocks (stride S, rows of L) -> FP16 blocks (rows padded to LP,
+// stride R*LP), padding zero
+__global__ void to16_kernel(size_t total16, int L, int LP, int S, int S16, const double *__restrict__ a,
+                            _Float16 *__restrict__ b) {
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= total16) return;
+  const size_t c = k / S16;
+  const int e = (int)(k % S16), r = e / LP, l = e % LP;
+  b[k] = l < L ? (_Float16)a[c * S + (size_t)r * L + l] : (_Float16)0.0f;
+}
+
 constexpr int kNB = 256;  // threads per block of the unknown kernel
 
 // ---------------------------------------------------------------------------
@@ -179,8 +205,9 @@ __global__ void __launch_bounds__(64) lsqr_r0(const double *__restrict__ A, cons
 // ---------------------------------------------------------------------------
 // iteration k, cell part.  Block 0 runs the scalar recurrences (stopping
 // tests of iteration k-2, rotation of iteration k-1) and stores the state;
-// blocks 1..G do u' = A v - alpha u and P = A^T u' for cells b-1, b-1+G, ...
-// (G sized to the blocks resident on the device at once), prefetching the
+// blocks 1..G do u' = A v - alpha u and P = A^T u' for a contiguous range of
+// about ncell/G cells (G sized to the blocks resident on the device at once;
+// the ranges of one XCD adjacent, see xcd_logical), prefetching the
 // next cell's block into registers while computing on the current one.  The
 // cell blocks need only alpha_k = ||v_k||, which every block reduces from the
 // unknown kernel's partials in the same order; when block 0 finds that the
@@ -192,11 +219,11 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
                                              double *__restrict__ P, double *__restrict__ pa1,
                                              const double *__restrict__ pb1, const double *__restrict__ pb2, int nb,
                                              const LsqrState *__restrict__ sin, LsqrState *__restrict__ sout) {
-  constexpr int S = (R * L + 3) / 4 * 4;
+  constexpr int S = block_len<T, R, L>();
   __shared__ __align__(16) T blk[S];
   __shared__ double vs[64], uu[64], sh[1];
-  __shared__ float vsf[64], uuf[64];
-  constexpr bool f32 = sizeof(T) == 4;  // FP32 blocks: FP32 products, two accumulators
+  __shared__ __align__(16) float vsf[64], uuf[64];
+  constexpr bool f32 = sizeof(T) <= 4;  // FP32 / FP16 blocks: FP32 products, two accumulators
   const int t = threadIdx.x;
   if (blockIdx.x == 0) {
     const double nvs = reduce_partials<64>(pb1, nb, sh);
@@ -209,8 +236,10 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
     return;
   }
   if (sin->done) return;
-  const int G = gridDim.x - 1;
-  int c = blockIdx.x - 1;
+  // this block's cells: a contiguous range, consecutive ranges on one XCD
+  const int G = gridDim.x - 1, lb = xcd_logical(blockIdx.x, 1, G + 1);
+  const int cend = (int)((long long)(lb + 1) * ncell / G);
+  int c = (int)((long long)lb * ncell / G);
   int j = t < L ? colmap[(size_t)c * L + t] : -1;
   double uo = t < R ? u[(size_t)c * R + t] : 0.0;
   RegBlock<T, S> rb;
@@ -231,10 +260,10 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
         vs[t] = vn;
     }
     __syncthreads();
-    const int cn = c + G;
+    const int cn = c + 1;
     int jn = -1;
     double uon = 0.0;
-    if (cn < ncell) {
+    if (cn < cend) {
       jn = t < L ? colmap[(size_t)cn * L + t] : -1;
       uon = t < R ? u[(size_t)cn * R + t] : 0.0;
       rb.load(A + (size_t)cn * S);
@@ -243,7 +272,28 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
     {
       double s = 0.0;
       const int row = t < R ? t : R - 1;
-      if constexpr (f32) {
+      if constexpr (sizeof(T) == 2) {
+        // row of LP halves (zero padded), eight per 16-byte read; v padded with zeros
+        constexpr int LP = row_len<T, L>();
+        const uint4 *rp = reinterpret_cast<const uint4 *>(blk + row * LP);
+        float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+        for (int k = 0; k < LP / 8; ++k) {
+          const uint4 cw = rp[k];
+          const float4 va = *reinterpret_cast<const float4 *>(vsf + 8 * k);
+          const float4 vb = *reinterpret_cast<const float4 *>(vsf + 8 * k + 4);
+          const _Float16 *h = reinterpret_cast<const _Float16 *>(&cw);
+          f0 += (float)h[0] * va.x;
+          f1 += (float)h[1] * va.y;
+          f0 += (float)h[2] * va.z;
+          f1 += (float)h[3] * va.w;
+          f0 += (float)h[4] * vb.x;
+          f1 += (float)h[5] * vb.y;
+          f0 += (float)h[6] * vb.z;
+          f1 += (float)h[7] * vb.w;
+        }
+        s = (double)f0 + (double)f1;
+      } else if constexpr (f32) {
         float f0 = 0.f, f1 = 0.f;
 #pragma unroll
         for (int l = 0; l + 1 < L; l += 2) {
@@ -267,7 +317,31 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
     else
       uu[t] = un;
     __syncthreads();
-    {
+    if constexpr (sizeof(T) == 2) {
+      // lane t: columns 2p, 2p+1 (p = t mod 32) over one half of the rows
+      // (t / 32), four-byte reads; the halves are added across the wave
+      constexpr int LP = row_len<T, L>(), NPAIR = (L + 1) / 2, RH = (R + 1) / 2;
+      const int p = t & 31, hf = t >> 5, i0 = hf ? RH : 0;
+      const int pc = p < NPAIR ? p : 0;
+      float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+      for (int k = 0; k < RH; ++k) {
+        const int i = i0 + k;
+        if (i < R) {
+          const unsigned int pw = *reinterpret_cast<const unsigned int *>(blk + i * LP + 2 * pc);
+          const _Float16 *h = reinterpret_cast<const _Float16 *>(&pw);
+          const float ui = uuf[i];
+          s0 += (float)h[0] * ui;
+          s1 += (float)h[1] * ui;
+        }
+      }
+      s0 += __shfl_xor(s0, 32, 64);
+      s1 += __shfl_xor(s1, 32, 64);
+      if (hf == 0 && p < NPAIR) {
+        P[(size_t)c * L + 2 * p] = (double)s0;
+        if (2 * p + 1 < L) P[(size_t)c * L + 2 * p + 1] = (double)s1;
+      }
+    } else {
       double s = 0.0;
       const int col = t < L ? t : L - 1;
       if constexpr (f32) {
@@ -286,7 +360,7 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
       if (t < L) P[(size_t)c * L + t] = s;
     }
     acc += un * un;
-    if (cn >= ncell) break;
+    if (cn >= cend) break;
     vj = jn >= 0 ? vp[jn] : 0.0;
     j = jn, uo = uon, c = cn;
     __syncthreads();  // blk, vs and uu are rewritten next
@@ -305,7 +379,7 @@ __global__ void __launch_bounds__(kNB) lsqr_B(int n, int k, const int *__restric
                                               double *__restrict__ pb2, LsqrState *st) {
   __shared__ double sh[kNB / 64];
   if (k >= 1 && st->done) return;
-  const int j = blockIdx.x * kNB + threadIdx.x;
+  const int j = xcd_logical(blockIdx.x, 0, gridDim.x) * kNB + threadIdx.x;
   int idx[W];
   if (j < n) {
     const int4 *g = reinterpret_cast<const int4 *>(gtab + (size_t)j * W);
@@ -315,14 +389,19 @@ __global__ void __launch_bounds__(kNB) lsqr_B(int n, int k, const int *__restric
       idx[4 * q] = v.x, idx[4 * q + 1] = v.y, idx[4 * q + 2] = v.z, idx[4 * q + 3] = v.w;
     }
   }
+  // the column's gathers and vector loads are issued before the partial-sum
+  // reduction, so their latency overlaps it
+  double pv[W], vold = 0.0, wo = 0.0, xo = 0.0;
+  if (j < n) {
+#pragma unroll
+    for (int e = 0; e < W; ++e) pv[e] = idx[e] >= 0 ? P[idx[e]] : 0.0;
+    vold = vp[j];
+    if (k >= 2) wo = w[j], xo = x[j];
+  }
   const double beta = sqrt(reduce_partials<kNB>(pa1, na, sh));  // beta_{k+1}
   const double alpha = st->alpha, t1 = st->t1, t2 = st->t2, t3 = st->t3;
   double nv = 0.0, dk = 0.0;
   if (j < n) {
-    double pv[W];
-#pragma unroll
-    for (int e = 0; e < W; ++e) pv[e] = idx[e] >= 0 ? P[idx[e]] : 0.0;
-    const double vold = vp[j];
     double vk = 0.0;
     if (k >= 1) {
       vk = alpha > 0.0 ? vold / alpha : vold;
@@ -331,8 +410,7 @@ __global__ void __launch_bounds__(kNB) lsqr_B(int n, int k, const int *__restric
         x[j] = 0.0;
       } else {
         // x/w update of iteration k-1 (rotation computed by this iteration's cell kernel)
-        const double wo = w[j];
-        x[j] = t1 * wo + x[j];
+        x[j] = t1 * wo + xo;
         w[j] = t2 * wo + vk;
         const double tt = t3 * wo;
         dk = tt * tt;
@@ -427,14 +505,18 @@ void r0(BlockSys &s, LsqrWork &wk, const double *b, const double *x, hipStream_t
   else
     launch_r0<27, 27>(s, wk, b, x, st);
 }
-void stepA(BlockSys &s, LsqrWork &wk, int k, bool use32, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+void stepA(BlockSys &s, LsqrWork &wk, int k, int prec, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
   if (s.R == 51) {
-    if (use32)
+    if (prec == kBlocks16)
+      launch_A<_Float16, 51, 51>(s, wk, k, (const _Float16 *)s.A16.p, st, e0, e1);
+    else if (prec == kBlocks32)
       launch_A<float, 51, 51>(s, wk, k, (const float *)s.A32.p, st, e0, e1);
     else
       launch_A<double, 51, 51>(s, wk, k, (const double *)s.A.p, st, e0, e1);
   } else {
-    if (use32)
+    if (prec == kBlocks16)
+      launch_A<_Float16, 27, 27>(s, wk, k, (const _Float16 *)s.A16.p, st, e0, e1);
+    else if (prec == kBlocks32)
       launch_A<float, 27, 27>(s, wk, k, (const float *)s.A32.p, st, e0, e1);
     else
       launch_A<double, 27, 27>(s, wk, k, (const double *)s.A.p, st, e0, e1);
@@ -525,6 +607,17 @@ void block_setup(BlockSys &s, int ncell, int R, int L, int n, const std::vector<
   TT_HIP(hipStreamSynchronize(st));  // host vectors above are released on return
 }
 
+void block_make16(BlockSys &s, hipStream_t st) {
+  if (s.has16) return;
+  const int S16 = block_stride16(s.R, s.L);
+  const size_t total16 = (size_t)s.ncell * S16;
+  s.A16.alloc(total16);
+  hipLaunchKernelGGL(to16_kernel, dim3(ceil_div(total16, 256)), dim3(256), 0, st, total16, s.L, block_row16(s.L),
+                     s.stride, S16, s.A.p, s.A16.p);
+  TT_LAUNCH_CHECK();
+  s.has16 = true;
+}
+
 void block_make32(BlockSys &s, hipStream_t st) {
   if (s.has32) return;
   size_t total = (size_t)s.ncell * s.stride;
@@ -556,11 +649,12 @@ LsqrWork::~LsqrWork() {
   if (host_state) (void)hipHostFree(host_state);
 }
 
-LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, bool use32,
+LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
                       hipStream_t st, LsqrProbe *probe) {
   const int m = s.ncell * s.R, n = s.n;
   wk.ensure(m, n, s.ncell, s.L);
-  if (use32 && !s.has32) throw std::runtime_error("single-precision blocks not built");
+  if (prec == kBlocks32 && !s.has32) throw std::runtime_error("single-precision blocks not built");
+  if (prec == kBlocks16 && !s.has16) throw std::runtime_error("half-precision blocks not built");
   LsqrState init;
   std::memset(&init, 0, sizeof(init));
   init.cs2 = -1.0;
@@ -588,7 +682,7 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
   for (;;) {
     for (int b2 = 0; b2 < batch; ++b2, ++k) {
       const long pi = probe_begin(probe, k);
-      stepA(s, wk, k, use32, st, pi >= 0 ? probe->pool[2 * pi] : nullptr, pi >= 0 ? probe->pool[2 * pi + 1] : nullptr);
+      stepA(s, wk, k, prec, st, pi >= 0 ? probe->pool[2 * pi] : nullptr, pi >= 0 ? probe->pool[2 * pi + 1] : nullptr);
       stepB(s, wk, k, st);
     }
     TT_LAUNCH_CHECK();

@@ -15,6 +15,20 @@ __device__ __forceinline__ double d2norm(double a, double b) {
   return scale * sqrt(sa * sa + sb * sb);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on XCD
+// b % 8), and each XCD has its own L2.  xcd_logical numbers the workgroups of
+// [lo, hi) so that those of one XCD take consecutive indices: a kernel that
+// hands out consecutive cells or unknowns by this index keeps each XCD's
+// gathers within one region of the vector it gathers from.  A bijection of
+// [lo, hi) onto [0, hi - lo) for any dispatch order; only the locality assumes
+// the round-robin.
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_logical(int b, int lo, int hi) {
+  const int N = hi - lo, q = N / kXcd, rem = N % kXcd, p = b - lo;
+  const int dx = p % kXcd;  // position of b's XCD in the order starting at lo's
+  return dx * q + min(dx, rem) + p / kXcd;
+}
+
 // sum over the 64 lanes of a wavefront (butterfly; every lane gets the total)
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

@@ -289,6 +289,199 @@ __device__ void integrate_cell(const QuadDev &q, const double *p, const double (
       }
 }
 
+// ---------------------------------------------------------------------------
+// Wavefront-parallel domain integral of one (source, cell) pair.  The same
+// quadrature points as integrate_cell, taken 64 at a time: in the first phase
+// every lane evaluates one point (geometry, Jacobian, kernel) and stages its
+// three 1-D quadratic Lagrange factor triples and NC weighted kernel values in
+// LDS; in the second phase lane k of each half-wave (k < 27) accumulates shape
+// function k's NC sums over that half's 32 points.  Each lane holds NC
+// accumulators instead of 27 NC, so nothing spills.  The 27-node shape
+// functions are the products l_a(xi) l_b(eta) l_c(zeta) of the quadratic
+// Lagrange factors (a, b, c the node's position in {-1, 0, 1}, cshape27's node
+// order), equal to cshape27 to rounding.
+__constant__ static const signed char kNode27[27][3] = {
+    {-1, 1, 1},  {-1, 1, -1}, {1, 1, -1},  {1, 1, 1},  {-1, -1, 1}, {-1, -1, -1}, {1, -1, -1},
+    {1, -1, 1},  {-1, 1, 0},  {0, 1, -1},  {1, 1, 0},  {0, 1, 1},   {-1, 0, 1},   {-1, 0, -1},
+    {1, 0, -1},  {1, 0, 1},   {-1, -1, 0}, {0, -1, -1}, {1, -1, 0}, {0, -1, 1},   {0, 1, 0},
+    {-1, 0, 0},  {0, 0, -1},  {1, 0, 0},   {0, 0, 1},  {0, -1, 0},  {0, 0, 0}};
+
+// LDS doubles one wavefront needs for wave_cell<NC>
+template <int NC>
+constexpr int wave_cell_lds() {
+  return (9 + NC) * 64;
+}
+
+template <int NC, bool Helm>
+struct WaveCell {
+  double acc[NC];  // lane k < 27 of the lower half-wave: the sums of shape function k
+  long npts = 0;   // points this lane evaluated
+
+  __device__ __forceinline__ void stage(const double *p, const double (&c)[8][3], double mu, bool ok, double eti,
+                                        double etj, double etk, double wpre, bool skip_zero, double *lds,
+                                        int ax, int ay, int az) {
+    const int lane = threadIdx.x & 63;
+    double f[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) f[q] = 0.0;
+    if (ok) {
+      ++npts;
+      double xc[3], dj;
+      cell_map(c, eti, etj, etk, xc, &dj);
+      const double ajac = wpre * dj;
+      const double rx = p[0] - xc[0], ry = p[1] - xc[1], rz = p[2] - xc[2];
+      double ra = sqrt(rx * rx + ry * ry + rz * rz);
+      if (!(skip_zero && ra == 0.0)) {
+        if (ra < 1.0e-12) ra = 1.0e-12;
+        const double ira2 = 1.0 / (ra * ra);
+        double fung, gf;
+        kernel_factors<Helm>(ra, mu, fung, gf);
+        f[0] = rx * ira2 * fung * gf * ajac;
+        f[1] = ry * ira2 * fung * gf * ajac;
+        f[2] = rz * ira2 * fung * gf * ajac;
+        if (NC == 4) f[NC - 1] = fung * ajac;
+      }
+    }
+    double *L = lds, *F = lds + 9 * 64;
+    const double t[3] = {eti, etj, etk};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      L[(3 * d + 0) * 64 + lane] = 0.5 * t[d] * (t[d] - 1.0);
+      L[(3 * d + 1) * 64 + lane] = (1.0 - t[d]) * (1.0 + t[d]);
+      L[(3 * d + 2) * 64 + lane] = 0.5 * t[d] * (t[d] + 1.0);
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) F[q * 64 + lane] = f[q];
+    __syncthreads();
+    const int half = lane >> 5;
+#pragma unroll 4
+    for (int j = 0; j < 32; ++j) {
+      const int pi = half * 32 + j;
+      const double phi = L[ax * 64 + pi] * L[(3 + ay) * 64 + pi] * L[(6 + az) * 64 + pi];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) acc[q] += F[q * 64 + pi] * phi;
+    }
+    __syncthreads();
+  }
+
+  // the integral over the cell; on return lanes 0..26 hold the sums of shape
+  // functions 0..26 (both halves combined)
+  __device__ void run(const QuadDev &q, const double *p, const double (&c)[8][3], int isrc, double mu, double *lds) {
+    const double PI = 3.141592653589793, PI2 = 2.0 * PI;
+    const int lane = threadIdx.x & 63, k = lane & 31;
+    const int ax = k < 27 ? kNode27[k][0] + 1 : 1, ay = k < 27 ? kNode27[k][1] + 1 : 1,
+              az = k < 27 ? kNode27[k][2] + 1 : 1;
+#pragma unroll
+    for (int qq = 0; qq < NC; ++qq) acc[qq] = 0.0;
+    const int nd = isrc != 0 ? q.iiDiv : 1;
+    for (int ix = 1; ix <= nd; ++ix)
+      for (int iy = 1; iy <= nd; ++iy)
+        for (int iz = 1; iz <= nd; ++iz) {
+          const double a = -1.0 + (ix - 1) * 2.0 / nd, b = -1.0 + ix * 2.0 / nd;
+          const double cc = -1.0 + (iy - 1) * 2.0 / nd, d = -1.0 + iy * 2.0 / nd;
+          const double e = -1.0 + (iz - 1) * 2.0 / nd, f = -1.0 + iz * 2.0 / nd;
+          const double dex = 0.125 * (b - a) * (d - cc) * (f - e);
+          bool ising = false;
+          double xx1 = 0, yy1 = 0, zz1 = 0;
+          if (isrc != 0) {
+            xx1 = ((-1.0 + 2.0 * kKsi51[isrc - 1]) - a) / (b - a);
+            yy1 = ((-1.0 + 2.0 * kEta51[isrc - 1]) - cc) / (d - cc);
+            zz1 = ((-1.0 + 2.0 * kZet51[isrc - 1]) - e) / (f - e);
+            ising = xx1 >= 0.0 && xx1 <= 1.0 && yy1 >= 0.0 && yy1 <= 1.0 && zz1 >= 0.0 && zz1 <= 1.0;
+          }
+          if (ising) {
+            for (int Ls = 1; Ls <= 4; ++Ls) {
+              double aa0, gama, ffi0, ffi1;
+              if (Ls == 1) {
+                if (1.0 - xx1 == 0.0) continue;
+                aa0 = sqrt(yy1 * yy1 + (1.0 - xx1) * (1.0 - xx1));
+                gama = acos(yy1 / aa0);
+                ffi0 = 1.5 * PI + acos(yy1 / aa0);
+                ffi1 = PI2 + atan((1.0 - yy1) / (1.0 - xx1));
+              } else if (Ls == 2) {
+                if (1.0 - yy1 == 0.0) continue;
+                aa0 = sqrt((1.0 - yy1) * (1.0 - yy1) + (1.0 - xx1) * (1.0 - xx1));
+                gama = acos((1.0 - xx1) / aa0);
+                ffi0 = asin((1.0 - yy1) / aa0);
+                ffi1 = 0.5 * PI + atan(xx1 / (1.0 - yy1));
+              } else if (Ls == 3) {
+                if (xx1 == 0.0) continue;
+                aa0 = sqrt(xx1 * xx1 + (1.0 - yy1) * (1.0 - yy1));
+                gama = acos((1.0 - yy1) / aa0);
+                ffi0 = 0.5 * PI + acos((1.0 - yy1) / aa0);
+                ffi1 = PI + atan(yy1 / xx1);
+              } else {
+                if (yy1 == 0.0) continue;
+                aa0 = sqrt(yy1 * yy1 + xx1 * xx1);
+                gama = acos(xx1 / aa0);
+                ffi0 = PI + acos(xx1 / aa0);
+                ffi1 = 1.5 * PI + atan((1.0 - xx1) / yy1);
+              }
+              for (int K = 0; K < q.nds; ++K) {
+                const double ffi = (ffi1 - ffi0) * q.gds[K] / 2.0 + (ffi1 + ffi0) / 2.0;
+                const double ro0 = aa0 * sin(gama) / sin(ffi - ffi0 + gama);
+                const double sf = sin(ffi), cf = cos(ffi);
+                for (int N = 1; N <= 3; ++N) {
+                  double aa1, eps, th0, th1;
+                  if (N == 1) {
+                    aa1 = 1.0 - zz1;
+                    eps = PI / 2.0;
+                    th0 = 0.0;
+                    th1 = acos(aa1 / sqrt(aa1 * aa1 + ro0 * ro0));
+                  } else if (N == 2) {
+                    aa1 = sqrt(ro0 * ro0 + (1.0 - zz1) * (1.0 - zz1));
+                    eps = acos((1.0 - zz1) / aa1);
+                    th0 = eps;
+                    const double rr1 = fmin(1.0, ro0 / sqrt(zz1 * zz1 + ro0 * ro0));
+                    th1 = 0.5 * PI + acos(rr1);
+                  } else {
+                    aa1 = sqrt(ro0 * ro0 + zz1 * zz1);
+                    const double rr1 = fmin(1.0, ro0 / aa1);
+                    eps = acos(rr1);
+                    th0 = 0.5 * PI + eps;
+                    th1 = PI;
+                  }
+                  const int np = q.nds * q.nds;
+                  for (int base = 0; base < np; base += 64) {
+                    const int t = base + lane;
+                    const bool ok = t < np;
+                    const int I = ok ? t / q.nds : 0, J = ok ? t % q.nds : 0;
+                    const double th = (th1 - th0) * q.gds[I] / 2.0 + (th1 + th0) / 2.0;
+                    const double ro1 = aa1 * sin(eps) / sin(th - th0 + eps);
+                    const double sth = sin(th), cth = cos(th);
+                    const double ro = (ro1 * q.gds[J] + ro1) / 2.0;
+                    double eti = 2.0 * (xx1 + ro * sth * cf) - 1.0;
+                    double etj = 2.0 * (yy1 + ro * sth * sf) - 1.0;
+                    double etk = 2.0 * (zz1 + ro * cth) - 1.0;
+                    eti = a + 0.5 * (eti + 1.0) * (b - a);
+                    etj = cc + 0.5 * (etj + 1.0) * (d - cc);
+                    etk = e + 0.5 * (etk + 1.0) * (f - e);
+                    const double wpre = dex * 1.953125e-3 * q.ods[J] * (q.ods[I] * q.ods[K]) * (ffi1 - ffi0) *
+                                        (th1 - th0) * ro1 * (ro * ro) * sth;
+                    stage(p, c, mu, ok, eti, etj, etk, wpre, true, lds, ax, ay, az);
+                  }
+                }
+              }
+            }
+          } else {
+            const int n1 = q.ndr, np = n1 * n1 * n1;
+            for (int base = 0; base < np; base += 64) {
+              const int t = base + lane;
+              const bool ok = t < np;
+              const int I = ok ? t % n1 : 0, J = ok ? (t / n1) % n1 : 0, K = ok ? t / (n1 * n1) : 0;
+              const double gii = a + 0.5 * (q.gdr[I] + 1.0) * (b - a);
+              const double gij = cc + 0.5 * (q.gdr[J] + 1.0) * (d - cc);
+              const double gik = e + 0.5 * (q.gdr[K] + 1.0) * (f - e);
+              const double wpre = 1.953125e-3 * q.odr[I] * (q.odr[J] * q.odr[K] * dex);
+              stage(p, c, mu, ok, gii, gij, gik, wpre, false, lds, ax, ay, az);
+            }
+          }
+        }
+#pragma unroll
+    for (int qq = 0; qq < NC; ++qq) acc[qq] += __shfl_xor(acc[qq], 32, 64);
+  }
+};
+
 // Accumulator of the 9-node face integrals; Macro also keeps ge and ad*.
 template <bool Macro, bool Helm = false>
 struct BndAcc {

@@ -35,7 +35,7 @@ MODULE tritok_amd_bind
     INTEGER(C_INT32_T) :: iBw
   END TYPE
 
-  INTEGER(C_INT32_T), PARAMETER :: TT_PREC_FP64 = 0, TT_PREC_FP32 = 1
+  INTEGER(C_INT32_T), PARAMETER :: TT_PREC_FP64 = 0, TT_PREC_FP32 = 1, TT_PREC_BF16 = 2, TT_PREC_FP16 = 3
 
   INTERFACE
     FUNCTION tt_last_error() BIND(C, NAME="tt_last_error")
@@ -174,6 +174,128 @@ MODULE tritok_amd_bind
       INTEGER(C_INT) :: tt_case_fields
     END FUNCTION
     SUBROUTINE tt_case_free(h) BIND(C, NAME="tt_case_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+    ! precision of the iteration matrices and the FP64 anchor period
+    FUNCTION tt_case_set_precision(h, prec) BIND(C, NAME="tt_case_set_precision")
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: prec
+      INTEGER(C_INT) :: tt_case_set_precision
+    END FUNCTION
+    FUNCTION tt_case_set_anchor_period(h, period) BIND(C, NAME="tt_case_set_anchor_period")
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: period
+      INTEGER(C_INT) :: tt_case_set_anchor_period
+    END FUNCTION
+
+    ! AcaDo (lib_fm.f:2003): aca_CrossAprox / acaj_CrossAprox / aca_RankCrossAprox
+    FUNCTION tt_aca_compress(m, n, a, typ, eps, vec, h, rank, norm) BIND(C, NAME="tt_aca_compress")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      INTEGER(C_INT32_T), VALUE :: m, n, typ
+      REAL(C_DOUBLE), INTENT(IN) :: a(*), vec(*)
+      REAL(C_DOUBLE), VALUE :: eps
+      TYPE(C_PTR) :: h
+      INTEGER(C_INT32_T) :: rank
+      REAL(C_DOUBLE) :: norm
+      INTEGER(C_INT) :: tt_aca_compress
+    END FUNCTION
+    FUNCTION tt_aca_factors(h, a, b) BIND(C, NAME="tt_aca_factors")
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE) :: a(*), b(*)
+      INTEGER(C_INT) :: tt_aca_factors
+    END FUNCTION
+    ! aca_abxv (lib_ACA.f:226)
+    FUNCTION tt_aca_mv(h, x, y) BIND(C, NAME="tt_aca_mv")
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(IN) :: x(*)
+      REAL(C_DOUBLE) :: y(*)
+      INTEGER(C_INT) :: tt_aca_mv
+    END FUNCTION
+    SUBROUTINE tt_aca_free(h) BIND(C, NAME="tt_aca_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+
+    ! Che_WcomMatrix (lib_WT.f:148), Che_FormCRS (:355), Che_WTAWx (:297)
+    FUNCTION tt_wt_compress(m, n, a, kappa, h, nnz) BIND(C, NAME="tt_wt_compress")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      INTEGER(C_INT32_T), VALUE :: m, n
+      REAL(C_DOUBLE), INTENT(IN) :: a(*)
+      REAL(C_DOUBLE), VALUE :: kappa
+      TYPE(C_PTR) :: h
+      INTEGER(C_INT32_T) :: nnz
+      INTEGER(C_INT) :: tt_wt_compress
+    END FUNCTION
+    FUNCTION tt_wt_csr(h, zvr, c, v) BIND(C, NAME="tt_wt_csr")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T) :: zvr(*), c(*)
+      REAL(C_DOUBLE) :: v(*)
+      INTEGER(C_INT) :: tt_wt_csr
+    END FUNCTION
+    FUNCTION tt_wt_mv(h, x, y) BIND(C, NAME="tt_wt_mv")
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(IN) :: x(*)
+      REAL(C_DOUBLE) :: y(*)
+      INTEGER(C_INT) :: tt_wt_mv
+    END FUNCTION
+    SUBROUTINE tt_wt_free(h) BIND(C, NAME="tt_wt_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+
+    ! cTree_FillMatrices (lib_ctree.f:2) on HmatrixType parts, cTree_MxV (:99)
+    FUNCTION tt_hmat_create_parts(nrow, ncol, a, lda, nparts, rptr, rnodes, cptr, cnodes, admiss, ctype, &
+                                  aca_type, aca_eps, wt_kappa, h) BIND(C, NAME="tt_hmat_create_parts")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+      INTEGER(C_INT32_T), VALUE :: nrow, ncol, nparts, ctype, aca_type
+      REAL(C_DOUBLE), INTENT(IN) :: a(*)
+      INTEGER(C_INT64_T), VALUE :: lda
+      INTEGER(C_INT32_T), INTENT(IN) :: rptr(*), rnodes(*), cptr(*), cnodes(*), admiss(*)
+      REAL(C_DOUBLE), VALUE :: aca_eps, wt_kappa
+      TYPE(C_PTR) :: h
+      INTEGER(C_INT) :: tt_hmat_create_parts
+    END FUNCTION
+    FUNCTION tt_hmat_mv(h, nrhs, x, y) BIND(C, NAME="tt_hmat_mv")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: nrhs
+      REAL(C_DOUBLE), INTENT(IN) :: x(*)
+      REAL(C_DOUBLE) :: y(*)
+      INTEGER(C_INT) :: tt_hmat_mv
+    END FUNCTION
+    SUBROUTINE tt_hmat_free(h) BIND(C, NAME="tt_hmat_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+
+    ! many independent prelsqr2 / slvlsqr2 systems in one launch
+    FUNCTION tt_lsqr_batch_setup(nsys, m, n, nnz, ia, ja, a, q, scale_host, h) BIND(C, NAME="tt_lsqr_batch_setup")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+      INTEGER(C_INT32_T), VALUE :: nsys, scale_host
+      INTEGER(C_INT32_T), INTENT(IN) :: m(*), n(*), ia(*), ja(*)
+      INTEGER(C_INT64_T), INTENT(IN) :: nnz(*)
+      REAL(C_DOUBLE) :: a(*), q(*)
+      TYPE(C_PTR) :: h
+      INTEGER(C_INT) :: tt_lsqr_batch_setup
+    END FUNCTION
+    FUNCTION tt_lsqr_batch_solve(h, b, x, maxit, eps4, nit, istop, ms) BIND(C, NAME="tt_lsqr_batch_solve")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_FLOAT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(IN) :: b(*)
+      REAL(C_DOUBLE) :: x(*), ms
+      INTEGER(C_INT32_T), VALUE :: maxit
+      REAL(C_FLOAT), VALUE :: eps4
+      INTEGER(C_INT32_T) :: nit(*), istop(*)
+      INTEGER(C_INT) :: tt_lsqr_batch_solve
+    END FUNCTION
+    SUBROUTINE tt_lsqr_batch_free(h) BIND(C, NAME="tt_lsqr_batch_free")
       IMPORT :: C_PTR
       TYPE(C_PTR), VALUE :: h
     END SUBROUTINE

@@ -76,6 +76,7 @@ class Lsqr:
 _bsetup = _lib._sig("tt_lsqr_batch_setup", i32, i32, PI, PI, C.POINTER(C.c_int64), PI, PI, PD, PD, i32,
                     C.POINTER(P))
 _bsolve = _lib._sig("tt_lsqr_batch_solve", i32, P, PD, PD, i32, C.c_float, PI, PI, C.POINTER(C.c_double))
+_bwidth = _lib._sig("tt_lsqr_batch_set_width", i32, P, i32, PI)
 _bfree = _lib._sig("tt_lsqr_batch_free", None, P)
 
 
@@ -100,6 +101,12 @@ class LsqrBatch:
         self._h = h
         self.moff = np.concatenate([[0], np.cumsum(self.m)])
         self.noff = np.concatenate([[0], np.cumsum(self.n)])
+
+    def set_width(self, width):
+        """Gang width of every system (0 = automatic); returns the widest gang."""
+        mw = C.c_int32()
+        check(_bwidth(self._h, int(width), C.byref(mw)), "tt_lsqr_batch_set_width")
+        return mw.value
 
     def solve(self, b, x0, maxit, eps4):
         """b, x0: lists (or concatenated arrays).  Returns (xs, istop, nit, ms)."""
