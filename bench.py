@@ -237,6 +237,95 @@ def batched_solves_leg(count, seed=20251203):
                          "bytes": "per system iteration 2*((8+4)*nnz + 4*(m+1)) + 8*(3m + 6n)"}}
 
 
+def mixed_kinematics_leg(n, tol=1e-10):
+    """Mixed-precision kinematics on an n^3 cube: the x equation's system
+    (nbnodes x nbnodes) solved for four right-hand sides by FP64 iterative
+    refinement with its inverse held in FP32, BF16 and FP16 (per-row 2^k
+    scales, matrix cores, FP32 accumulation), against the FP64 solve."""
+    from tritok_amd import integrals as I
+    from tritok_amd import solvers as S
+    from tritok_amd import workloads
+    host = workloads.natural_convection(n, ra=1e5, pr=0.71)
+    m = host.mesh_arrays()
+    nb = len(m["gbn"])
+    t0 = time.perf_counter()
+    sd = S.SdKinematics(host, I.quad(4, 4, 4, 4, 2))
+    setup = time.perf_counter() - t0
+    # the boundary values of a smooth velocity and temperature field as the
+    # four right-hand-side columns (u_x, u_y, u_z, T)
+    p = m["x"][m["gbn"] - 1]
+    x, y, z = p[:, 0], p[:, 1], p[:, 2]
+    B = np.stack([np.sin(np.pi * y) * np.sin(np.pi * z), np.sin(np.pi * z) * np.sin(np.pi * x),
+                  np.sin(np.pi * x) * np.sin(np.pi * y), x - 0.5], axis=1)
+    X64, it64, rr64, ms64, mv64 = sd.refine(1, "fp64", B, tol=1e-14, maxit=4)
+    out = {"workload": "kinematics x equation of the %d^3 heated cube (%d x %d), right-hand sides u_x, u_y, u_z, T "
+                       "(smooth fields at the boundary nodes), FP64 iterative refinement to %g" % (n, nb, nb, tol),
+           "nbnodes": nb, "setup_s": setup,
+           "fp64": {"corrections": it64, "relres": rr64, "ms": ms64, "ms_product": mv64}}
+    bytes_per = {"fp32": 4, "bf16": 2, "fp16": 2}
+    for prec in ("fp32", "bf16", "fp16"):
+        sd.refine(1, prec, B, tol=tol, maxit=60)  # first use builds the tile copy
+        X, it, rr, ms, mv = sd.refine(1, prec, B, tol=tol, maxit=60)
+        err = float(np.linalg.norm(X - X64) / np.linalg.norm(X64))
+        gbs = nb * nb * bytes_per[prec] / (mv * 1e-3) / 1e9
+        tflops = 2.0 * nb * nb * 16 / (mv * 1e-3) / 1e12  # the instructions' 16 columns
+        log("mixed kinematics %s: %d corrections to relres %.2e, error vs FP64 %.2e, %.2f ms (product %.3f ms, "
+            "%.0f GB/s)" % (prec, it, rr, err, ms, mv, gbs))
+        out[prec] = {"corrections": it, "relres": rr, "rel_err_vs_fp64": err, "ms": ms, "ms_product": mv,
+                     "product_GBps": gbs, "product_hbm_frac": gbs / HBM_PEAK_GBS, "mfma_tflops": tflops}
+    log("mixed kinematics fp64: %d corrections, relres %.2e, %.2f ms (product %.3f ms)" % (it64, rr64, ms64, mv64))
+    sd.close()
+    return out
+
+
+FLOP_SDKM_DOMAIN = 3 * 27 * 2 + 15  # three gradient outputs x 27 shape functions, plus the kernel
+
+
+def sdkm_regular_points(host, points_per_cell=512):
+    """Regular (row, cell, point) evaluations of the single-domain domain
+    integrals: every boundary row against every cell that does not hold its
+    node, at the 8-point rule's 512 points per cell (GAUS 4)."""
+    m = host.mesh_arrays()
+    nb, nc = len(m["gbn"]), m["idc"].shape[0]
+    cells_of = np.bincount(m["idc"].ravel() - 1, minlength=m["nn"])  # cells holding each node
+    singular = int(cells_of[m["gbn"] - 1].sum())
+    return (nb * nc - singular) * points_per_cell
+
+
+def assembly_entry(host, tm, macro_stats):
+    """Integral-assembly throughput of the case setup: device times of the
+    macro and single-domain matrices, credited flop per evaluation as in
+    the batched-integrals leg (singular transforms and geometry uncredited)."""
+    pts = sdkm_regular_points(host)
+    sd_tf = pts * FLOP_SDKM_DOMAIN / (tm["setup_sdkm"] * 1e-3) / 1e12
+    mac = macro_stats["domain_points"] * FLOP_DOMAIN_MH + macro_stats["boundary_points"] * FLOP_BOUNDARY_MH
+    mac_tf = mac / (tm["setup_macro"] * 1e-3) / 1e12 if tm["setup_macro"] > 0 else None
+    log("assembly: macro %.0f ms (%.2f TFLOP/s credited), sdkm %.0f ms (%.3g regular points, %.1f TFLOP/s credited "
+        "over the whole sdkm setup)" % (tm["setup_macro"], mac_tf or 0.0, tm["setup_sdkm"], pts, sd_tf))
+    return {"macro_ms": tm["setup_macro"], "macro_points": {"domain": macro_stats["domain_points"],
+                                                            "boundary": macro_stats["boundary_points"]},
+            "macro_tflops": mac_tf, "sdkm_ms": tm["setup_sdkm"], "sdkm_regular_points": pts,
+            "sdkm_tflops": sd_tf, "sdkm_frac_fp64_peak": sd_tf / FP64_PEAK_TFLOPS, "lu_ms": tm["setup_lu"],
+            "flop_per_point": {"sdkm_domain": FLOP_SDKM_DOMAIN, "macro_domain": FLOP_DOMAIN_MH,
+                               "macro_boundary": FLOP_BOUNDARY_MH}}
+
+
+def assembly_only(n):
+    """Macro and single-domain integrals of the n^3 heated cube through the
+    routine-level entry points (no factorisation), for counter runs."""
+    from tritok_amd import integrals as I
+    from tritok_amd import workloads
+    host = workloads.natural_convection(n, ra=1e5, pr=0.71)
+    q = I.quad(4, 4, 4, 4, 2)
+    t0 = time.perf_counter()
+    I.macro_integrals(host.view(), q)
+    t1 = time.perf_counter()
+    I.sd_integrals(host.view(), q)
+    t2 = time.perf_counter()
+    log("assembly only %d^3: macro %.2f s, sdkm %.2f s (with transfers), %.3g sdkm regular points" % (
+        n, t1 - t0, t2 - t1, sdkm_regular_points(host)))
+
+
 def lsqr_matrix_bytes(info, value_bytes):
     """Algorithmic bytes of the two sparse products of one LSQR iteration for a
     CSR system (values + int32 column indices, row pointers), per the
@@ -265,8 +354,12 @@ def main():
     ap.add_argument("--serve-max-iter", type=int, default=400)
     ap.add_argument("--batch-cells", type=int, default=16,
                     help="n: batched modified-Helmholtz integrals of n^3 independent cells (0: skip)")
-    ap.add_argument("--batch-solves", type=int, default=256,
+    ap.add_argument("--batch-solves", type=int, default=1024,
                     help="independent cavity systems solved in one launch (0: skip)")
+    ap.add_argument("--mixed-kin", type=int, default=24,
+                    help="n: mixed-precision kinematics solve on an n^3 cube (0: skip)")
+    ap.add_argument("--assembly-only", type=int, default=0,
+                    help="n: only the macro and single-domain integrals of an n^3 cube, then exit (counter runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB of a device-to-device copy issued first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -275,9 +368,14 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.assembly_only:
+        assembly_only(args.assembly_only)
+        return
+
     from tritok_amd import workloads
     from tritok_amd.case import Case
     from tritok_amd import device_info
+    from tritok_amd import integrals as I
 
     dev = device_info()
     log("device", dev)
@@ -296,6 +394,7 @@ def main():
     log("setup %.2fs (macro %.0f ms, sdkm %.0f ms, LU %.0f ms, systems %.0f ms), device bytes %.2f GB" % (
         setup_s, tm["setup_macro"], tm["setup_sdkm"], tm["setup_lu"], tm["setup_systems"],
         case.device_bytes() / 1e9))
+    assembly = assembly_entry(host, tm, I.last_integral_stats()) if not args.no_kinematics else None
 
     legs = {}
     for prec in ("fp64", args.mixed):
@@ -317,6 +416,8 @@ def main():
     kname = "lsqr_A<_Float16,51,51>" if half else "lsqr_A<float,51,51>"
     vbytes = 2 if half else 4
     infos = [case.system_info(s) for s in range(4)]
+    tuned = {"vorticity_energy": {k: infos[0][k] for k in ("grid_fp64", "grid_fp32", "grid_fp16")},
+             "kinematics": {k: case.system_info(4)[k] for k in ("grid_fp64", "grid_fp32", "grid_fp16")}}
     nit = mixed["nits"].sum(0)
     w = np.array([nit[6], nit[7], nit[8], nit[9]], float)
     alg = np.array([lsqr_matrix_bytes(i, vbytes) for i in infos])
@@ -330,7 +431,7 @@ def main():
                 "traffic": traffic,
                 "traffic_GBps": traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None,
                 "traffic_frac": traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic and avg_ms > 0 else None,
-                "avg_launch_ms": avg_ms, "launches_timed": mixed["probe51"]["sampled"],
+                "avg_launch_ms": avg_ms, "launches_timed": mixed["probe51"]["sampled"], "tuned_grid": tuned,
                 "launches": mixed["probe51"]["launches"], "algorithmic_bytes_per_launch": alg_bytes,
                 "block_bytes_per_launch": float(infos[0]["cells"] * (2608 * 2 if half else 2604 * 4))}
 
@@ -342,6 +443,7 @@ def main():
 
     case.close()  # free the 20^3 case before the other legs
     acc = accuracy_leg(args.accuracy_n, args.mixed) if args.accuracy_n > 0 else None
+    mixkin = mixed_kinematics_leg(args.mixed_kin) if args.mixed_kin > 0 else None
     batched = batched_integrals_leg(args.batch_cells) if args.batch_cells > 0 else None
     bsolves = batched_solves_leg(args.batch_solves) if args.batch_solves > 0 else None
     serving = (serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, ("fp64", args.mixed))
@@ -370,7 +472,8 @@ def main():
                                 "mixed": (nit[3:10] / args.steps).tolist()},
         "setup_s": setup_s, "device": dev, "roofline": roofline, "cpu_baseline": cpu,
         "p99_solve_latency_s": serving[args.mixed]["latency_s"]["p99"] if serving else None,
-        "serving": serving, "batched_integrals": batched, "batched_solves": bsolves,
+        "serving": serving, "batched_integrals": batched, "batched_solves": bsolves, "mixed_kinematics": mixkin,
+        "assembly": assembly,
     }
     print(json.dumps(out), flush=True)
 

@@ -250,6 +250,19 @@ int tt_sd_create(const tt_mesh_view *m, const tt_quad *q, const int32_t *kode, t
  * boundary vorticities from vel(nnodes,3) and vort(nnodes,3), relaxed with
  * urBw on rows with kode 0/3 and written back into vort. */
 int tt_sd_solve(tt_sd *h, const double *vel, double *vort, const double urBw[3]);
+/* Mixed-precision solve of one kinematics equation (eqn 1..3; no reference
+ * counterpart -- the reference solves with dgesl in FP64, lib_Solve_v11.f:
+ * 905-1037): A_eqn X = B for nrhs = 1..4 right-hand sides (B, X host arrays
+ * nbnodes x nrhs, column-major) by FP64 iterative refinement, X <- X + M (B -
+ * A X) until ||B - A X||_F <= tol ||B||_F or maxit corrections.  M is the
+ * inverse held in prec: TT_PREC_FP64 (FP64 products), TT_PREC_FP32, _BF16 or
+ * _FP16 (a copy with a power-of-two scale per row, applied on the matrix
+ * cores with FP32 accumulation to the residual scaled per column by a power
+ * of two; made on first use and kept).  *iters = corrections, *relres = the
+ * final relative residual, ms (may be NULL) = {device ms of the solve, mean
+ * device ms of one correction product}. */
+int tt_sd_refine(tt_sd *h, int32_t eqn, int32_t prec, int32_t nrhs, const double *B, double *X, double tol,
+                 int32_t maxit, int32_t *iters, double *relres, double ms[2]);
 void tt_sd_free(tt_sd *h);
 
 /* ------------------------------------------------------------------------ */
@@ -317,6 +330,11 @@ int tt_case_reset(tt_case *c, const double *vel, const double *vort, const doubl
 int tt_case_set_precision(tt_case *c, int32_t prec);
 /* Anchor period of the reduced-precision kinematics (>= 1; 1 = every step FP64). */
 int tt_case_set_anchor_period(tt_case *c, int32_t period);
+/* Grid of the least-squares cell kernel (no reference counterpart; a tuning
+ * and test hook): grid > 0 uses that many blocks for every system and block
+ * precision, 0 returns to the autotuner's choice.  Results do not depend on
+ * it (the partial sums are per fixed group of cells). */
+int tt_case_set_grid(tt_case *c, int32_t grid);
 /* Measurement hooks (no reference counterpart): time every stride-th launch
  * of the least-squares block kernel with HIP events on the case's stream
  * (stride 0 = off; resets the statistics), and read the average duration,
@@ -327,8 +345,9 @@ int tt_case_probe_stats(tt_case *c, int32_t rows, double *avg_ms, int64_t *sampl
                         int64_t *launches);
 /* Sizes of a least-squares system (sys 0..2 vorticity x,y,z, 3 energy, 4
  * kinematics): info = {rows m, unknowns n, nonzeros, cells, rows per cell,
- * columns per cell}. */
-int tt_case_system_info(tt_case *c, int32_t sys, int64_t info[6]);
+ * columns per cell, then the cell-kernel grid the autotuner chose for FP64,
+ * FP32 and FP16 blocks (0: not tuned yet)}. */
+int tt_case_system_info(tt_case *c, int32_t sys, int64_t info[9]);
 /* One Picard iteration (main.f90:1200-1589): err[9] = the nine relative
  * changes (tri.sta), nit[12] = solver iterations (tri.ite).  Returns 1 while
  * the loop should continue, 0 when converged or at nnlit, < 0 on error --

@@ -9,11 +9,11 @@ fields too.  Parity bars:
     anchored by FP64 steps);
   * default cases (RELA eps 1e-6, LSQR eps 1e-7 with the dynamic tolerance):
     the iteration trajectory follows the reference (same iteration count
-    within 2, per-iteration changes within 1e-2 relative over the first ten
+    within 2, per-iteration changes within 5e-2 relative over the first ten
     iterations; LSQR stopping counts may differ by one where a stopping test
     sits at its threshold) and the converged fields agree within 2e-5
-    relative L2, the
-    scale of the reference's own stopping tolerance.
+    relative L2, the scale of the reference's own stopping tolerance.  dc8 is
+    the 8^3 cavity (4,913 nodes) run to a change of 1e-6 in FP64.
 """
 import numpy as np
 import pytest
@@ -57,7 +57,7 @@ def test_tight_case_matches_reference(case, prec, gauss_table):
     _compare(g, f, 1e-8, case + "/" + prec)
 
 
-@pytest.mark.parametrize("case", ["dc2", "nc2", "jit2"])
+@pytest.mark.parametrize("case", ["dc2", "nc2", "jit2", "dc8"])
 def test_default_case_follows_reference(case, gauss_table):
     g = load_golden(case)
     print("case", case, flush=True)
@@ -79,6 +79,28 @@ def test_default_case_follows_reference(case, gauss_table):
             if r > 1e-12:
                 assert abs(sta[it, j] - r) <= 5e-2 * r, (it, j, sta[it, j], r)
     _compare(g, f, 2e-5, case)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_results_do_not_depend_on_cell_kernel_grid(prec, gauss_table):
+    """The autotuner only changes how the cells are spread over blocks: the
+    least-squares partial sums are per fixed group of cells, so any grid
+    gives the same bits."""
+    from tritok_amd import workloads
+    from tritok_amd.case import Case
+    h = workloads.natural_convection(6, ra=1e4)  # 216 cells: 27 groups of 8
+    outs = []
+    for grid in (0, 1, 5):
+        with Case(h, prec=prec) as c:
+            c.set_grid(grid)
+            sta, ite = c.run(max_iter=6)
+            outs.append((grid, sta, ite, c.fields(), c.system_info(0)))
+    print(prec, [(o[0], o[4]["grid_%s" % prec]) for o in outs], flush=True)
+    assert outs[0][4]["grid_%s" % prec] > 0
+    for o in outs[1:]:
+        assert np.array_equal(o[1], outs[0][1]) and np.array_equal(o[2], outs[0][2])
+        for k in FIELDS:
+            assert np.array_equal(o[3][k], outs[0][3][k]), (o[0], k)
 
 
 def test_case_is_bitwise_repeatable(gauss_table):

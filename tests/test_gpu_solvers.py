@@ -160,6 +160,53 @@ def test_sd_kinematics_step_matches_oracle(case, gauss_table):
     sd.close()
 
 
+def _oracle_sys(h, g, e):
+    """A_e of sdKm_sysm (lib_sdkm.f:1171-1210) from the reference's sdkm matrices."""
+    v = h.view()
+    m = v.mesh
+    nb = m.nbnodes
+    gbn = np.ctypeslib.as_array(m.gbn, shape=(nb,)).copy()
+    kd = np.ctypeslib.as_array(v.kode, shape=(3 * nb,)).copy().reshape(3, nb)[e]
+    nx, ny, nz = (np.ctypeslib.as_array(p, shape=(nb,)).copy() for p in (m.nx, m.ny, m.nz))
+    D = [np.asfortranarray(g["sd_" + k]) for k in ("Dx", "Dy", "Dz")]
+    A = np.zeros((nb, nb))
+    for i in range(nb):
+        if kd[i] in (0, 3):
+            A[i] = nx[i] * D[0][i, gbn - 1] + ny[i] * D[1][i, gbn - 1] + nz[i] * D[2][i, gbn - 1]
+        else:
+            A[i, i] = 1.0
+    return A
+
+
+@pytest.mark.parametrize("case", ["dc2", "jit2"])
+def test_sd_refine_every_precision_reaches_fp64(case, gauss_table):
+    """Four right-hand sides solved by FP64 iterative refinement with the
+    inverse in FP64, FP32, BF16 and FP16 on the matrix cores: every precision
+    reaches the 1e-10 relative residual (checked here in numpy) and the
+    oracle's dgefa/dgesl solution."""
+    from tritok_amd import integrals as I, solvers as S
+    g = load_golden(case)
+    h = host_for(g)
+    sd = S.SdKinematics(h, I.quad(4, 4, 4, 4, 2, table=gauss_table))
+    rng = np.random.default_rng(31)
+    for e in range(3):
+        A = _oracle_sys(h, g, e)
+        B = rng.standard_normal((A.shape[0], 4))
+        lu, piv, _ = O.dgefa(A)
+        X_ref = np.stack([O.dgesl(lu, piv, B[:, c].copy()) for c in range(4)], axis=1)
+        its = {}
+        for prec in ("fp64", "fp32", "bf16", "fp16"):
+            X, it, rr, ms, ms_mv = sd.refine(e + 1, prec, B, tol=1e-10, maxit=60)
+            res = np.linalg.norm(B - A @ X) / np.linalg.norm(B)
+            err = rel_l2(X, X_ref)
+            print(case, e, prec, "corrections", it, "relres", rr, res, "err", err, flush=True)
+            assert rr <= 1e-10 and res <= 2e-10
+            assert err < 1e-8
+            its[prec] = it
+        assert its["fp64"] <= 2 and its["fp16"] >= its["fp32"]
+    sd.close()
+
+
 def test_lsqr_batch_matches_oracle_per_system():
     """Systems of different sizes and kinds in one launch: each agrees with
     the oracle's slvlsqr2 as the single-system solver does."""

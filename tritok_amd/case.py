@@ -27,6 +27,7 @@ _setanchor = _lib._sig("tt_case_set_anchor_period", i32, P, i32)
 _probe = _lib._sig("tt_case_probe", i32, P, i32)
 _probe_stats = _lib._sig("tt_case_probe_stats", i32, P, i32, C.POINTER(C.c_double), C.POINTER(i64), C.POINTER(i64))
 _sysinfo = _lib._sig("tt_case_system_info", i32, P, i32, C.POINTER(i64))
+_setgrid = _lib._sig("tt_case_set_grid", i32, P, i32)
 
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
 TIMING_KEYS = ["boundary_vorticity", "km", "temperature", "vte", "convergence", "reserved", "step_total",
@@ -106,11 +107,16 @@ class Case:
         check(_probe_stats(self._h, int(rows), C.byref(avg), C.byref(sampled), C.byref(seen)), "tt_case_probe_stats")
         return dict(avg_ms=avg.value, sampled=sampled.value, launches=seen.value)
 
+    def set_grid(self, grid):
+        """Cell-kernel grid for every least-squares system (0: autotuned)."""
+        check(_setgrid(self._h, int(grid)), "tt_case_set_grid")
+
     def system_info(self, sys):
         """sys: 0..2 vorticity x, y, z; 3 energy; 4 kinematics."""
-        info = (i64 * 6)()
+        info = (i64 * 9)()
         check(_sysinfo(self._h, int(sys), info), "tt_case_system_info")
-        return dict(zip(["m", "n", "nnz", "cells", "rows_per_cell", "cols_per_cell"], list(info)))
+        return dict(zip(["m", "n", "nnz", "cells", "rows_per_cell", "cols_per_cell", "grid_fp64", "grid_fp32",
+                         "grid_fp16"], list(info)))
 
     def device_bytes(self):
         return int(_bytes(self._h))

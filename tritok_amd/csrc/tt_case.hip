@@ -721,13 +721,23 @@ extern "C" int tt_case_probe_stats(tt_case *c, int32_t rows, double *avg_ms, int
   });
 }
 
-extern "C" int tt_case_system_info(tt_case *c, int32_t sys, int64_t info[6]) {
+extern "C" int tt_case_set_grid(tt_case *c, int32_t grid) {
+  return guarded("tt_case_set_grid", [&] {
+    if (!c || grid < 0) throw std::runtime_error("grid must be >= 0");
+    for (SubSys *ss : {&c->vte[0], &c->vte[1], &c->vte[2], &c->tsys, &c->km})
+      for (int p = 0; p < 3; ++p) ss->bs.grid_a[p] = grid;  // 0: tuned again on next use
+    return 0;
+  });
+}
+
+extern "C" int tt_case_system_info(tt_case *c, int32_t sys, int64_t info[9]) {
   return guarded("tt_case_system_info", [&] {
     if (!c || !info) throw std::runtime_error("NULL argument");
     if (sys < 0 || sys > 4) throw std::runtime_error("sys must be 0..4 (VTE x, y, z, T, KM)");
     if (sys == 3 && c->cv.iDT != 1) throw std::runtime_error("the case has no energy equation");
     const SubSys &S = sys < 3 ? c->vte[sys] : sys == 3 ? c->tsys : c->km;
     info[0] = (int64_t)S.ncell * S.R, info[1] = S.n, info[2] = S.nnz, info[3] = S.ncell, info[4] = S.R, info[5] = S.L;
+    for (int p = 0; p < 3; ++p) info[6 + p] = S.bs.grid_a[p];
     return 0;
   });
 }

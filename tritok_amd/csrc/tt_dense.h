@@ -12,6 +12,9 @@ namespace tt {
 // Storage precision of the dense kinematics matrices for the products.
 enum class SdPrec { F64 = 0, F32 = 1, BF16 = 2, F16 = 3 };
 
+struct DenseMx;  // tile-major reduced-precision inverses (dense_refine)
+void dense_mx_free(DenseMx *p);
+
 struct DenseKin {
   int nb = 0, nn = 0;
   SdDev mats;                 // H, Htx, Hty, Htz (nb x nb), Dx, Dy, Dz (nb x nn), FP64
@@ -32,7 +35,27 @@ struct DenseKin {
   int anchor_period = 8;      // FP64 step every anchor_period steps
   int since_anchor = 0;       // 0: the next step is an anchor step
   DBuf<double> YH_a, YD_a, dfi_a, sol_a, vel_a, vort_a;
+  DBuf<float4> dv4, dw4;      // FP32 changes of the fields since the anchor step
+  DenseMx *mx = nullptr;      // made by dense_refine on first use
+  DenseKin() = default;
+  DenseKin(const DenseKin &) = delete;
+  DenseKin &operator=(const DenseKin &) = delete;
+  ~DenseKin() { dense_mx_free(mx); }
 };
+
+struct RefineResult {
+  int iters = 0;
+  double relres = 0.0;   // ||B - A X||_F / ||B||_F after the last iteration
+  double ms = 0.0;       // device time of the refinement
+  double ms_apply = 0.0;  // mean device time of one correction product M R
+};
+
+// Solves A_e X = B (e = 0, 1, 2; B, X device arrays n x nrhs, nrhs <= 4) by
+// FP64 iterative refinement with the inverse applied in prec (F64: rocBLAS
+// FP64; F32, BF16, F16: tile-major copies with per-row power-of-two scales
+// on the matrix cores, FP32 accumulation).
+RefineResult dense_refine(DenseKin &d, const DevMesh &mesh, int e, SdPrec prec, int nrhs, const double *B,
+                          double *X, double tol, int maxit, hipStream_t s);
 
 // Builds A_e, factorises and inverts it (rocSOLVER getrf/getri).
 void dense_setup(DenseKin &d, const DevMesh &m, const int *kode_host, SdPrec prec, hipStream_t s, double *ms_lu);

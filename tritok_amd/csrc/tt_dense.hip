@@ -297,18 +297,31 @@ __global__ void to_low_kernel(int nr, int nc, int ldl, const double *__restrict_
     out[k] = f_to_bf16(v);
 }
 
+// the change of a field since the anchor step in FP32: out[k] = (f0, f1, f2,
+// flag) for node gather[k] (or k), flag = 1 for a domain node (isbnd == 0)
+__global__ void delta4_kernel(int n, int nn, const int *__restrict__ gather, const double *__restrict__ f,
+                              const double *__restrict__ fa, const int *__restrict__ isbnd, float4 *__restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const size_t g = gather ? (size_t)(gather[k] - 1) : (size_t)k;
+  out[k] = make_float4((float)(f[g] - fa[g]), (float)(f[g + nn] - fa[g + nn]),
+                       (float)(f[g + 2 * (size_t)nn] - fa[g + 2 * (size_t)nn]),
+                       isbnd ? (isbnd[g] ? 0.f : 1.f) : 0.f);
+}
+
 constexpr int kLoRows = 4;                    // rows per thread
 constexpr int kLoBlockRows = 64 * kLoRows;    // rows per block
 
-// delta of [H Htx Hty Htz] products: sum_j Hq(i,j) * (vel - vel_a)(gbn(j), c)
+// delta of [H Htx Hty Htz] products: sum_j Hq(i,j) * dv(j, c), dv = the change of
+// the velocity at boundary node gbn(j) (delta4_kernel); four columns per step,
+// their loads issued together
 template <Lo L>
-__global__ void __launch_bounds__(64) lprod_h_kernel(int nb, int nn, int ldl, int ks_cnt,
+__global__ void __launch_bounds__(64) lprod_h_kernel(int nb, int ldl, int ks_cnt,
                                                      const typename Load4<L>::T *__restrict__ H0,
                                                      const typename Load4<L>::T *__restrict__ H1,
                                                      const typename Load4<L>::T *__restrict__ H2,
                                                      const typename Load4<L>::T *__restrict__ H3,
-                                                     const float *__restrict__ rscale, const double *__restrict__ vel,
-                                                     const double *__restrict__ vel_a, const int *__restrict__ gbn,
+                                                     const float *__restrict__ rscale, const float4 *__restrict__ dv,
                                                      double *__restrict__ part) {
   const int i0 = blockIdx.x * kLoBlockRows + threadIdx.x * kLoRows, ks = blockIdx.y;
   const int chunk = (nb + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(nb, j0 + chunk);
@@ -319,46 +332,61 @@ __global__ void __launch_bounds__(64) lprod_h_kernel(int nb, int nn, int ldl, in
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[q][c][r] = 0.f;
-  if (i0 < nb) {
-    for (int j = j0; j < j1; ++j) {
-      const size_t g = gbn[j] - 1;
-      const float d0 = (float)(vel[g] - vel_a[g]), d1 = (float)(vel[g + nn] - vel_a[g + nn]),
-                  d2 = (float)(vel[g + 2 * (size_t)nn] - vel_a[g + 2 * (size_t)nn]);
-      const size_t o = (size_t)ldl * j + i0;
-      const float4 m[4] = {Load4<L>::at(H0 + o), Load4<L>::at(H1 + o), Load4<L>::at(H2 + o), Load4<L>::at(H3 + o)};
+  if (i0 >= nb) return;
+  const typename Load4<L>::T *Hq[4] = {H0, H1, H2, H3};
+  auto step = [&](const float4 d, const float4 (&m)[4]) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float mv[4] = {m[q].x, m[q].y, m[q].z, m[q].w};
+    for (int q = 0; q < 4; ++q) {
+      const float mv[4] = {m[q].x, m[q].y, m[q].z, m[q].w};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          acc[q][0][r] += mv[r] * d0;
-          acc[q][1][r] += mv[r] * d1;
-          acc[q][2][r] += mv[r] * d2;
-        }
+      for (int r = 0; r < 4; ++r) {
+        acc[q][0][r] += mv[r] * d.x;
+        acc[q][1][r] += mv[r] * d.y;
+        acc[q][2][r] += mv[r] * d.z;
       }
     }
+  };
+  int j = j0;
+  for (; j + 4 <= j1; j += 4) {
+    float4 d[4], m[4][4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = i0 + r;
-      if (i >= nb) break;
+    for (int u = 0; u < 4; ++u) {
+      d[u] = dv[j + u];
+      const size_t o = (size_t)ldl * (j + u) + i0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          part[((size_t)ks * 12 + q * 3 + c) * nb + i] = (double)acc[q][c][r] * (double)rscale[(size_t)q * ldl + i];
+      for (int q = 0; q < 4; ++q) m[u][q] = Load4<L>::at(Hq[q] + o);
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) step(d[u], m[u]);
+  }
+  for (; j < j1; ++j) {
+    float4 m[4];
+    const size_t o = (size_t)ldl * j + i0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) m[q] = Load4<L>::at(Hq[q] + o);
+    step(dv[j], m);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + r;
+    if (i >= nb) break;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        part[((size_t)ks * 12 + q * 3 + c) * nb + i] = (double)acc[q][c][r] * (double)rscale[(size_t)q * ldl + i];
   }
 }
 
 // delta of [Dx Dy Dz] products split into boundary (0) / domain (1) columns:
-// slot m*6 + c*2 + dom
+// slot m*6 + c*2 + dom; dw(k) = (change of the vorticity at node k, domain
+// flag) from delta4_kernel; four columns per step, loads issued together
 template <Lo L>
 __global__ void __launch_bounds__(64) lprod_d_kernel(int nb, int nn, int ldl, int ks_cnt,
                                                      const typename Load4<L>::T *__restrict__ D0,
                                                      const typename Load4<L>::T *__restrict__ D1,
                                                      const typename Load4<L>::T *__restrict__ D2,
-                                                     const float *__restrict__ rscale, const double *__restrict__ vort,
-                                                     const double *__restrict__ vort_a, const int *__restrict__ isbnd,
+                                                     const float *__restrict__ rscale, const float4 *__restrict__ dw,
                                                      double *__restrict__ part) {
   const int i0 = blockIdx.x * kLoBlockRows + threadIdx.x * kLoRows, ks = blockIdx.y;
   const int chunk = (nn + ks_cnt - 1) / ks_cnt, k0 = ks * chunk, k1 = min(nn, k0 + chunk);
@@ -369,43 +397,57 @@ __global__ void __launch_bounds__(64) lprod_d_kernel(int nb, int nn, int ldl, in
     for (int o = 0; o < 6; ++o)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[q][o][r] = 0.f;
-  if (i0 < nb) {
-    for (int k = k0; k < k1; ++k) {
-      const float w0 = (float)(vort[k] - vort_a[k]), w1 = (float)(vort[k + nn] - vort_a[k + nn]),
-                  w2 = (float)(vort[k + 2 * (size_t)nn] - vort_a[k + 2 * (size_t)nn]);
-      const size_t o = (size_t)ldl * k + i0;
-      const float4 m[3] = {Load4<L>::at(D0 + o), Load4<L>::at(D1 + o), Load4<L>::at(D2 + o)};
-      const int dom = isbnd[k] ? 0 : 1;
+  if (i0 >= nb) return;
+  const typename Load4<L>::T *Dq[3] = {D0, D1, D2};
+  // boundary columns add into slots 0, 2, 4, domain columns into 1, 3, 5
+#define TT_LPROD_D_ADD(W, M, S0)                                    \
+  _Pragma("unroll") for (int q = 0; q < 3; ++q) {                   \
+    const float mv[4] = {(M)[q].x, (M)[q].y, (M)[q].z, (M)[q].w};   \
+    _Pragma("unroll") for (int r = 0; r < 4; ++r) {                 \
+      acc[q][(S0)][r] += mv[r] * (W).x;                             \
+      acc[q][(S0) + 2][r] += mv[r] * (W).y;                         \
+      acc[q][(S0) + 4][r] += mv[r] * (W).z;                         \
+    }                                                               \
+  }
+  // the branch is uniform across the wavefront (one column per step)
+#define TT_LPROD_D_STEP(W, M)   \
+  if ((W).w == 0.f) {           \
+    TT_LPROD_D_ADD(W, M, 0)     \
+  } else {                      \
+    TT_LPROD_D_ADD(W, M, 1)     \
+  }
+  int k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    float4 w[4], m[4][3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const float mv[4] = {m[q].x, m[q].y, m[q].z, m[q].w};
-        if (dom == 0) {
+    for (int u = 0; u < 4; ++u) {
+      w[u] = dw[k + u];
+      const size_t o = (size_t)ldl * (k + u) + i0;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            acc[q][0][r] += mv[r] * w0;
-            acc[q][2][r] += mv[r] * w1;
-            acc[q][4][r] += mv[r] * w2;
-          }
-        } else {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            acc[q][1][r] += mv[r] * w0;
-            acc[q][3][r] += mv[r] * w1;
-            acc[q][5][r] += mv[r] * w2;
-          }
-        }
-      }
+      for (int q = 0; q < 3; ++q) m[u][q] = Load4<L>::at(Dq[q] + o);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = i0 + r;
-      if (i >= nb) break;
+    for (int u = 0; u < 4; ++u) TT_LPROD_D_STEP(w[u], m[u])
+  }
+  for (; k < k1; ++k) {
+    float4 m[3];
+    const size_t o = (size_t)ldl * k + i0;
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double sc = rscale[(size_t)(4 + q) * ldl + i];
+    for (int q = 0; q < 3; ++q) m[q] = Load4<L>::at(Dq[q] + o);
+    const float4 w = dw[k];
+    TT_LPROD_D_STEP(w, m)
+  }
+#undef TT_LPROD_D_STEP
+#undef TT_LPROD_D_ADD
 #pragma unroll
-        for (int o = 0; o < 6; ++o) part[((size_t)ks * 18 + q * 6 + o) * nb + i] = (double)acc[q][o][r] * sc;
-      }
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + r;
+    if (i >= nb) break;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double sc = rscale[(size_t)(4 + q) * ldl + i];
+#pragma unroll
+      for (int o = 0; o < 6; ++o) part[((size_t)ks * 18 + q * 6 + o) * nb + i] = (double)acc[q][o][r] * sc;
     }
   }
 }
@@ -617,13 +659,17 @@ void step_delta(DenseKin &d, const DevMesh &m, const double *vel, double *vort, 
   const unsigned rb = ceil_div(nb, kLoBlockRows);
   const int ksh = lo_split(nb, nb), ksd = lo_split(nb, nn);
   const float *sc = d.rscale.p;
-  hipLaunchKernelGGL((lprod_h_kernel<L>), dim3(rb, ksh), dim3(64), 0, s, nb, nn, ldl, ksh, lo(0), lo(1), lo(2), lo(3),
-                     sc, vel, d.vel_a.p, m.gbn.p, d.part.p);
+  hipLaunchKernelGGL(delta4_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, m.gbn.p, vel, d.vel_a.p,
+                     (const int *)nullptr, d.dv4.p);
+  hipLaunchKernelGGL(delta4_kernel, dim3(ceil_div(nn, 256)), dim3(256), 0, s, nn, nn, (const int *)nullptr, vort,
+                     d.vort_a.p, d.isbnd.p, d.dw4.p);
+  hipLaunchKernelGGL((lprod_h_kernel<L>), dim3(rb, ksh), dim3(64), 0, s, nb, ldl, ksh, lo(0), lo(1), lo(2), lo(3), sc,
+                     d.dv4.p, d.part.p);
   TT_LAUNCH_CHECK();
   hipLaunchKernelGGL(reduce_base_kernel, dim3(ceil_div((size_t)12 * nb, 256)), dim3(256), 0, s, nb, 12, ksh, d.part.p,
                      d.YH_a.p, d.YH.p);
   hipLaunchKernelGGL((lprod_d_kernel<L>), dim3(rb, ksd), dim3(64), 0, s, nb, nn, ldl, ksd, lo(4), lo(5), lo(6), sc,
-                     vort, d.vort_a.p, d.isbnd.p, d.part.p);
+                     d.dw4.p, d.part.p);
   TT_LAUNCH_CHECK();
   hipLaunchKernelGGL(reduce_base_kernel, dim3(ceil_div((size_t)18 * nb, 256)), dim3(256), 0, s, nb, 18, ksd, d.part.p,
                      d.YD_a.p, d.YD.p);
@@ -709,9 +755,353 @@ void dense_set_precision(DenseKin &d, SdPrec prec, hipStream_t s) {
   d.YH_a.alloc((size_t)12 * nb), d.YD_a.alloc((size_t)18 * nb);
   d.dfi_a.alloc((size_t)3 * nb), d.sol_a.alloc((size_t)3 * nb);
   d.vel_a.alloc((size_t)3 * nn), d.vort_a.alloc((size_t)3 * nn);
+  d.dv4.alloc(nb), d.dw4.alloc(nn);
   const int need = std::max({lo_split(nb, nn) * 18, lo_split(nb, nb) * 12, split_for(nb, nn) * 18,
                              split_for(nb, nb) * 12});
   if ((size_t)need * nb > d.part.n) d.part.alloc((size_t)need * nb);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision solve of one equation's system with FP64 iterative
+// refinement: X <- X + M (B - A X) until ||B - A X||_F <= tol ||B||_F, with A
+// the FP64 system matrix and M its inverse held in FP32, BF16 or FP16 with a
+// power-of-two scale per row, applied to up to four right-hand sides on the
+// matrix cores with FP32 accumulation (the right-hand sides scaled per column
+// by a power of two into the same format).
+
+namespace {
+
+typedef _Float16 mx_h8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mx_b8 __attribute__((ext_vector_type(8)));
+typedef float mx_f4 __attribute__((ext_vector_type(4)));
+
+// k per tile: 32 for 16-bit entries (one v_mfma_f32_16x16x32_{f16,bf16}, eight
+// per lane), 16 for FP32 (four v_mfma_f32_16x16x4_f32 steps, four per lane)
+template <Lo L>
+constexpr int mx_kt() {
+  return L == Lo::F32 ? 16 : 32;
+}
+
+template <Lo L>
+__device__ __forceinline__ typename Load4<L>::T mx_conv(float x) {
+  if constexpr (L == Lo::F32)
+    return x;
+  else if constexpr (L == Lo::F16)
+    return f_to_f16(x);
+  else
+    return f_to_bf16(x);
+}
+
+// Tile-major copy of M / rowscale: tile (ti, tk), lane l, entry j holds row
+// ti*16 + l%16 and column
+//   16-bit: tk*32 + 8*(l/16) + j (j < 8), the A operand of 16x16x32;
+//   FP32:   tk*16 + 4*j + l/16  (j < 4), the A operands of four 16x16x4 steps;
+// zero outside the n x n matrix.  A wavefront reads a tile as 64 consecutive
+// 16-byte pieces.
+template <Lo L>
+__global__ void __launch_bounds__(64) mx_tiles_kernel(int n, int nkt, const double *__restrict__ M,
+                                                      const float *__restrict__ scale,
+                                                      typename Load4<L>::T *__restrict__ out) {
+  constexpr int KT = mx_kt<L>(), PER = KT / 4;
+  const int ti = blockIdx.x, tk = blockIdx.y, l = threadIdx.x;
+  const int row = ti * 16 + (l & 15);
+  const double sc = row < n ? (double)scale[row] : 1.0;
+  typename Load4<L>::T v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int k = L == Lo::F32 ? tk * KT + 4 * j + (l >> 4) : tk * KT + 8 * (l >> 4) + j;
+    const float x = (row < n && k < n) ? (float)(M[row + (size_t)n * k] / sc) : 0.f;
+    v[j] = mx_conv<L>(x);
+  }
+  typename Load4<L>::T *o = out + ((size_t)(ti * nkt + tk) * 64 + l) * PER;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) o[j] = v[j];
+}
+
+// per right-hand side c: cscale[c] = 2^e >= max |R(:, c)| (1 for a zero column)
+__global__ void __launch_bounds__(256) mx_colscale_kernel(int n, const double *__restrict__ R,
+                                                          double *__restrict__ cscale) {
+  __shared__ double sh[4];
+  const int c = blockIdx.x;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(R[i + (size_t)n * c]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    int e = 0;
+    if (m > 0.0) frexp(m, &e);
+    cscale[c] = m > 0.0 ? ldexp(1.0, e) : 1.0;
+  }
+}
+
+// R / cscale in the B-operand layout: 16-bit column-major with kp = nkt*32
+// rows (lane reads eight consecutive k); FP32 as [c][tk][l/16][j] with k =
+// tk*16 + 4j + l/16 (lane reads four)
+template <Lo L>
+__global__ void mx_rhs_kernel(int n, int nkt, int nrhs, const double *__restrict__ R,
+                              const double *__restrict__ cscale, typename Load4<L>::T *__restrict__ out) {
+  constexpr int KT = mx_kt<L>();
+  const size_t kp = (size_t)nkt * KT;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= kp * nrhs) return;
+  const int c = (int)(t / kp), p = (int)(t % kp);
+  int k;
+  if constexpr (L == Lo::F32) {
+    const int tk = p / 16, r = p % 16, g = r / 4, j = r % 4;
+    k = tk * 16 + 4 * j + g;
+  } else {
+    k = p;
+  }
+  const float x = k < n ? (float)(R[k + (size_t)n * c] / cscale[c]) : 0.f;
+  out[t] = mx_conv<L>(x);
+}
+
+// part[ks][c][row] = sum over this slice's k tiles of (M/s)(row, k) (R/t)(k, c).
+// Block: four wavefronts, one 16-row strip each; blockIdx.y: the k slice.
+// Output register i of lane l: row 4*(l/16) + i of the strip, column l%16
+// (the 16x16 FP32-accumulator layout); columns >= nrhs carry zeros.
+template <Lo L>
+__global__ void __launch_bounds__(256) mx_apply_kernel(int n, int nkt, int nrhs, int kchunk, int npad,
+                                                       const typename Load4<L>::T *__restrict__ tiles,
+                                                       const typename Load4<L>::T *__restrict__ rlow,
+                                                       float *__restrict__ part) {
+  constexpr int KT = mx_kt<L>(), PER = KT / 4;
+  const int l = threadIdx.x & 63, ti = blockIdx.x * 4 + (threadIdx.x >> 6), ks = blockIdx.y;
+  if (ti * 16 >= n) return;
+  const int tk0 = ks * kchunk, tk1 = min(nkt, tk0 + kchunk);
+  const int col = l & 15, g = l >> 4;
+  const bool hasc = col < nrhs;
+  mx_f4 acc = {0.f, 0.f, 0.f, 0.f};
+  const size_t kp = (size_t)nkt * KT;
+  auto load = [&](int tk, uint4 &a, uint4 &b) {
+    a = *reinterpret_cast<const uint4 *>(tiles + ((size_t)(ti * nkt + tk) * 64 + l) * PER);
+    b = uint4{0u, 0u, 0u, 0u};
+    if (hasc) {
+      const size_t off = L == Lo::F32 ? ((size_t)col * nkt + tk) * 16 + g * 4 : (size_t)col * kp + tk * KT + 8 * g;
+      b = *reinterpret_cast<const uint4 *>(rlow + off);
+    }
+  };
+  auto mma = [&](const uint4 &a, const uint4 &b) {
+    if constexpr (L == Lo::F16) {
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(mx_h8, a), __builtin_bit_cast(mx_h8, b), acc,
+                                                   0, 0, 0);
+    } else if constexpr (L == Lo::BF16) {
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mx_b8, a), __builtin_bit_cast(mx_b8, b), acc,
+                                                    0, 0, 0);
+    } else {
+      const mx_f4 af = __builtin_bit_cast(mx_f4, a), bf = __builtin_bit_cast(mx_f4, b);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[j], bf[j], acc, 0, 0, 0);
+    }
+  };
+  // four tiles per step, their loads issued together
+  int tk = tk0;
+  for (; tk + 4 <= tk1; tk += 4) {
+    uint4 a[4], b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) load(tk + q, a[q], b[q]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) mma(a[q], b[q]);
+  }
+  for (; tk < tk1; ++tk) {
+    uint4 a, b;
+    load(tk, a, b);
+    mma(a, b);
+  }
+  if (!hasc) return;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int row = ti * 16 + 4 * g + i;
+    part[((size_t)ks * nrhs + col) * npad + row] = acc[i];
+  }
+}
+
+// X(i, c) += rowscale(i) * cscale(c) * sum over slices (in slice order)
+__global__ void mx_update_kernel(int n, int nrhs, int ks_cnt, int npad, const float *__restrict__ part,
+                                 const float *__restrict__ rscale, const double *__restrict__ cscale,
+                                 double *__restrict__ X) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * nrhs) return;
+  const int i = (int)(t % n), c = (int)(t / n);
+  double s = 0.0;
+  for (int k = 0; k < ks_cnt; ++k) s += (double)part[((size_t)k * nrhs + c) * npad + i];
+  X[t] += s * (double)rscale[i] * cscale[c];
+}
+
+// part[ks][c][i] = sum over column slice ks of M(i, j) X(j, c), c < nrhs <= 4
+// (FP64, column-major M read once for all right-hand sides)
+__global__ void __launch_bounds__(kRowsPerBlock) gemm4_kernel(int n, int ks_cnt, int nrhs, const double *__restrict__ M,
+                                                              const double *__restrict__ X, double *__restrict__ part) {
+  const int i = blockIdx.x * kRowsPerBlock + threadIdx.x, ks = blockIdx.y;
+  const int chunk = (n + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(n, j0 + chunk);
+  if (i >= n) return;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = j0; j < j1; ++j) {
+    const double m = M[i + (size_t)n * j];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c < nrhs) acc[c] += m * X[j + (size_t)n * c];
+  }
+  for (int c = 0; c < nrhs; ++c) part[((size_t)ks * nrhs + c) * n + i] = acc[c];
+}
+
+// out = base + sign * (sum of the slices), element-wise (out may alias base)
+__global__ void combine4_kernel(int n, int nrhs, int ks_cnt, const double *__restrict__ part, const double *base,
+                                double sign, double *out) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * nrhs) return;
+  const int i = (int)(t % n), c = (int)(t / n);
+  double s = 0.0;
+  for (int k = 0; k < ks_cnt; ++k) s += part[((size_t)k * nrhs + c) * n + i];
+  out[t] = base[t] + sign * s;
+}
+
+struct MxBuf {
+  Lo lo = Lo::F32;
+  int nkt = 0;
+  DBuf<uint16_t> t16;
+  DBuf<float> t32, rscale;
+};
+
+}  // namespace
+
+struct DenseMx {
+  MxBuf m[3][3];  // [equation][FP32, BF16, FP16]
+};
+
+void dense_mx_free(DenseMx *p) { delete p; }
+
+RefineResult dense_refine(DenseKin &d, const DevMesh &mesh, int e, SdPrec prec, int nrhs, const double *B,
+                          double *X, double tol, int maxit, hipStream_t s) {
+  const int n = d.nb;
+  if (e < 0 || e > 2) throw std::runtime_error("equation must be 1..3");
+  if (nrhs < 1 || nrhs > 4) throw std::runtime_error("nrhs must be 1..4");
+  if (!d.mx) d.mx = new DenseMx();
+  rocblas_handle h;
+  if (rocblas_create_handle(&h) != rocblas_status_success) throw std::runtime_error("rocblas_create_handle failed");
+  rocblas_set_stream(h, s);
+  rocblas_set_pointer_mode(h, rocblas_pointer_mode_host);
+  // the FP64 system matrix A_e (rebuilt; the resident copy holds its inverse)
+  DBuf<double> A((size_t)n * n);
+  hipLaunchKernelGGL(build_sys_kernel, dim3(ceil_div((size_t)n * n, 256)), dim3(256), 0, s, n, e, d.kode.p,
+                     mesh.nx.p, mesh.ny.p, mesh.nz.p, mesh.gbn.p, d.mats.Dx.p, d.mats.Dy.p, d.mats.Dz.p, A.p);
+  TT_LAUNCH_CHECK();
+  const Lo lo = prec == SdPrec::F32 ? Lo::F32 : prec == SdPrec::BF16 ? Lo::BF16 : Lo::F16;
+  const int KT = lo == Lo::F32 ? 16 : 32, nkt = (int)ceil_div(n, KT), nst = (int)ceil_div(n, 16), npad = nst * 16;
+  MxBuf *mb = nullptr;
+  if (prec != SdPrec::F64) {
+    mb = &d.mx->m[e][(int)prec - 1];
+    if (mb->nkt == 0) {  // tile-major copy of the inverse, made on first use
+      const int ks = split_for(n, n);
+      DBuf<float> pm((size_t)ks * n);
+      mb->rscale.alloc(n);
+      hipLaunchKernelGGL(rowmax_kernel, dim3(ceil_div(n, 256), ks), dim3(256), 0, s, n, n, ks, d.Ainv[e].p, pm.p);
+      hipLaunchKernelGGL(rowscale_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, ks, pm.p, mb->rscale.p);
+      const size_t tot = (size_t)nst * nkt * 64 * (KT / 4);
+      if (lo == Lo::F32) {
+        mb->t32.alloc(tot);
+        hipLaunchKernelGGL(mx_tiles_kernel<Lo::F32>, dim3(nst, nkt), dim3(64), 0, s, n, nkt, d.Ainv[e].p,
+                           mb->rscale.p, mb->t32.p);
+      } else {
+        mb->t16.alloc(tot);
+        if (lo == Lo::F16)
+          hipLaunchKernelGGL(mx_tiles_kernel<Lo::F16>, dim3(nst, nkt), dim3(64), 0, s, n, nkt, d.Ainv[e].p,
+                             mb->rscale.p, mb->t16.p);
+        else
+          hipLaunchKernelGGL(mx_tiles_kernel<Lo::BF16>, dim3(nst, nkt), dim3(64), 0, s, n, nkt, d.Ainv[e].p,
+                             mb->rscale.p, mb->t16.p);
+      }
+      TT_LAUNCH_CHECK();
+      TT_HIP(hipStreamSynchronize(s));  // pm is released on return
+      mb->lo = lo, mb->nkt = nkt;
+    }
+  }
+  // slices of k so that the launch has about four wavefronts per SIMD
+  const int ksl = std::max(1, std::min(nkt, (int)ceil_div(4096, nst)));
+  const int kchunk = (int)ceil_div(nkt, ksl), ks_cnt = (int)ceil_div(nkt, kchunk);
+  DBuf<double> R((size_t)n * nrhs), cs(nrhs);
+  // FP64 products (residual; the FP64 correction) in column slices: about
+  // four wavefronts per SIMD
+  const int ks64 = std::max(1, std::min(n / 64, (int)ceil_div(4096, ceil_div(n, kRowsPerBlock))));
+  DBuf<double> p64((size_t)ks64 * nrhs * n);
+  DBuf<float> part((size_t)ks_cnt * nrhs * npad);
+  DBuf<uint16_t> r16;
+  DBuf<float> r32;
+  if (lo == Lo::F32)
+    r32.alloc((size_t)nkt * KT * nrhs);
+  else
+    r16.alloc((size_t)nkt * KT * nrhs);
+  TT_HIP(hipMemsetAsync(X, 0, sizeof(double) * n * nrhs, s));
+  TT_HIP(hipMemcpyAsync(R.p, B, sizeof(double) * n * nrhs, hipMemcpyDeviceToDevice, s));
+  double nb2 = 0.0;
+  if (rocblas_dnrm2(h, n * nrhs, B, 1, &nb2) != rocblas_status_success) throw std::runtime_error("rocblas_dnrm2");
+  RefineResult res;
+  hipEvent_t ev[4];
+  for (auto &x : ev) TT_HIP(hipEventCreate(&x));
+  TT_HIP(hipEventRecord(ev[0], s));
+  double apply_ms = 0.0;
+  int applies = 0;
+  for (int it = 0; it < maxit; ++it) {
+    // correction X += M R
+    TT_HIP(hipEventRecord(ev[2], s));
+    if (prec == SdPrec::F64) {
+      hipLaunchKernelGGL(gemm4_kernel, dim3(ceil_div(n, kRowsPerBlock), ks64), dim3(kRowsPerBlock), 0, s, n, ks64,
+                         nrhs, d.Ainv[e].p, R.p, p64.p);
+      hipLaunchKernelGGL(combine4_kernel, dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, ks64,
+                         p64.p, X, 1.0, X);
+      TT_LAUNCH_CHECK();
+    } else {
+      hipLaunchKernelGGL(mx_colscale_kernel, dim3(nrhs), dim3(256), 0, s, n, R.p, cs.p);
+      const size_t nr = (size_t)nkt * KT * nrhs;
+      const dim3 grid(ceil_div(nst, 4), ks_cnt);
+      if (lo == Lo::F32) {
+        hipLaunchKernelGGL(mx_rhs_kernel<Lo::F32>, dim3(ceil_div(nr, 256)), dim3(256), 0, s, n, nkt, nrhs, R.p, cs.p,
+                           r32.p);
+        hipLaunchKernelGGL(mx_apply_kernel<Lo::F32>, grid, dim3(256), 0, s, n, nkt, nrhs, kchunk, npad, mb->t32.p,
+                           r32.p, part.p);
+      } else if (lo == Lo::F16) {
+        hipLaunchKernelGGL(mx_rhs_kernel<Lo::F16>, dim3(ceil_div(nr, 256)), dim3(256), 0, s, n, nkt, nrhs, R.p, cs.p,
+                           r16.p);
+        hipLaunchKernelGGL(mx_apply_kernel<Lo::F16>, grid, dim3(256), 0, s, n, nkt, nrhs, kchunk, npad, mb->t16.p,
+                           r16.p, part.p);
+      } else {
+        hipLaunchKernelGGL(mx_rhs_kernel<Lo::BF16>, dim3(ceil_div(nr, 256)), dim3(256), 0, s, n, nkt, nrhs, R.p, cs.p,
+                           r16.p);
+        hipLaunchKernelGGL(mx_apply_kernel<Lo::BF16>, grid, dim3(256), 0, s, n, nkt, nrhs, kchunk, npad, mb->t16.p,
+                           r16.p, part.p);
+      }
+      hipLaunchKernelGGL(mx_update_kernel, dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, ks_cnt,
+                         npad, part.p, mb->rscale.p, cs.p, X);
+      TT_LAUNCH_CHECK();
+    }
+    TT_HIP(hipEventRecord(ev[3], s));
+    // FP64 residual R = B - A X and its norm
+    hipLaunchKernelGGL(gemm4_kernel, dim3(ceil_div(n, kRowsPerBlock), ks64), dim3(kRowsPerBlock), 0, s, n, ks64, nrhs,
+                       A.p, X, p64.p);
+    hipLaunchKernelGGL(combine4_kernel, dim3(ceil_div((size_t)n * nrhs, 256)), dim3(256), 0, s, n, nrhs, ks64, p64.p,
+                       B, -1.0, R.p);
+    TT_LAUNCH_CHECK();
+    double nr2 = 0.0;
+    if (rocblas_dnrm2(h, n * nrhs, R.p, 1, &nr2) != rocblas_status_success) throw std::runtime_error("rocblas_dnrm2");
+    float ms = 0.f;
+    TT_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+    apply_ms += ms, ++applies;
+    res.iters = it + 1;
+    res.relres = nb2 > 0.0 ? nr2 / nb2 : 0.0;
+    if (!(res.relres > tol)) break;  // converged (or not finite: reported as is)
+  }
+  TT_HIP(hipEventRecord(ev[1], s));
+  TT_HIP(hipEventSynchronize(ev[1]));
+  float tot = 0.f;
+  TT_HIP(hipEventElapsedTime(&tot, ev[0], ev[1]));
+  res.ms = tot;
+  res.ms_apply = applies ? apply_ms / applies : 0.0;
+  for (auto &x : ev) (void)hipEventDestroy(x);
+  rocblas_destroy_handle(h);
+  return res;
 }
 
 }  // namespace tt

@@ -156,4 +156,29 @@ extern "C" int tt_sd_solve(tt_sd *h, const double *vel, double *vort, const doub
   });
 }
 
+extern "C" int tt_sd_refine(tt_sd *h, int32_t eqn, int32_t prec, int32_t nrhs, const double *B, double *X,
+                            double tol, int32_t maxit, int32_t *iters, double *relres, double ms[2]) {
+  return guarded("tt_sd_refine", [&] {
+    if (!h || !B || !X) throw std::runtime_error("NULL argument");
+    if (eqn < 1 || eqn > 3) throw std::runtime_error("eqn must be 1..3");
+    if (prec < TT_PREC_FP64 || prec > TT_PREC_FP16) throw std::runtime_error("unknown precision");
+    if (nrhs < 1 || nrhs > 4) throw std::runtime_error("nrhs must be 1..4");
+    if (!(tol >= 0.0) || maxit < 1) throw std::runtime_error("tol must be >= 0 and maxit >= 1");
+    const size_t len = (size_t)h->dk.nb * nrhs;
+    DBuf<double> b, x(len);
+    b.upload(B, len, h->s);
+    const SdPrec sp = prec == TT_PREC_FP64 ? SdPrec::F64
+                      : prec == TT_PREC_FP32 ? SdPrec::F32
+                      : prec == TT_PREC_BF16 ? SdPrec::BF16
+                                             : SdPrec::F16;
+    const RefineResult r = dense_refine(h->dk, h->mesh, eqn - 1, sp, nrhs, b.p, x.p, tol, maxit, h->s);
+    x.download(X, len, h->s);
+    TT_HIP(hipStreamSynchronize(h->s));
+    if (iters) *iters = r.iters;
+    if (relres) *relres = r.relres;
+    if (ms) ms[0] = r.ms, ms[1] = r.ms_apply;
+    return 0;
+  });
+}
+
 extern "C" void tt_sd_free(tt_sd *h) { delete h; }

@@ -558,7 +558,8 @@ __global__ void __launch_bounds__(256) sd_dom_mfma_kernel(MeshPtrs m, int row0, 
     // 16 points (four k-steps) per pass: all loads of the pass first, then the
     // kernel values, then the 24 products
     const int npass = NP > 0 ? NP : Pp / 16;
-#pragma unroll(NP > 0 ? NP : 1)
+    constexpr int kUnroll = NP > 0 ? NP : 1;
+#pragma unroll kUnroll
     for (int ps = 0; ps < npass; ++ps) {
       const int k0 = 16 * ps;
       double gx[4], gy[4], gz[4], gw[4], a0[4], a1[4];

@@ -52,6 +52,7 @@ struct BlockSys {
   DBuf<int> gtab;         // n*gw: the entries of tidx per unknown in cell order, -1 padded
   DBuf<double> q;         // column scaling
   bool has32 = false, has16 = false;
+  int grid_a[3] = {0, 0, 0};  // cell-kernel blocks per block precision (BlockPrec), 0 = resident blocks
   int last_nit = 0;       // iterations of the previous solve (sizes the first launch batch)
   float last_eps = 0.f;   // its tolerance
 };
@@ -97,6 +98,10 @@ struct LsqrResult {
 // the initial residual b - A*x0 always uses the double-precision blocks.
 LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
                       hipStream_t st, LsqrProbe *probe = nullptr);
+
+// Picks the cell kernel's grid for block precision prec by timing candidate
+// grids on a scratch solve (once per system shape and precision in a process).
+void block_tune(BlockSys &s, int prec, hipStream_t st);
 
 // single- and half-precision copies of the scaled blocks (built on demand)
 void block_make32(BlockSys &s, hipStream_t st);

@@ -26,6 +26,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include <hip/hip_ext.h>
 
@@ -162,6 +165,9 @@ __global__ void to16_kernel(size_t total16, int L, int LP, int S, int S16, const
 }
 
 constexpr int kNB = 256;  // threads per block of the unknown kernel
+// cells per partial sum of the cell kernel: at most 4096 partials per launch,
+// a fixed function of the system so that the sums do not depend on the grid
+inline int cell_group(int ncell) { return std::max(1, (ncell + 4095) / 4096); }
 
 // ---------------------------------------------------------------------------
 // r0 = b - A (x0/q), P = A^T r0; per-cell partials of ||r0||^2 and ||b||^2
@@ -214,7 +220,8 @@ __global__ void __launch_bounds__(64) lsqr_r0(const double *__restrict__ A, cons
 // iteration stopped, the cell blocks' work of this launch is simply unused
 // (the unknown kernel that follows sees the stop and leaves x unchanged).
 template <typename T, int R, int L>
-__global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restrict__ A, const int *__restrict__ colmap,
+__global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, int cgroup, const T *__restrict__ A,
+                                             const int *__restrict__ colmap,
                                              const double *__restrict__ vp, double *__restrict__ u,
                                              double *__restrict__ P, double *__restrict__ pa1,
                                              const double *__restrict__ pb1, const double *__restrict__ pb2, int nb,
@@ -236,10 +243,15 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
     return;
   }
   if (sin->done) return;
-  // this block's cells: a contiguous range, consecutive ranges on one XCD
+  // this block's cells: a contiguous range of whole cell groups (consecutive
+  // ranges on one XCD); one partial sum of ||u'||^2 per group, so the sums
+  // do not depend on the grid size
   const int G = gridDim.x - 1, lb = xcd_logical(blockIdx.x, 1, G + 1);
-  const int cend = (int)((long long)(lb + 1) * ncell / G);
-  int c = (int)((long long)lb * ncell / G);
+  const int ngroup = (ncell + cgroup - 1) / cgroup;
+  const int g0 = (int)((long long)lb * ngroup / G), g1 = (int)((long long)(lb + 1) * ngroup / G);
+  const int cend = min(ncell, g1 * cgroup);
+  int c = g0 * cgroup;
+  if (c >= cend) return;
   int j = t < L ? colmap[(size_t)c * L + t] : -1;
   double uo = t < R ? u[(size_t)c * R + t] : 0.0;
   RegBlock<T, S> rb;
@@ -360,13 +372,16 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, const T *__restri
       if (t < L) P[(size_t)c * L + t] = s;
     }
     acc += un * un;
+    if (cn % cgroup == 0 || cn >= cend) {  // the group ends with cell c
+      const double s1 = wave_sum(acc);
+      if (t == 0) pa1[c / cgroup] = s1;
+      acc = 0.0;
+    }
     if (cn >= cend) break;
     vj = jn >= 0 ? vp[jn] : 0.0;
     j = jn, uo = uon, c = cn;
     __syncthreads();  // blk, vs and uu are rewritten next
   }
-  const double s1 = wave_sum(acc);
-  if (t == 0) pa1[blockIdx.x - 1] = s1;
 }
 
 // ---------------------------------------------------------------------------
@@ -480,15 +495,19 @@ int persistent_grid(int ncell) {
 // e0/e1 non-null: the dispatch itself records its start and end (sampled timing)
 template <typename T, int R, int L>
 void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-  const int g = persistent_grid<T, R, L>(s.ncell);
-  wk.na = g;
+  const int ngroup = ceil_div(s.ncell, cell_group(s.ncell));
+  const int tuned = s.grid_a[sizeof(T) == 8 ? 0 : sizeof(T) == 4 ? 1 : 2];
+  const int g = std::min(ngroup, tuned > 0 ? tuned : persistent_grid<T, R, L>(s.ncell));
+  wk.na = ngroup;
   if (e0)
-    hipExtLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, e0, e1, 0, k, s.ncell, A, s.colmap.p,
+    hipExtLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, e0, e1, 0, k, s.ncell,
+                          cell_group(s.ncell), A, s.colmap.p,
                           (const double *)wk.v.p, wk.u.p, wk.P.p, wk.pa1.p, (const double *)wk.pb1.p,
                           (const double *)wk.pb2.p, (int)ceil_div(s.n, kNB),
                           (const LsqrState *)(wk.state.p + ((k - 1) & 1)), wk.state.p + (k & 1));
   else
-    hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, k, s.ncell, A, s.colmap.p, wk.v.p, wk.u.p,
+    hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, k, s.ncell, cell_group(s.ncell), A,
+                       s.colmap.p, wk.v.p, wk.u.p,
                        wk.P.p, wk.pa1.p, wk.pb1.p, wk.pb2.p, (int)ceil_div(s.n, kNB), wk.state.p + ((k - 1) & 1),
                        wk.state.p + (k & 1));
 }
@@ -649,12 +668,108 @@ LsqrWork::~LsqrWork() {
   if (host_state) (void)hipHostFree(host_state);
 }
 
+namespace {
+
+__global__ void fill_kernel(int n, double v, double *__restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// cell-kernel grid chosen per (rows per cell, cells, block precision), kept
+// for the process: systems of the same shape tune once
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int>, int> g_tuned;
+
+}  // namespace
+
+// Autotuning of the cell kernel's grid: a scratch LSQR on this system (b = 1,
+// zero tolerance, its own work buffers) times a few iterations at each
+// candidate grid -- the blocks resident on the device and its halvings, down
+// to 1/8 -- twice each, and keeps the fastest.  The cell partial sums are per
+// fixed group of cells, so the grid does not change any result.
+void block_tune(BlockSys &s, int prec, hipStream_t st) {
+  if (s.grid_a[prec] > 0) return;
+  const auto key = std::make_tuple(s.R, s.ncell, prec);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      s.grid_a[prec] = it->second;
+      return;
+    }
+  }
+  const int m = s.ncell * s.R, n = s.n, ngroup = ceil_div(s.ncell, cell_group(s.ncell));
+  int resident = 0;
+  if (s.R == 51)
+    resident = prec == kBlocks16 ? persistent_grid<_Float16, 51, 51>(s.ncell)
+               : prec == kBlocks32 ? persistent_grid<float, 51, 51>(s.ncell)
+                                   : persistent_grid<double, 51, 51>(s.ncell);
+  else
+    resident = prec == kBlocks16 ? persistent_grid<_Float16, 27, 27>(s.ncell)
+               : prec == kBlocks32 ? persistent_grid<float, 27, 27>(s.ncell)
+                                   : persistent_grid<double, 27, 27>(s.ncell);
+  std::vector<int> cand;
+  for (int d = 1; d <= 8; d *= 2) {
+    const int g = std::min(ngroup, std::max(1, resident / d));
+    if (std::find(cand.begin(), cand.end(), g) == cand.end()) cand.push_back(g);
+  }
+  LsqrWork tw;
+  tw.ensure(m, n, s.ncell, s.L);
+  DBuf<double> b(m), x(n);
+  hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, 1.0, b.p);
+  hipLaunchKernelGGL(fill_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, 0.0, x.p);
+  LsqrState init;
+  std::memset(&init, 0, sizeof(init));
+  init.cs2 = -1.0;
+  init.ctol = 0.0;
+  init.itnlim = 1 << 30;
+  LsqrState init2[2] = {init, init};
+  TT_HIP(hipMemcpyAsync(tw.state.p, init2, sizeof(init2), hipMemcpyHostToDevice, st));
+  TT_HIP(hipMemsetAsync(tw.x.p, 0, sizeof(double) * n, st));
+  r0(s, tw, b.p, x.p, st);
+  stepB(s, tw, 0, st);
+  constexpr int kIts = 6, kReps = 2;
+  std::vector<hipEvent_t> ev(2 * cand.size() * kReps);
+  for (auto &e : ev) TT_HIP(hipEventCreate(&e));
+  int k = 1;
+  for (int rep = 0; rep < kReps; ++rep)
+    for (size_t ci = 0; ci < cand.size(); ++ci) {
+      s.grid_a[prec] = cand[ci];
+      TT_HIP(hipEventRecord(ev[2 * (rep * cand.size() + ci)], st));
+      for (int i = 0; i < kIts; ++i, ++k) {
+        stepA(s, tw, k, prec, st, nullptr, nullptr);
+        stepB(s, tw, k, st);
+      }
+      TT_HIP(hipEventRecord(ev[2 * (rep * cand.size() + ci) + 1], st));
+    }
+  TT_LAUNCH_CHECK();
+  TT_HIP(hipMemcpyAsync(tw.host_state, tw.state.p + ((k - 1) & 1), sizeof(LsqrState), hipMemcpyDeviceToHost, st));
+  TT_HIP(hipStreamSynchronize(st));
+  int best = 0;
+  float best_ms = 1e30f;
+  for (size_t ci = 0; ci < cand.size(); ++ci) {
+    float t = 1e30f;
+    for (int rep = 0; rep < kReps; ++rep) {
+      float ms = 0.f;
+      TT_HIP(hipEventElapsedTime(&ms, ev[2 * (rep * cand.size() + ci)], ev[2 * (rep * cand.size() + ci) + 1]));
+      t = std::min(t, ms);
+    }
+    if (t < best_ms) best_ms = t, best = cand[ci];
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  // a scratch solve that stopped early timed empty launches: keep the default
+  s.grid_a[prec] = tw.host_state->done ? resident : best;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[key] = s.grid_a[prec];
+}
+
 LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
                       hipStream_t st, LsqrProbe *probe) {
   const int m = s.ncell * s.R, n = s.n;
-  wk.ensure(m, n, s.ncell, s.L);
   if (prec == kBlocks32 && !s.has32) throw std::runtime_error("single-precision blocks not built");
   if (prec == kBlocks16 && !s.has16) throw std::runtime_error("half-precision blocks not built");
+  block_tune(s, prec, st);
+  wk.ensure(m, n, s.ncell, s.L);
   LsqrState init;
   std::memset(&init, 0, sizeof(init));
   init.cs2 = -1.0;

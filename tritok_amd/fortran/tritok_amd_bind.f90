@@ -145,6 +145,17 @@ MODULE tritok_amd_bind
       REAL(C_DOUBLE) :: vort(*)
       INTEGER(C_INT) :: tt_sd_solve
     END FUNCTION
+    ! mixed-precision solve of one equation: FP64 iterative refinement
+    FUNCTION tt_sd_refine(h, eqn, prec, nrhs, B, X, tol, maxit, iters, relres, ms) BIND(C, NAME="tt_sd_refine")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: eqn, prec, nrhs, maxit
+      REAL(C_DOUBLE), INTENT(IN) :: B(*)
+      REAL(C_DOUBLE) :: X(*), relres, ms(2)
+      REAL(C_DOUBLE), VALUE :: tol
+      INTEGER(C_INT32_T) :: iters
+      INTEGER(C_INT) :: tt_sd_refine
+    END FUNCTION
     SUBROUTINE tt_sd_free(h) BIND(C, NAME="tt_sd_free")
       IMPORT :: C_PTR
       TYPE(C_PTR), VALUE :: h
@@ -183,6 +194,12 @@ MODULE tritok_amd_bind
       TYPE(C_PTR), VALUE :: h
       INTEGER(C_INT32_T), VALUE :: prec
       INTEGER(C_INT) :: tt_case_set_precision
+    END FUNCTION
+    FUNCTION tt_case_set_grid(h, grid) BIND(C, NAME="tt_case_set_grid")
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: grid
+      INTEGER(C_INT) :: tt_case_set_grid
     END FUNCTION
     FUNCTION tt_case_set_anchor_period(h, period) BIND(C, NAME="tt_case_set_anchor_period")
       IMPORT :: C_INT, C_INT32_T, C_PTR

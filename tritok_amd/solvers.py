@@ -18,6 +18,8 @@ _lu = _lib._sig("tt_dense_lu", i32, i32, PD, PD)
 _lu_solve = _lib._sig("tt_dense_lu_solve", i32, i32, PD, PD, PD)
 _sd_create = _lib._sig("tt_sd_create", i32, C.POINTER(_lib.MeshView), C.POINTER(_lib.Quad), PI, C.POINTER(P))
 _sd_solve = _lib._sig("tt_sd_solve", i32, P, PD, PD, PD)
+_sd_refine = _lib._sig("tt_sd_refine", i32, P, i32, i32, i32, PD, PD, C.c_double, i32, PI, PD, PD)
+SD_PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
 _sd_free = _lib._sig("tt_sd_free", None, P)
 
 
@@ -167,6 +169,19 @@ class SdKinematics:
         ur = _f64(urbw)
         check(_sd_solve(self._h, vel.ctypes.data_as(PD), w.ctypes.data_as(PD), ur.ctypes.data_as(PD)), "tt_sd_solve")
         return w
+
+    def refine(self, eqn, prec, B, tol=1e-10, maxit=50):
+        """Solves A_eqn X = B (nb x nrhs, nrhs <= 4) by FP64 iterative
+        refinement with the inverse applied in prec.  Returns (X, corrections,
+        relative residual, device ms of the solve, mean ms per correction)."""
+        B = np.asfortranarray(B, dtype=np.float64)
+        if B.ndim == 1:
+            B = B[:, None]
+        X = np.zeros_like(B, order="F")
+        it, rr, ms = C.c_int32(), C.c_double(), np.zeros(2)
+        check(_sd_refine(self._h, int(eqn), SD_PREC[prec], B.shape[1], B.ctypes.data_as(PD), X.ctypes.data_as(PD),
+                         float(tol), int(maxit), C.byref(it), C.byref(rr), ms.ctypes.data_as(PD)), "tt_sd_refine")
+        return X, it.value, rr.value, ms[0], ms[1]
 
     def close(self):
         if getattr(self, "_h", None):
