@@ -350,7 +350,7 @@ def main():
                     help="EQNS iBw = 0: skip the single-domain kinematics (least-squares systems only)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving stream replayed in real time (0: skip)")
-    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--serve-max-iter", type=int, default=400)
     ap.add_argument("--batch-cells", type=int, default=16,
                     help="n: batched modified-Helmholtz integrals of n^3 independent cells (0: skip)")
