@@ -339,7 +339,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=20)
-    ap.add_argument("--probe-stride", type=int, default=8)
+    ap.add_argument("--probe-stride", type=int, default=32)
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay the least-squares iterations from captured HIP graph chunks instead of launching "
+                         "every kernel on the stream")
     ap.add_argument("--mixed", default="fp16", choices=["fp32", "fp16", "bf16"],
                     help="reduced precision of the mixed leg")
     ap.add_argument("--no-cpu", action="store_true")
@@ -390,6 +393,7 @@ def main():
     t0 = time.perf_counter()
     case = Case(host, prec=args.mixed)
     setup_s = time.perf_counter() - t0
+    case.set_graphs(args.graphs)
     tm = case.timings()
     log("setup %.2fs (macro %.0f ms, sdkm %.0f ms, LU %.0f ms, systems %.0f ms), device bytes %.2f GB" % (
         setup_s, tm["setup_macro"], tm["setup_sdkm"], tm["setup_lu"], tm["setup_systems"],
@@ -460,7 +464,9 @@ def main():
                    "subdomains": n ** 3,
                    "precision": "%s least-squares blocks (FP32 accumulation) with FP64 residual anchors, "
                                 "scalars and vectors; kinematics products in %s with per-row 2^k scales, FP64 anchor "
-                                "step every 8 iterations" % ("FP16" if half else "FP32", args.mixed.upper())},
+                                "step every 8 iterations" % ("FP16" if half else "FP32", args.mixed.upper()),
+                   "lsqr_launch": "chunks of 2..64 iterations replayed from captured HIP graphs" if args.graphs
+                                  else "kernel by kernel on the stream"},
         "fp64_iters_per_s": fp64_rate, "mixed_over_fp64": value / fp64_rate,
         "mixed_vs_fp64_rel_l2_converged": acc["max_rel_l2"] if acc else None, "accuracy": acc,
         "mixed_vs_fp64_rel_l2_after_steps": max(diff.values()),

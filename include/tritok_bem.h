@@ -335,6 +335,12 @@ int tt_case_set_anchor_period(tt_case *c, int32_t period);
  * precision, 0 returns to the autotuner's choice.  Results do not depend on
  * it (the partial sums are per fixed group of cells). */
 int tt_case_set_grid(tt_case *c, int32_t grid);
+/* How the least-squares iterations are issued (no reference counterpart; a
+ * tuning and test hook): 0 (default) launches every kernel on the case's
+ * stream, 1 replays chunks of 2..64 iterations from HIP graphs captured once
+ * per system.  The kernels and their arguments are the same, so are the
+ * results. */
+int tt_case_set_graphs(tt_case *c, int32_t on);
 /* Measurement hooks (no reference counterpart): time every stride-th launch
  * of the least-squares block kernel with HIP events on the case's stream
  * (stride 0 = off; resets the statistics), and read the average duration,

@@ -103,6 +103,31 @@ def test_results_do_not_depend_on_cell_kernel_grid(prec, gauss_table):
             assert np.array_equal(o[3][k], outs[0][3][k]), (o[0], k)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_graph_replay_matches_stream_launches(prec):
+    """The least-squares iterations give the same bits replayed from captured
+    graph chunks, launched kernel by kernel, and replayed with sampled
+    launches (timing probe, stride 3) splitting the chunks."""
+    from tritok_amd import workloads
+    from tritok_amd.case import Case
+    h = workloads.natural_convection(6, ra=1e4)
+    outs = []
+    for graphs, stride in ((False, 0), (True, 0), (True, 3)):
+        print(prec, "graphs", graphs, "probe stride", stride, flush=True)
+        with Case(h, prec=prec) as c:
+            c.set_graphs(graphs)
+            c.probe(stride)
+            sta, ite = c.run(max_iter=6)
+            st = c.probe_stats(51)
+            outs.append((sta, ite, c.fields()))
+        if stride:
+            assert st["sampled"] > 0 and st["avg_ms"] > 0
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
+        for k in FIELDS:
+            assert np.array_equal(o[2][k], outs[0][2][k]), k
+
+
 def test_case_is_bitwise_repeatable(gauss_table):
     g = load_golden("nc2")
     outs = []

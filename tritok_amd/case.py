@@ -28,6 +28,7 @@ _probe = _lib._sig("tt_case_probe", i32, P, i32)
 _probe_stats = _lib._sig("tt_case_probe_stats", i32, P, i32, C.POINTER(C.c_double), C.POINTER(i64), C.POINTER(i64))
 _sysinfo = _lib._sig("tt_case_system_info", i32, P, i32, C.POINTER(i64))
 _setgrid = _lib._sig("tt_case_set_grid", i32, P, i32)
+_setgraphs = _lib._sig("tt_case_set_graphs", i32, P, i32)
 
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
 TIMING_KEYS = ["boundary_vorticity", "km", "temperature", "vte", "convergence", "reserved", "step_total",
@@ -110,6 +111,11 @@ class Case:
     def set_grid(self, grid):
         """Cell-kernel grid for every least-squares system (0: autotuned)."""
         check(_setgrid(self._h, int(grid)), "tt_case_set_grid")
+
+    def set_graphs(self, on):
+        """Least-squares iterations replayed from captured HIP graphs (True)
+        or launched kernel by kernel on the stream (False, the default)."""
+        check(_setgraphs(self._h, 1 if on else 0), "tt_case_set_graphs")
 
     def system_info(self, sys):
         """sys: 0..2 vorticity x, y, z; 3 energy; 4 kinematics."""

@@ -251,6 +251,7 @@ struct tt_case {
   int lsqr_prec = kBlocks64;  // blocks the least-squares iterations read (BlockPrec)
   float dlse[9];
   LsqrProbe probe51, probe27;  // lsqr_A samples on 51-row (VTE, T) and 27-row (KM) systems
+  bool graphs = false;         // LSQR iterations replayed from captured graphs (else stream launches)
   double beta = 0, beta2 = 0, beta3 = 0, ren = 0;
   int it = 0;
   bool finished = false;
@@ -324,7 +325,7 @@ int solve_subsys(tt_case &c, SubSys &S, const double *fnode, const double *fflux
                      fnode, fflux, c.x.p);
   TT_LAUNCH_CHECK();
   LsqrResult r = block_lsqr(S.bs, c.work, c.b.p, c.x.p, c.cv.lsqs_maxit, eps4, c.lsqr_prec, c.s,
-                            S.R == 51 ? &c.probe51 : &c.probe27);
+                            S.R == 51 ? &c.probe51 : &c.probe27, c.graphs);
   if (r.istop > 3) throw std::runtime_error("LSQR stopped with istop > 3 (ill-conditioned or iteration limit)");
   return r.nit;
 }
@@ -726,6 +727,14 @@ extern "C" int tt_case_set_grid(tt_case *c, int32_t grid) {
     if (!c || grid < 0) throw std::runtime_error("grid must be >= 0");
     for (SubSys *ss : {&c->vte[0], &c->vte[1], &c->vte[2], &c->tsys, &c->km})
       for (int p = 0; p < 3; ++p) ss->bs.grid_a[p] = grid;  // 0: tuned again on next use
+    return 0;
+  });
+}
+
+extern "C" int tt_case_set_graphs(tt_case *c, int32_t on) {
+  return guarded("tt_case_set_graphs", [&] {
+    if (!c || on < 0 || on > 1) throw std::runtime_error("on must be 0 or 1");
+    c->graphs = on == 1;
     return 0;
   });
 }

@@ -27,6 +27,7 @@ struct LsqrState {
   double atol, btol, ctol, nrm_b, nrm_r;
   double pending_phi, pending_rho, pending_theta, pending_tau;  // rotation awaiting its test
   int itn, istop, done, itnlim;
+  int kseq;                           // iteration index of the cell kernel that wrote this copy (0 = initial)
 };
 
 // Block storage: per cell an R x L row-major block, padded to `stride`
@@ -40,6 +41,26 @@ inline int block_stride16(int R, int L) { return R * block_row16(L); }
 
 // precision of the blocks an LSQR solve iterates with
 enum BlockPrec { kBlocks64 = 0, kBlocks32 = 1, kBlocks16 = 2 };
+
+struct LsqrWork;
+
+// Captured iteration chunks of one system for one work area and block
+// precision: exec[i] runs 2^(i+1) iterations (cell kernel + unknown kernel
+// each) starting at an odd iteration index.  Move-only; destroys its graphs.
+struct LsqrGraphs {
+  static constexpr int kLevels = 6;  // chunks of 2 .. 64 iterations
+  const LsqrWork *wk = nullptr;      // work area the graphs were captured with
+  unsigned gen = 0;                  // its allocation generation at capture
+  int grid = 0;                      // cell-kernel grid at capture
+  hipGraphExec_t exec[kLevels] = {};
+  LsqrGraphs() = default;
+  LsqrGraphs(const LsqrGraphs &) = delete;
+  LsqrGraphs &operator=(const LsqrGraphs &) = delete;
+  LsqrGraphs(LsqrGraphs &&o) noexcept;
+  LsqrGraphs &operator=(LsqrGraphs &&o) noexcept;
+  void reset();
+  ~LsqrGraphs() { reset(); }
+};
 
 struct BlockSys {
   int ncell = 0, R = 0, L = 0, n = 0, stride = 0;
@@ -55,6 +76,7 @@ struct BlockSys {
   int grid_a[3] = {0, 0, 0};  // cell-kernel blocks per block precision (BlockPrec), 0 = resident blocks
   int last_nit = 0;       // iterations of the previous solve (sizes the first launch batch)
   float last_eps = 0.f;   // its tolerance
+  LsqrGraphs graphs[3];   // captured iteration chunks per block precision
 };
 
 // Builds the solver structure from the full unscaled blocks on the device
@@ -71,6 +93,7 @@ struct LsqrWork {
   DBuf<double> pb1, pb2;    // per-block partial sums of the unknown kernel
   DBuf<LsqrState> state;    // two copies
   LsqrState *host_state = nullptr;  // pinned
+  unsigned gen = 0;         // bumped whenever a buffer above is reallocated
   void ensure(int m, int n, int ncell, int L);
   ~LsqrWork();
 };
@@ -96,8 +119,11 @@ struct LsqrResult {
 // start on entry and the solution on exit (device).  prec (BlockPrec) selects
 // the blocks the iterations read (FP64; FP32 or FP16 with FP32 arithmetic);
 // the initial residual b - A*x0 always uses the double-precision blocks.
+// graphs: the iterations are replayed from captured chunks (s.graphs) rather
+// than launched one kernel at a time; the kernels and their arguments are the
+// same either way, and so are the results.
 LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
-                      hipStream_t st, LsqrProbe *probe = nullptr);
+                      hipStream_t st, LsqrProbe *probe = nullptr, bool graphs = true);
 
 // Picks the cell kernel's grid for block precision prec by timing candidate
 // grids on a scratch solve (once per system shape and precision in a process).

@@ -7,9 +7,8 @@
 //           block obtains the same alpha), runs the scalar recurrences
 //           (stopping tests of iteration k-2, plane rotation of iteration
 //           k-1), then u' <- A v - alpha u, ||u'||^2 partial and P <- A^T u'
-//           for its cell.  The block is read once from HBM with 16-byte loads
-//           that are all in flight together, staged in LDS and used for both
-//           products (FP64 arithmetic on FP64 blocks; FP32 arithmetic with two
+//           for its cell.  The block is read once from HBM with 16-byte
+//           loads, staged in LDS and used for both products (FP64 arithmetic on FP64 blocks; FP32 arithmetic with two
 //           accumulators on the FP32 blocks of the mixed mode, whose solves are
 //           anchored by an FP64 initial residual).  Block 0 stores the scalars.
 //   lsqr_B (one thread per unknown): reduces the cell partials to beta,
@@ -24,6 +23,7 @@
 // kernel of iteration k+1 has applied the x update of iteration k and
 // nothing more.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(64) lsqr_r0(const double *__restrict__ A, cons
 // iteration stopped, the cell blocks' work of this launch is simply unused
 // (the unknown kernel that follows sees the stop and leaves x unchanged).
 template <typename T, int R, int L>
-__global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, int cgroup, const T *__restrict__ A,
+__global__ void __launch_bounds__(64) lsqr_A(int ncell, int cgroup, const T *__restrict__ A,
                                              const int *__restrict__ colmap,
                                              const double *__restrict__ vp, double *__restrict__ u,
                                              double *__restrict__ P, double *__restrict__ pa1,
@@ -232,12 +232,16 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, int cgroup, const
   __shared__ __align__(16) float vsf[64], uuf[64];
   constexpr bool f32 = sizeof(T) <= 4;  // FP32 / FP16 blocks: FP32 products, two accumulators
   const int t = threadIdx.x;
+  // the iteration index travels with the scalars, so a launch takes no
+  // per-iteration argument (a captured chunk replays unchanged)
+  const int k = sin->kseq + 1;
   if (blockIdx.x == 0) {
     const double nvs = reduce_partials<64>(pb1, nb, sh);
     const double dks = k >= 3 ? reduce_partials<64>(pb2, nb, sh) : 0.0;
     if (t == 0) {
       LsqrState o;
       scalar_update(k, *sin, nvs, dks, o);
+      o.kseq = k;
       *sout = o;
     }
     return;
@@ -252,7 +256,10 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, int cgroup, const
   const int cend = min(ncell, g1 * cgroup);
   int c = g0 * cgroup;
   if (c >= cend) return;
-  int j = t < L ? colmap[(size_t)c * L + t] : -1;
+  // column maps run one cell ahead of the v gathers that need them, so the
+  // gather of cell c+1 is issued before cell c is computed
+  const int j = t < L ? colmap[(size_t)c * L + t] : -1;
+  int jn = (c + 1 < cend && t < L) ? colmap[(size_t)(c + 1) * L + t] : -1;
   double uo = t < R ? u[(size_t)c * R + t] : 0.0;
   RegBlock<T, S> rb;
   rb.load(A + (size_t)c * S);
@@ -265,7 +272,7 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, int cgroup, const
   for (;;) {
     rb.store(blk);
     {
-      const double vn = (t < L && j >= 0) ? (alpha > 0.0 ? vj / alpha : vj) : 0.0;
+      const double vn = t < L ? (alpha > 0.0 ? vj / alpha : vj) : 0.0;
       if constexpr (f32)
         vsf[t] = (float)vn;
       else
@@ -273,12 +280,13 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, int cgroup, const
     }
     __syncthreads();
     const int cn = c + 1;
-    int jn = -1;
-    double uon = 0.0;
+    double vjn = 0.0, uon = 0.0;
+    int jnn = -1;
     if (cn < cend) {
-      jn = t < L ? colmap[(size_t)cn * L + t] : -1;
+      vjn = jn >= 0 ? vp[jn] : 0.0;
       uon = t < R ? u[(size_t)cn * R + t] : 0.0;
       rb.load(A + (size_t)cn * S);
+      if (cn + 1 < cend && t < L) jnn = colmap[(size_t)(cn + 1) * L + t];
     }
     double un = 0.0;
     {
@@ -378,8 +386,7 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, int cgroup, const
       acc = 0.0;
     }
     if (cn >= cend) break;
-    vj = jn >= 0 ? vp[jn] : 0.0;
-    j = jn, uo = uon, c = cn;
+    vj = vjn, jn = jnn, uo = uon, c = cn;
     __syncthreads();  // blk, vs and uu are rewritten next
   }
 }
@@ -387,12 +394,13 @@ __global__ void __launch_bounds__(64) lsqr_A(int k, int ncell, int cgroup, const
 // ---------------------------------------------------------------------------
 // iteration k, unknown part (k = 0: initialisation after r0)
 template <int W>
-__global__ void __launch_bounds__(kNB) lsqr_B(int n, int k, const int *__restrict__ gtab, const double *__restrict__ P,
+__global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gtab, const double *__restrict__ P,
                                               double *__restrict__ vp, double *__restrict__ w,
                                               double *__restrict__ x, const double *__restrict__ pa1,
                                               const double *__restrict__ pa2, int na, double *__restrict__ pb1,
                                               double *__restrict__ pb2, LsqrState *st) {
   __shared__ double sh[kNB / 64];
+  const int k = st->kseq;  // written by this iteration's cell kernel (0: the initial copy)
   if (k >= 1 && st->done) return;
   const int j = xcd_logical(blockIdx.x, 0, gridDim.x) * kNB + threadIdx.x;
   int idx[W];
@@ -500,20 +508,20 @@ void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st, hipE
   const int g = std::min(ngroup, tuned > 0 ? tuned : persistent_grid<T, R, L>(s.ncell));
   wk.na = ngroup;
   if (e0)
-    hipExtLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, e0, e1, 0, k, s.ncell,
+    hipExtLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, e0, e1, 0, s.ncell,
                           cell_group(s.ncell), A, s.colmap.p,
                           (const double *)wk.v.p, wk.u.p, wk.P.p, wk.pa1.p, (const double *)wk.pb1.p,
                           (const double *)wk.pb2.p, (int)ceil_div(s.n, kNB),
                           (const LsqrState *)(wk.state.p + ((k - 1) & 1)), wk.state.p + (k & 1));
   else
-    hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, k, s.ncell, cell_group(s.ncell), A,
+    hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, s.ncell, cell_group(s.ncell), A,
                        s.colmap.p, wk.v.p, wk.u.p,
                        wk.P.p, wk.pa1.p, wk.pb1.p, wk.pb2.p, (int)ceil_div(s.n, kNB), wk.state.p + ((k - 1) & 1),
                        wk.state.p + (k & 1));
 }
 template <int W>
 void launch_B(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
-  hipLaunchKernelGGL((lsqr_B<W>), dim3(ceil_div(s.n, kNB)), dim3(kNB), 0, st, s.n, k, s.gtab.p, wk.P.p, wk.v.p,
+  hipLaunchKernelGGL((lsqr_B<W>), dim3(ceil_div(s.n, kNB)), dim3(kNB), 0, st, s.n, s.gtab.p, wk.P.p, wk.v.p,
                      wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, k == 0 ? s.ncell : wk.na, wk.pb1.p, wk.pb2.p,
                      wk.state.p + (k & 1));
 }
@@ -581,7 +589,51 @@ void probe_collect(LsqrProbe *p, int nit) {
   p->used = 0;
 }
 
+std::atomic<unsigned> g_work_gen{0};
+
+// records 2^(lvl+1) iterations starting at an odd index (the state copies
+// alternate with the parity of the index, so such a chunk replays at any odd
+// index); nothing runs during the capture
+hipGraphExec_t capture_chunk(BlockSys &s, LsqrWork &wk, int prec, int iters, hipStream_t st) {
+  hipGraph_t g = nullptr;
+  TT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int i = 1; i <= iters; ++i) {
+    stepA(s, wk, i, prec, st, nullptr, nullptr);
+    stepB(s, wk, i, st);
+  }
+  const hipError_t le = hipGetLastError();
+  const hipError_t ee = hipStreamEndCapture(st, &g);
+  if (le != hipSuccess || ee != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    ::tt::hip_check(le != hipSuccess ? le : ee, "LSQR iteration capture", __FILE__, __LINE__);
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  ::tt::hip_check(ie, "LSQR iteration graph instantiation", __FILE__, __LINE__);
+  return ex;
+}
+
 }  // namespace
+
+LsqrGraphs::LsqrGraphs(LsqrGraphs &&o) noexcept : wk(o.wk), gen(o.gen), grid(o.grid) {
+  for (int i = 0; i < kLevels; ++i) exec[i] = o.exec[i], o.exec[i] = nullptr;
+  o.wk = nullptr;
+}
+LsqrGraphs &LsqrGraphs::operator=(LsqrGraphs &&o) noexcept {
+  if (this != &o) {
+    reset();
+    wk = o.wk, gen = o.gen, grid = o.grid;
+    for (int i = 0; i < kLevels; ++i) exec[i] = o.exec[i], o.exec[i] = nullptr;
+    o.wk = nullptr;
+  }
+  return *this;
+}
+void LsqrGraphs::reset() {
+  for (auto &e : exec)
+    if (e) (void)hipGraphExecDestroy(e), e = nullptr;
+  wk = nullptr, gen = 0, grid = 0;
+}
 
 // ---------------------------------------------------------------------------
 void block_setup(BlockSys &s, int ncell, int R, int L, int n, const std::vector<int> &colmap, const double *Afull,
@@ -651,17 +703,22 @@ LsqrProbe::~LsqrProbe() {
 }
 
 void LsqrWork::ensure(int m_, int n_, int ncell_, int L_) {
-  if (m_ > m) u.alloc(m_), m = m_;
-  if (n_ > n) v.alloc(n_), w.alloc(n_), x.alloc(n_), n = n_;
-  if ((size_t)ncell_ * L_ > P.n) P.alloc((size_t)ncell_ * L_);
-  if ((size_t)ncell_ > pa1.n) pa1.alloc(ncell_), pa2.alloc(ncell_);
+  bool moved = false;
+  if (m_ > m) u.alloc(m_), m = m_, moved = true;
+  if (n_ > n) v.alloc(n_), w.alloc(n_), x.alloc(n_), n = n_, moved = true;
+  if ((size_t)ncell_ * L_ > P.n) P.alloc((size_t)ncell_ * L_), moved = true;
+  if ((size_t)ncell_ > pa1.n) pa1.alloc(ncell_), pa2.alloc(ncell_), moved = true;
   const size_t nb = ceil_div(n_, kNB);
-  if (nb > pb1.n) pb1.alloc(nb), pb2.alloc(nb);
+  if (nb > pb1.n) pb1.alloc(nb), pb2.alloc(nb), moved = true;
   ncell = std::max(ncell, ncell_), L = std::max(L, L_);
   if (!state.p) {
     state.alloc(2);
     TT_HIP(hipHostMalloc((void **)&host_state, sizeof(LsqrState), hipHostMallocDefault));
+    moved = true;
   }
+  // process-wide numbering: a later work area at the same address never
+  // matches graphs captured with an earlier one
+  if (moved) gen = ++g_work_gen;
 }
 
 LsqrWork::~LsqrWork() {
@@ -764,7 +821,7 @@ void block_tune(BlockSys &s, int prec, hipStream_t st) {
 }
 
 LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
-                      hipStream_t st, LsqrProbe *probe) {
+                      hipStream_t st, LsqrProbe *probe, bool graphs) {
   const int m = s.ncell * s.R, n = s.n;
   if (prec == kBlocks32 && !s.has32) throw std::runtime_error("single-precision blocks not built");
   if (prec == kBlocks16 && !s.has16) throw std::runtime_error("half-precision blocks not built");
@@ -793,12 +850,48 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
     if (eps4 > s.last_eps) batch = std::max(8, batch / 2);  // looser tolerance: fewer iterations
   }
   const int more = std::max(4, s.last_nit / 8);
+  LsqrGraphs &G = s.graphs[prec];
+  if (graphs && (G.wk != &wk || G.gen != wk.gen || G.grid != s.grid_a[prec])) {
+    G.reset();
+    G.wk = &wk, G.gen = wk.gen, G.grid = s.grid_a[prec];
+  }
+  // one iteration on the stream (sampled launches record their own events)
+  auto one = [&](long pi) {
+    stepA(s, wk, k, prec, st, pi >= 0 ? probe->pool[2 * pi] : nullptr, pi >= 0 ? probe->pool[2 * pi + 1] : nullptr);
+    stepB(s, wk, k, st);
+    ++k;
+  };
+  // replays `pending` (even) iterations from k as chunks of 64, 32, .., 2
+  auto replay = [&](int pending) {
+    for (int lvl = LsqrGraphs::kLevels - 1; pending > 0; --lvl) {
+      const int len = 2 << lvl;
+      while (pending >= len) {
+        if (!G.exec[lvl]) G.exec[lvl] = capture_chunk(s, wk, prec, len, st);
+        TT_HIP(hipGraphLaunch(G.exec[lvl], st));
+        k += len, pending -= len;
+      }
+    }
+  };
   LsqrResult res;
   for (;;) {
-    for (int b2 = 0; b2 < batch; ++b2, ++k) {
-      const long pi = probe_begin(probe, k);
-      stepA(s, wk, k, prec, st, pi >= 0 ? probe->pool[2 * pi] : nullptr, pi >= 0 ? probe->pool[2 * pi + 1] : nullptr);
-      stepB(s, wk, k, st);
+    if (!graphs) {
+      for (int b2 = 0; b2 < batch; ++b2) one(probe_begin(probe, k));
+    } else {
+      // even counts from odd indices; a pair holding a sampled cell-kernel
+      // launch runs on the stream
+      int pending = 0;
+      for (int b2 = 0; b2 < batch; b2 += 2) {
+        const long p0 = probe_begin(probe, k + pending), p1 = probe_begin(probe, k + pending + 1);
+        if (p0 < 0 && p1 < 0) {
+          pending += 2;
+          continue;
+        }
+        replay(pending);
+        pending = 0;
+        one(p0);
+        one(p1);
+      }
+      replay(pending);
     }
     TT_LAUNCH_CHECK();
     TT_HIP(hipMemcpyAsync(wk.host_state, wk.state.p + ((k - 1) & 1), sizeof(LsqrState), hipMemcpyDeviceToHost, st));

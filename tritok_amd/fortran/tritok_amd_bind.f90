@@ -201,6 +201,12 @@ MODULE tritok_amd_bind
       INTEGER(C_INT32_T), VALUE :: grid
       INTEGER(C_INT) :: tt_case_set_grid
     END FUNCTION
+    FUNCTION tt_case_set_graphs(h, on) BIND(C, NAME="tt_case_set_graphs")
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: on
+      INTEGER(C_INT) :: tt_case_set_graphs
+    END FUNCTION
     FUNCTION tt_case_set_anchor_period(h, period) BIND(C, NAME="tt_case_set_anchor_period")
       IMPORT :: C_INT, C_INT32_T, C_PTR
       TYPE(C_PTR), VALUE :: h
