@@ -319,13 +319,14 @@ struct EvTimer {
   }
 };
 
-int solve_subsys(tt_case &c, SubSys &S, const double *fnode, const double *fflux, float eps4) {
+// slot: the field component a system shared by components is solved for
+int solve_subsys(tt_case &c, SubSys &S, const double *fnode, const double *fflux, float eps4, int slot) {
   const int n = S.n;
   hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, c.s, n, c.cv.mesh.nnodes, S.unk_field.p,
                      fnode, fflux, c.x.p);
   TT_LAUNCH_CHECK();
   LsqrResult r = block_lsqr(S.bs, c.work, c.b.p, c.x.p, c.cv.lsqs_maxit, eps4, c.lsqr_prec, c.s,
-                            S.R == 51 ? &c.probe51 : &c.probe27, c.graphs);
+                            S.R == 51 ? &c.probe51 : &c.probe27, c.graphs, slot);
   if (r.istop > 3) throw std::runtime_error("LSQR stopped with istop > 3 (ill-conditioned or iteration limit)");
   return r.nit;
 }
@@ -495,7 +496,7 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
         hipLaunchKernelGGL(km_rhs_kernel, dim3(ceil_div((size_t)nc * 27, 128)), dim3(128), 0, s, nc, nn, e, mp,
                            c->mesh.idc.p, c->Af_km.p, c->km.colknown.p, c->km.colfield.p, c->vel.p, c->vort.p, c->b.p);
         TT_LAUNCH_CHECK();
-        its[3 + e] = solve_subsys(*c, c->km, c->vel.p + (size_t)nn * e, nullptr, c->dlse[e]);
+        its[3 + e] = solve_subsys(*c, c->km, c->vel.p + (size_t)nn * e, nullptr, c->dlse[e], e);
         hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c->km.n, 256)), dim3(256), 0, s, c->km.n, nn, cv.urDv[e],
                            c->km.unk_field.p, c->x.p, c->vel.p + (size_t)nn * e, nullptr);
         if (c->kobc.p)
@@ -513,7 +514,7 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
                          c->Af_t.p, c->tsys.colknown.p, c->tsys.colfield.p, c->vel.p, c->temp.p, c->qtemp.p,
                          c->ttime.p, enazalfa, c->b.p);
       TT_LAUNCH_CHECK();
-      its[9] = solve_subsys(*c, c->tsys, c->temp.p, c->qtemp.p, c->dlse[6]);
+      its[9] = solve_subsys(*c, c->tsys, c->temp.p, c->qtemp.p, c->dlse[6], 0);
       hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c->tsys.n, 256)), dim3(256), 0, s, c->tsys.n, nn, cv.urDT,
                          c->tsys.unk_field.p, c->x.p, c->temp.p, c->qtemp.p);
       TT_LAUNCH_CHECK();
@@ -529,7 +530,8 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
                            c->vort.p, c->qvort.p + (size_t)nq * e, c->rhsv.p + (size_t)nn * e, c->temp.p, c->ren,
                            cv.iDT > 0 ? 1 : 0, fac, cv.gx, cv.gy, cv.gz, c->b.p);
         TT_LAUNCH_CHECK();
-        its[6 + e] = solve_subsys(*c, c->vte[e], c->vort.p + (size_t)nn * e, c->qvort.p + (size_t)nq * e, c->dlse[3 + e]);
+        its[6 + e] = solve_subsys(*c, c->vte[e], c->vort.p + (size_t)nn * e, c->qvort.p + (size_t)nq * e, c->dlse[3 + e],
+                                  0);
         hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c->vte[e].n, 256)), dim3(256), 0, s, c->vte[e].n, nn,
                            cv.urDw[e], c->vte[e].unk_field.p, c->x.p, c->vort.p + (size_t)nn * e,
                            c->qvort.p + (size_t)nq * e);

@@ -74,8 +74,17 @@ struct BlockSys {
   DBuf<double> q;         // column scaling
   bool has32 = false, has16 = false;
   int grid_a[3] = {0, 0, 0};  // cell-kernel blocks per block precision (BlockPrec), 0 = resident blocks
-  int last_nit = 0;       // iterations of the previous solve (sizes the first launch batch)
-  float last_eps = 0.f;   // its tolerance
+  // recent solves per right-hand-side slot (a system solved for several
+  // field components keeps one history per component): their iteration
+  // counts size the first launch batch of the next solve
+  struct SolveHistory {
+    static constexpr int kLen = 4;
+    int nit[kLen] = {};
+    int n = 0, head = 0;  // entries held; next entry to write
+    void add(int it);
+    int predict() const;  // -1 without history
+  };
+  SolveHistory hist[3];
   LsqrGraphs graphs[3];   // captured iteration chunks per block precision
 };
 
@@ -122,8 +131,9 @@ struct LsqrResult {
 // graphs: the iterations are replayed from captured chunks (s.graphs) rather
 // than launched one kernel at a time; the kernels and their arguments are the
 // same either way, and so are the results.
+// slot (0..2) selects the solve history that sizes the launch batches.
 LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
-                      hipStream_t st, LsqrProbe *probe = nullptr, bool graphs = true);
+                      hipStream_t st, LsqrProbe *probe = nullptr, bool graphs = true, int slot = 0);
 
 // Picks the cell kernel's grid for block precision prec by timing candidate
 // grids on a scratch solve (once per system shape and precision in a process).

@@ -400,10 +400,11 @@ __global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gta
                                               const double *__restrict__ pa2, int na, double *__restrict__ pb1,
                                               double *__restrict__ pb2, LsqrState *st) {
   __shared__ double sh[kNB / 64];
-  const int k = st->kseq;  // written by this iteration's cell kernel (0: the initial copy)
-  if (k >= 1 && st->done) return;
   const int j = xcd_logical(blockIdx.x, 0, gridDim.x) * kNB + threadIdx.x;
+  // the index table and the unknown's own vectors do not depend on the
+  // state: they are requested together with it
   int idx[W];
+  double vold = 0.0, wo = 0.0, xo = 0.0;
   if (j < n) {
     const int4 *g = reinterpret_cast<const int4 *>(gtab + (size_t)j * W);
 #pragma unroll
@@ -411,15 +412,16 @@ __global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gta
       int4 v = g[q];
       idx[4 * q] = v.x, idx[4 * q + 1] = v.y, idx[4 * q + 2] = v.z, idx[4 * q + 3] = v.w;
     }
+    vold = vp[j], wo = w[j], xo = x[j];
   }
-  // the column's gathers and vector loads are issued before the partial-sum
-  // reduction, so their latency overlaps it
-  double pv[W], vold = 0.0, wo = 0.0, xo = 0.0;
+  const int k = st->kseq;  // written by this iteration's cell kernel (0: the initial copy)
+  if (k >= 1 && st->done) return;
+  // the column's gathers are issued before the partial-sum reduction, so
+  // their latency overlaps it
+  double pv[W];
   if (j < n) {
 #pragma unroll
     for (int e = 0; e < W; ++e) pv[e] = idx[e] >= 0 ? P[idx[e]] : 0.0;
-    vold = vp[j];
-    if (k >= 2) wo = w[j], xo = x[j];
   }
   const double beta = sqrt(reduce_partials<kNB>(pa1, na, sh));  // beta_{k+1}
   const double alpha = st->alpha, t1 = st->t1, t2 = st->t2, t3 = st->t3;
@@ -678,6 +680,25 @@ void block_setup(BlockSys &s, int ncell, int R, int L, int n, const std::vector<
   TT_HIP(hipStreamSynchronize(st));  // host vectors above are released on return
 }
 
+void BlockSys::SolveHistory::add(int it) {
+  nit[head] = it;
+  head = (head + 1) % kLen;
+  n = std::min(n + 1, kLen);
+}
+
+// The count of the previous solve, or of the one before it when that lag
+// has predicted the recent solves better: under the Picard loop the counts
+// of a system often alternate between two levels from step to step (the
+// right-hand side's change alternates), which a lag of two follows exactly.
+int BlockSys::SolveHistory::predict() const {
+  if (n == 0) return -1;
+  auto at = [&](int i) { return nit[(head - 1 - i + 2 * kLen) % kLen]; };  // i = 0: most recent
+  if (n < 3) return at(0);
+  const int e1 = std::abs(at(0) - at(1)) + (n >= 4 ? std::abs(at(1) - at(2)) : 0);
+  const int e2 = std::abs(at(0) - at(2)) + (n >= 4 ? std::abs(at(1) - at(3)) : 0);
+  return e2 < e1 ? at(1) : at(0);
+}
+
 void block_make16(BlockSys &s, hipStream_t st) {
   if (s.has16) return;
   const int S16 = block_stride16(s.R, s.L);
@@ -821,8 +842,9 @@ void block_tune(BlockSys &s, int prec, hipStream_t st) {
 }
 
 LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
-                      hipStream_t st, LsqrProbe *probe, bool graphs) {
+                      hipStream_t st, LsqrProbe *probe, bool graphs, int slot) {
   const int m = s.ncell * s.R, n = s.n;
+  if (slot < 0 || slot > 2) throw std::runtime_error("solve history slot must be 0..2");
   if (prec == kBlocks32 && !s.has32) throw std::runtime_error("single-precision blocks not built");
   if (prec == kBlocks16 && !s.has16) throw std::runtime_error("half-precision blocks not built");
   block_tune(s, prec, st);
@@ -840,16 +862,16 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
   TT_LAUNCH_CHECK();
   stepB(s, wk, 0, st);
   TT_LAUNCH_CHECK();
-  // the previous solve's count predicts this one's (the Picard loop changes
-  // only the right-hand side and, through the dynamic tolerance, eps):
-  // termination after nit iterations is seen by the cell kernel of
-  // iteration nit + 2
+  // recent solves of this slot predict the count (the Picard loop changes
+  // only the right-hand side and, through the dynamic tolerance, eps)
   int k = 1, batch = 8;
-  if (s.last_nit > 0) {
-    batch = s.last_nit + 2;
-    if (eps4 > s.last_eps) batch = std::max(8, batch / 2);  // looser tolerance: fewer iterations
-  }
-  const int more = std::max(4, s.last_nit / 8);
+  // termination after nit iterations shows two launches later; two more
+  // cover a small rise (an undershoot costs a host round trip, an idle
+  // launch a few microseconds)
+  const int pred = s.hist[slot].predict();
+  if (pred >= 0) batch = pred + 4;
+  const int more = std::max(4, pred / 8);
+  int extra = 0;  // batches after the first
   LsqrGraphs &G = s.graphs[prec];
   if (graphs && (G.wk != &wk || G.gen != wk.gen || G.grid != s.grid_a[prec])) {
     G.reset();
@@ -902,11 +924,13 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
       break;
     }
     if (k > init.itnlim + 3) throw std::runtime_error("LSQR did not signal termination");
-    batch = std::max(more, std::min(64, k / 4));
+    // after a predicted first batch the remainder is usually short: small
+    // batches, doubling; without a prediction a quarter of the count so far
+    batch = pred >= 0 ? std::min(64, more << std::min(extra, 4)) : std::max(more, std::min(64, k / 4));
+    ++extra;
   }
   probe_collect(probe, res.nit);
-  s.last_eps = eps4;
-  s.last_nit = res.nit;
+  s.hist[slot].add(res.nit);
   // x = q * dx + x0 (x0 is still in x: the kernels only wrote wk.x)
   hipLaunchKernelGGL(finish_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, s.q.p, wk.x.p, x, x);
   TT_LAUNCH_CHECK();
