@@ -93,7 +93,7 @@ def read_traffic(kernel):
     return None
 
 
-def accuracy_leg(n, mixed):
+def accuracy_leg(n, mixed, anchor_period=16):
     """Converged fields of the heated cube at tight tolerances (RELA 1e-10,
     LSQR eps 1e-12, fixed tolerance), FP64 and mixed, from the same start."""
     from tritok_amd import workloads
@@ -101,6 +101,7 @@ def accuracy_leg(n, mixed):
     host = workloads.natural_convection(n, eps=1e-10, lsqs_eps=1e-12, dlse=(0, 10.0, 0.001), nnlit=20000)
     out = {}
     with Case(host, prec=mixed) as c:
+        c.set_anchor_period(anchor_period)
         for prec in ("fp64", mixed):
             c.set_precision(prec)
             c.reset()
@@ -336,10 +337,14 @@ def lsqr_matrix_bytes(info, value_bytes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    # 16 timed steps after 2 warm-up steps hold exactly one FP64 anchor step of
+    # the kinematics (period 16): the steady-state share of that work
+    ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=20)
     ap.add_argument("--probe-stride", type=int, default=32)
+    ap.add_argument("--anchor-period", type=int, default=16,
+                    help="Picard iterations per FP64 anchor step of the reduced-precision kinematics")
     ap.add_argument("--graphs", action="store_true",
                     help="replay the least-squares iterations from captured HIP graph chunks instead of launching "
                          "every kernel on the stream")
@@ -394,6 +399,7 @@ def main():
     case = Case(host, prec=args.mixed)
     setup_s = time.perf_counter() - t0
     case.set_graphs(args.graphs)
+    case.set_anchor_period(args.anchor_period)
     tm = case.timings()
     log("setup %.2fs (macro %.0f ms, sdkm %.0f ms, LU %.0f ms, systems %.0f ms), device bytes %.2f GB" % (
         setup_s, tm["setup_macro"], tm["setup_sdkm"], tm["setup_lu"], tm["setup_systems"],
@@ -446,7 +452,7 @@ def main():
         cpu = cpu_sample.picard_rate(host, case, legs["fp64"]["nits"], budget_s=args.cpu_seconds, log=log)
 
     case.close()  # free the 20^3 case before the other legs
-    acc = accuracy_leg(args.accuracy_n, args.mixed) if args.accuracy_n > 0 else None
+    acc = accuracy_leg(args.accuracy_n, args.mixed, args.anchor_period) if args.accuracy_n > 0 else None
     mixkin = mixed_kinematics_leg(args.mixed_kin) if args.mixed_kin > 0 else None
     batched = batched_integrals_leg(args.batch_cells) if args.batch_cells > 0 else None
     bsolves = batched_solves_leg(args.batch_solves) if args.batch_solves > 0 else None
@@ -464,7 +470,10 @@ def main():
                    "subdomains": n ** 3,
                    "precision": "%s least-squares blocks (FP32 accumulation) with FP64 residual anchors, "
                                 "scalars and vectors; kinematics products in %s with per-row 2^k scales, FP64 anchor "
-                                "step every 8 iterations" % ("FP16" if half else "FP32", args.mixed.upper()),
+                                "step every %d iterations" % ("FP16" if half else "FP32", args.mixed.upper(),
+                                                              args.anchor_period),
+                   "anchor_steps_timed": sum(1 for i in range(args.warmup, args.warmup + args.steps)
+                                             if i % max(1, args.anchor_period) == 0),
                    "lsqr_launch": "chunks of 2..64 iterations replayed from captured HIP graphs" if args.graphs
                                   else "kernel by kernel on the stream"},
         "fp64_iters_per_s": fp64_rate, "mixed_over_fp64": value / fp64_rate,

@@ -324,7 +324,7 @@ int tt_case_reset(tt_case *c, const double *vel, const double *vort, const doubl
  * sdkm_rhs and the three inverse system matrices) run on copies in that
  * format with a power-of-two scale per row, taken on the change of the
  * fields since the last FP64 anchor step; an anchor step (FP64 products
- * and solves) runs every anchor period (default 8) Picard iterations and after
+ * and solves) runs every anchor period (default 16) Picard iterations and after
  * a reset, so the loop converges to the FP64 fixed point.  Copies are made on
  * first use. */
 int tt_case_set_precision(tt_case *c, int32_t prec);
@@ -426,7 +426,7 @@ typedef struct tt_server_info {
 } tt_server_info;
 
 /* slots concurrent cases; mem_budget device bytes for admission (0: 80 % of
- * the free memory); anchor_period of reduced-precision cases (0: 8);
+ * the free memory); anchor_period of reduced-precision cases (0: 16);
  * max_iter caps the Picard steps per case (0: the case's own RELA limit). */
 int tt_server_create(int32_t slots, int64_t mem_budget, int32_t anchor_period, int32_t max_iter,
                      tt_server **out);

@@ -85,9 +85,17 @@ __device__ __forceinline__ double known_part(int c, int r, int R, int L, const d
   return s;
 }
 
+// Ex, Ey, Ez: Ax - Dx, Ay - Dy, Az - Dz (the convection and buoyancy terms
+// use only these differences; the case forms them once at setup, in place
+// of Ax, Ay, Az)
 struct MacroP {
-  const double *B, *Ax, *Ay, *Az, *Dx, *Dy, *Dz, *Htx, *Hty, *Htz;
+  const double *B, *Ex, *Ey, *Ez, *Dx, *Dy, *Dz, *Htx, *Hty, *Htz;
 };
+
+__global__ void adv_diff_kernel(size_t n, double *__restrict__ a, const double *__restrict__ d) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = a[i] - d[i];
+}
 
 // SolveFDwTE right-hand side (lib_solvFDwTE.f:245-285) incl. BouyancyRHS
 __global__ void vte_rhs_kernel(int nc, int nn, int e, MacroP mp, const int *__restrict__ idc, const double *__restrict__ Af,
@@ -102,31 +110,30 @@ __global__ void vte_rhs_kernel(int nc, int nn, int e, MacroP mp, const int *__re
   const size_t m1 = (size_t)nc * 51;
   double v = known_part(c, it, 51, 51, Af, colknown, colfield, vort + (size_t)nn * e, qvort_e);
   const double rn = ren / 1.0;
+  // one pass over the matrices: the buoyancy terms (BouyancyRHS, added after
+  // the convection sum as there) are summed beside the convection terms and
+  // subtracted at the end (the reference subtracts them one at a time: a
+  // rounding-level difference)
+  double bsum = 0.0;
+  const double Rro = 1.0e20;
   for (int j = 0; j < 27; ++j) {
     size_t node = idc[c + (size_t)nc * j] - 1;
     double V0 = vel[node], V1 = vel[node + nn], V2 = vel[node + 2 * (size_t)nn];
     double W0 = vort[node], W1 = vort[node + nn], W2 = vort[node + 2 * (size_t)nn];
     double we = vort[node + (size_t)nn * e], ve = vel[node + (size_t)nn * e];
     size_t a = i + m1 * j;
+    const double ax = mp.Ex[a], ay = mp.Ey[a], az = mp.Ez[a];
     v = v - mp.B[a] * rhsv_e[node] * ren / 1.0 -
-        rn * (+mp.Ax[a] * (V0 * we - ve * W0) + mp.Ay[a] * (V1 * we - ve * W1) + mp.Az[a] * (V2 * we - ve * W2) -
-              mp.Dx[a] * (V0 * we - ve * W0) - mp.Dy[a] * (V1 * we - ve * W1) - mp.Dz[a] * (V2 * we - ve * W2));
+        rn * (+ax * (V0 * we - ve * W0) + ay * (V1 * we - ve * W1) + az * (V2 * we - ve * W2));
+    const double tt = buoy ? temp[node] - 0.0 / Rro : 0.0;
+    if (e == 0)
+      bsum += fac * (+gz * ay * tt - gy * az * tt);
+    else if (e == 1)
+      bsum += fac * (-gz * ax * tt + gx * az * tt);
+    else
+      bsum += fac * (+gy * ax * tt - gx * ay * tt);
   }
-  if (buoy) {
-    const double Rro = 1.0e20;
-    for (int j = 0; j < 27; ++j) {
-      size_t node = idc[c + (size_t)nc * j] - 1;
-      size_t a = i + m1 * j;
-      double tt = temp[node] - 0.0 / Rro;
-      double ax = mp.Ax[a] - mp.Dx[a], ay = mp.Ay[a] - mp.Dy[a], az = mp.Az[a] - mp.Dz[a];
-      if (e == 0)
-        v = v - fac * (+gz * ay * tt - gy * az * tt);
-      else if (e == 1)
-        v = v - fac * (-gz * ax * tt + gx * az * tt);
-      else
-        v = v - fac * (+gy * ax * tt - gx * ay * tt);
-    }
-  }
+  if (buoy) v = v - bsum;
   r[i] = v;
 }
 
@@ -145,9 +152,7 @@ __global__ void t_rhs_kernel(int nc, int nn, MacroP mp, const int *__restrict__ 
     size_t node = idc[c + (size_t)nc * j] - 1;
     double V0 = vel[node], V1 = vel[node + nn], V2 = vel[node + 2 * (size_t)nn], u = temp[node];
     size_t a = i + m1 * j;
-    v = v - mp.B[a] * ttime[node] * enazalfa -
-        enazalfa * (+mp.Ax[a] * V0 * u + mp.Ay[a] * V1 * u + mp.Az[a] * V2 * u - mp.Dx[a] * V0 * u -
-                    mp.Dy[a] * V1 * u - mp.Dz[a] * V2 * u);
+    v = v - mp.B[a] * ttime[node] * enazalfa - enazalfa * (+mp.Ex[a] * V0 * u + mp.Ey[a] * V1 * u + mp.Ez[a] * V2 * u);
   }
   r[i] = v;
 }
@@ -417,6 +422,13 @@ extern "C" int tt_case_create(const tt_case_view *cv, const double *vel, const d
     {
       EvTimer t(s);
       macro_integrals_dev(c->mesh, c->qh, cv->copy, c->macro, ont, s);
+      // Ax - Dx, Ay - Dy, Az - Dz in place of Ax, Ay, Az (see MacroP)
+      const size_t na = (size_t)nc * 51 * 27;
+      for (int d = 0; d < 3; ++d)
+        hipLaunchKernelGGL(adv_diff_kernel, dim3(ceil_div(na, 256)), dim3(256), 0, s, na,
+                           d == 0 ? c->macro.Ax.p : d == 1 ? c->macro.Ay.p : c->macro.Az.p,
+                           d == 0 ? c->macro.Dx.p : d == 1 ? c->macro.Dy.p : c->macro.Dz.p);
+      TT_LAUNCH_CHECK();
       c->t_phase[7] = t.stop();
     }
     if (cv->iBw != 0 && cv->iBw != 1) throw std::runtime_error("iBw must be 0 or 1");

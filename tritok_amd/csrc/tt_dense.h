@@ -32,7 +32,7 @@ struct DenseKin {
   DBuf<uint16_t> lo16[10];
   DBuf<float> lo32[10];
   DBuf<float> rscale;
-  int anchor_period = 8;      // FP64 step every anchor_period steps
+  int anchor_period = 16;     // FP64 step every anchor_period steps
   int since_anchor = 0;       // 0: the next step is an anchor step
   DBuf<double> YH_a, YD_a, dfi_a, sol_a, vel_a, vort_a;
   DBuf<float4> dv4, dw4;      // FP32 changes of the fields since the anchor step

@@ -63,7 +63,7 @@ struct Req {
 }  // namespace
 
 struct tt_server {
-  int slots = 1, device = 0, anchor = 8, max_iter = 0;
+  int slots = 1, device = 0, anchor = 16, max_iter = 0;
   int64_t budget = 0, in_use = 0, next_id = 1;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -157,7 +157,7 @@ extern "C" int tt_server_create(int32_t slots, int64_t mem_budget, int32_t ancho
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) throw std::runtime_error("hipMemGetInfo failed");
     s->budget = mem_budget > 0 ? mem_budget : (int64_t)(0.8 * (double)fr);
     s->slots = slots;
-    s->anchor = anchor_period > 0 ? anchor_period : 8;
+    s->anchor = anchor_period > 0 ? anchor_period : 16;
     s->max_iter = max_iter;
     s->t0 = now_s();
     for (int k = 0; k < slots; ++k) s->workers.emplace_back([p = s.get()] { p->worker(); });

@@ -30,7 +30,7 @@ PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
 
 
 class Server:
-    def __init__(self, slots=4, mem_budget=0, anchor_period=8, max_iter=0):
+    def __init__(self, slots=4, mem_budget=0, anchor_period=16, max_iter=0):
         h = P()
         check(_create(int(slots), int(mem_budget), int(anchor_period), int(max_iter), C.byref(h)), "tt_server_create")
         self._h = h
