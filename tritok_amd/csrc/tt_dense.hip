@@ -461,6 +461,8 @@ __global__ void __launch_bounds__(64) lgemv_kernel(int nb, int ldl, int ks_cnt, 
   const int chunk = (nb + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(nb, j0 + chunk);
   if (i0 >= nb) return;
   float a[4] = {0.f, 0.f, 0.f, 0.f};
+  // eight columns' loads in flight per lane
+#pragma unroll 8
   for (int j = j0; j < j1; ++j) {
     const float d = (float)(x_a ? x[j] - x_a[j] : x[j]);
     const float4 m = Load4<L>::at(M + (size_t)ldl * j + i0);
@@ -483,10 +485,11 @@ __global__ void __launch_bounds__(64) ldcorr_kernel(int nb, int ldl, int ks_cnt,
   const int chunk = (nb + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(nb, j0 + chunk);
   if (i0 >= nb) return;
   float a0[4] = {0.f, 0.f, 0.f, 0.f}, a1[4] = {0.f, 0.f, 0.f, 0.f};
+  // eight columns' loads in flight per lane (a zero change adds zero: no
+  // branch, so the loop unrolls)
+#pragma unroll 8
   for (int j = j0; j < j1; ++j) {
-    const double dd = dw[j];
-    if (dd == 0.0) continue;  // uniform across the wavefront
-    const float d = (float)dd;
+    const float d = (float)dw[j];
     const size_t col = (size_t)(gbn[j] - 1) * ldl + i0;
     const float4 m = Load4<L>::at(D0 + col);
     a0[0] += m.x * d, a0[1] += m.y * d, a0[2] += m.z * d, a0[3] += m.w * d;

@@ -403,25 +403,29 @@ __global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gta
   const int j = xcd_logical(blockIdx.x, 0, gridDim.x) * kNB + threadIdx.x;
   // the index table and the unknown's own vectors do not depend on the
   // state: they are requested together with it
+  // (threads past the last unknown repeat its loads and store nothing, and
+  // padding entries of the table gather entry 0 and discard it: no load sits
+  // behind a branch, so each wait counts exactly the loads it needs)
+  const int jj = j < n ? j : n - 1;
   int idx[W];
-  double vold = 0.0, wo = 0.0, xo = 0.0;
-  if (j < n) {
-    const int4 *g = reinterpret_cast<const int4 *>(gtab + (size_t)j * W);
+  {
+    const int4 *g = reinterpret_cast<const int4 *>(gtab + (size_t)jj * W);
 #pragma unroll
     for (int q = 0; q < W / 4; ++q) {
       int4 v = g[q];
       idx[4 * q] = v.x, idx[4 * q + 1] = v.y, idx[4 * q + 2] = v.z, idx[4 * q + 3] = v.w;
     }
-    vold = vp[j], wo = w[j], xo = x[j];
   }
+  const double vold = vp[jj], wo = w[jj], xo = x[jj];
   const int k = st->kseq;  // written by this iteration's cell kernel (0: the initial copy)
   if (k >= 1 && st->done) return;
   // the column's gathers are issued before the partial-sum reduction, so
   // their latency overlaps it
   double pv[W];
-  if (j < n) {
 #pragma unroll
-    for (int e = 0; e < W; ++e) pv[e] = idx[e] >= 0 ? P[idx[e]] : 0.0;
+  for (int e = 0; e < W; ++e) {
+    const double pe = P[idx[e] >= 0 ? idx[e] : 0];
+    pv[e] = idx[e] >= 0 ? pe : 0.0;
   }
   const double beta = sqrt(reduce_partials<kNB>(pa1, na, sh));  // beta_{k+1}
   const double alpha = st->alpha, t1 = st->t1, t2 = st->t2, t3 = st->t3;
