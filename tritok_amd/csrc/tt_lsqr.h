@@ -34,10 +34,11 @@ struct LsqrState {
 // elements (a multiple of 4) so that every block starts on a 16-byte boundary
 // in both precisions and is read with 16-byte vector loads.
 inline int block_stride(int R, int L) { return (R * L + 3) / 4 * 4; }
-// 16-bit blocks: rows padded to a multiple of 8 entries (16 bytes), so that a
-// row is read with 16-byte loads; the padding columns hold zeros
-inline int block_row16(int L) { return (L + 7) / 8 * 8; }
-inline int block_stride16(int R, int L) { return R * block_row16(L); }
+// 16-bit blocks: rows padded to a multiple of 4 entries (8 bytes), so that a
+// row is read with 8-byte loads, and the block to a multiple of 8 entries
+// (16 bytes) for 16-byte block loads; the padding holds zeros
+inline int block_row16(int L) { return (L + 3) / 4 * 4; }
+inline int block_stride16(int R, int L) { return (R * block_row16(L) + 7) / 8 * 8; }
 
 // precision of the blocks an LSQR solve iterates with
 enum BlockPrec { kBlocks64 = 0, kBlocks32 = 1, kBlocks16 = 2 };
@@ -96,7 +97,7 @@ void block_setup(BlockSys &s, int ncell, int R, int L, int n, const std::vector<
 
 struct LsqrWork {
   int m = 0, n = 0, ncell = 0, L = 0;
-  DBuf<double> u, v, w, x, P;
+  DBuf<double> u, v, w, x, P;  // P: FP32 values (first half) while a solve runs on FP16 blocks
   DBuf<double> pa1, pa2;    // partial sums of the cell kernels (per cell for r0, per block after)
   int na = 0;               // partials written by the last cell kernel
   DBuf<double> pb1, pb2;    // per-block partial sums of the unknown kernel

@@ -66,14 +66,14 @@ struct Vec16<_Float16> {
 };
 
 // padded block length of an R x L block in element type T (16-byte multiple;
-// 16-bit blocks have rows padded to row_len)
+// 16-bit blocks have rows padded to row_len, a multiple of four halves)
 template <typename T, int L>
 constexpr int row_len() {
-  return sizeof(T) == 2 ? (L + 7) / 8 * 8 : L;
+  return sizeof(T) == 2 ? (L + 3) / 4 * 4 : L;
 }
 template <typename T, int R, int L>
 constexpr int block_len() {
-  return sizeof(T) == 2 ? R * row_len<T, L>() : (R * L + 3) / 4 * 4;
+  return sizeof(T) == 2 ? (R * row_len<T, L>() + 7) / 8 * 8 : (R * L + 3) / 4 * 4;
 }
 
 // Loads one padded block into LDS: all 16-byte loads of a lane are issued
@@ -154,14 +154,14 @@ __global__ void to32_kernel(size_t n, const double *__restrict__ a, float *__res
 }
 
 // FP64 blocks (stride S, rows of L) -> FP16 blocks (rows padded to LP,
-// stride R*LP), padding zero
-__global__ void to16_kernel(size_t total16, int L, int LP, int S, int S16, const double *__restrict__ a,
+// stride S16 >= R*LP), padding zero
+__global__ void to16_kernel(size_t total16, int R, int L, int LP, int S, int S16, const double *__restrict__ a,
                             _Float16 *__restrict__ b) {
   size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= total16) return;
   const size_t c = k / S16;
   const int e = (int)(k % S16), r = e / LP, l = e % LP;
-  b[k] = l < L ? (_Float16)a[c * S + (size_t)r * L + l] : (_Float16)0.0f;
+  b[k] = (r < R && l < L) ? (_Float16)a[c * S + (size_t)r * L + l] : (_Float16)0.0f;
 }
 
 constexpr int kNB = 256;  // threads per block of the unknown kernel
@@ -293,24 +293,19 @@ __global__ void __launch_bounds__(64) lsqr_A(int ncell, int cgroup, const T *__r
       double s = 0.0;
       const int row = t < R ? t : R - 1;
       if constexpr (sizeof(T) == 2) {
-        // row of LP halves (zero padded), eight per 16-byte read; v padded with zeros
+        // row of LP halves (zero padded), four per 8-byte read; v padded with zeros
         constexpr int LP = row_len<T, L>();
-        const uint4 *rp = reinterpret_cast<const uint4 *>(blk + row * LP);
+        const uint2 *rp = reinterpret_cast<const uint2 *>(blk + row * LP);
         float f0 = 0.f, f1 = 0.f;
 #pragma unroll
-        for (int k = 0; k < LP / 8; ++k) {
-          const uint4 cw = rp[k];
-          const float4 va = *reinterpret_cast<const float4 *>(vsf + 8 * k);
-          const float4 vb = *reinterpret_cast<const float4 *>(vsf + 8 * k + 4);
+        for (int k = 0; k < LP / 4; ++k) {
+          const uint2 cw = rp[k];
+          const float4 va = *reinterpret_cast<const float4 *>(vsf + 4 * k);
           const _Float16 *h = reinterpret_cast<const _Float16 *>(&cw);
           f0 += (float)h[0] * va.x;
           f1 += (float)h[1] * va.y;
           f0 += (float)h[2] * va.z;
           f1 += (float)h[3] * va.w;
-          f0 += (float)h[4] * vb.x;
-          f1 += (float)h[5] * vb.y;
-          f0 += (float)h[6] * vb.z;
-          f1 += (float)h[7] * vb.w;
         }
         s = (double)f0 + (double)f1;
       } else if constexpr (f32) {
@@ -358,8 +353,9 @@ __global__ void __launch_bounds__(64) lsqr_A(int ncell, int cgroup, const T *__r
       s0 += __shfl_xor(s0, 32, 64);
       s1 += __shfl_xor(s1, 32, 64);
       if (hf == 0 && p < NPAIR) {
-        P[(size_t)c * L + 2 * p] = (double)s0;
-        if (2 * p + 1 < L) P[(size_t)c * L + 2 * p + 1] = (double)s1;
+        float *Pf = reinterpret_cast<float *>(P);
+        Pf[(size_t)c * L + 2 * p] = s0;
+        if (2 * p + 1 < L) Pf[(size_t)c * L + 2 * p + 1] = s1;
       }
     } else {
       double s = 0.0;
@@ -393,8 +389,8 @@ __global__ void __launch_bounds__(64) lsqr_A(int ncell, int cgroup, const T *__r
 
 // ---------------------------------------------------------------------------
 // iteration k, unknown part (k = 0: initialisation after r0)
-template <int W>
-__global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gtab, const double *__restrict__ P,
+template <int W, typename PT>
+__global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gtab, const PT *__restrict__ P,
                                               double *__restrict__ vp, double *__restrict__ w,
                                               double *__restrict__ x, const double *__restrict__ pa1,
                                               const double *__restrict__ pa2, int na, double *__restrict__ pb1,
@@ -424,8 +420,8 @@ __global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gta
   double pv[W];
 #pragma unroll
   for (int e = 0; e < W; ++e) {
-    const double pe = P[idx[e] >= 0 ? idx[e] : 0];
-    pv[e] = idx[e] >= 0 ? pe : 0.0;
+    const PT pe = P[idx[e] >= 0 ? idx[e] : 0];
+    pv[e] = idx[e] >= 0 ? (double)pe : 0.0;
   }
   const double beta = sqrt(reduce_partials<kNB>(pa1, na, sh));  // beta_{k+1}
   const double alpha = st->alpha, t1 = st->t1, t2 = st->t2, t3 = st->t3;
@@ -525,9 +521,18 @@ void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st, hipE
                        wk.P.p, wk.pa1.p, wk.pb1.p, wk.pb2.p, (int)ceil_div(s.n, kNB), wk.state.p + ((k - 1) & 1),
                        wk.state.p + (k & 1));
 }
+// iteration 0 gathers the initial residual's FP64 partials; later ones the
+// cell kernel's partials in its block precision's type
 template <int W>
-void launch_B(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
-  hipLaunchKernelGGL((lsqr_B<W>), dim3(ceil_div(s.n, kNB)), dim3(kNB), 0, st, s.n, s.gtab.p, wk.P.p, wk.v.p,
+void launch_B(BlockSys &s, LsqrWork &wk, int k, int prec, hipStream_t st) {
+  if (k >= 1 && prec == kBlocks16) {
+    hipLaunchKernelGGL((lsqr_B<W, float>), dim3(ceil_div(s.n, kNB)), dim3(kNB), 0, st, s.n, s.gtab.p,
+                       (const float *)wk.P.p, wk.v.p, wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, wk.na, wk.pb1.p, wk.pb2.p,
+                       wk.state.p + (k & 1));
+    return;
+  }
+  hipLaunchKernelGGL((lsqr_B<W, double>), dim3(ceil_div(s.n, kNB)), dim3(kNB), 0, st, s.n, s.gtab.p,
+                     (const double *)wk.P.p, wk.v.p,
                      wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, k == 0 ? s.ncell : wk.na, wk.pb1.p, wk.pb2.p,
                      wk.state.p + (k & 1));
 }
@@ -555,12 +560,12 @@ void stepA(BlockSys &s, LsqrWork &wk, int k, int prec, hipStream_t st, hipEvent_
       launch_A<double, 27, 27>(s, wk, k, (const double *)s.A.p, st, e0, e1);
   }
 }
-void stepB(BlockSys &s, LsqrWork &wk, int k, hipStream_t st) {
+void stepB(BlockSys &s, LsqrWork &wk, int k, int prec, hipStream_t st) {
   switch (s.gw) {
-    case 4: launch_B<4>(s, wk, k, st); break;
-    case 8: launch_B<8>(s, wk, k, st); break;
-    case 12: launch_B<12>(s, wk, k, st); break;
-    default: launch_B<16>(s, wk, k, st); break;
+    case 4: launch_B<4>(s, wk, k, prec, st); break;
+    case 8: launch_B<8>(s, wk, k, prec, st); break;
+    case 12: launch_B<12>(s, wk, k, prec, st); break;
+    default: launch_B<16>(s, wk, k, prec, st); break;
   }
 }
 
@@ -605,7 +610,7 @@ hipGraphExec_t capture_chunk(BlockSys &s, LsqrWork &wk, int prec, int iters, hip
   TT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int i = 1; i <= iters; ++i) {
     stepA(s, wk, i, prec, st, nullptr, nullptr);
-    stepB(s, wk, i, st);
+    stepB(s, wk, i, prec, st);
   }
   const hipError_t le = hipGetLastError();
   const hipError_t ee = hipStreamEndCapture(st, &g);
@@ -708,7 +713,7 @@ void block_make16(BlockSys &s, hipStream_t st) {
   const int S16 = block_stride16(s.R, s.L);
   const size_t total16 = (size_t)s.ncell * S16;
   s.A16.alloc(total16);
-  hipLaunchKernelGGL(to16_kernel, dim3(ceil_div(total16, 256)), dim3(256), 0, st, total16, s.L, block_row16(s.L),
+  hipLaunchKernelGGL(to16_kernel, dim3(ceil_div(total16, 256)), dim3(256), 0, st, total16, s.R, s.L, block_row16(s.L),
                      s.stride, S16, s.A.p, s.A16.p);
   TT_LAUNCH_CHECK();
   s.has16 = true;
@@ -809,7 +814,7 @@ void block_tune(BlockSys &s, int prec, hipStream_t st) {
   TT_HIP(hipMemcpyAsync(tw.state.p, init2, sizeof(init2), hipMemcpyHostToDevice, st));
   TT_HIP(hipMemsetAsync(tw.x.p, 0, sizeof(double) * n, st));
   r0(s, tw, b.p, x.p, st);
-  stepB(s, tw, 0, st);
+  stepB(s, tw, 0, prec, st);
   constexpr int kIts = 6, kReps = 2;
   std::vector<hipEvent_t> ev(2 * cand.size() * kReps);
   for (auto &e : ev) TT_HIP(hipEventCreate(&e));
@@ -820,7 +825,7 @@ void block_tune(BlockSys &s, int prec, hipStream_t st) {
       TT_HIP(hipEventRecord(ev[2 * (rep * cand.size() + ci)], st));
       for (int i = 0; i < kIts; ++i, ++k) {
         stepA(s, tw, k, prec, st, nullptr, nullptr);
-        stepB(s, tw, k, st);
+        stepB(s, tw, k, prec, st);
       }
       TT_HIP(hipEventRecord(ev[2 * (rep * cand.size() + ci) + 1], st));
     }
@@ -864,7 +869,7 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
   TT_HIP(hipMemsetAsync(wk.x.p, 0, sizeof(double) * n, st));
   r0(s, wk, b, x, st);
   TT_LAUNCH_CHECK();
-  stepB(s, wk, 0, st);
+  stepB(s, wk, 0, prec, st);
   TT_LAUNCH_CHECK();
   // recent solves of this slot predict the count (the Picard loop changes
   // only the right-hand side and, through the dynamic tolerance, eps)
@@ -884,7 +889,7 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
   // one iteration on the stream (sampled launches record their own events)
   auto one = [&](long pi) {
     stepA(s, wk, k, prec, st, pi >= 0 ? probe->pool[2 * pi] : nullptr, pi >= 0 ? probe->pool[2 * pi + 1] : nullptr);
-    stepB(s, wk, k, st);
+    stepB(s, wk, k, prec, st);
     ++k;
   };
   // replays `pending` (even) iterations from k as chunks of 64, 32, .., 2
