@@ -102,13 +102,13 @@ struct RegBlock {
   V r[PER];
   __device__ __forceinline__ void load(const T *__restrict__ src) {
     const V *s = reinterpret_cast<const V *>(src);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x & 63;  // lane: one wavefront moves one block
 #pragma unroll
     for (int i = 0; i < PER; ++i) r[i] = (t + 64 * i < NV) ? s[t + 64 * i] : V{};
   }
   __device__ __forceinline__ void store(T *blk) const {
     V *d = reinterpret_cast<V *>(blk);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x & 63;
 #pragma unroll
     for (int i = 0; i < PER; ++i)
       if (t + 64 * i < NV) d[t + 64 * i] = r[i];
@@ -167,7 +167,15 @@ __global__ void to16_kernel(size_t total16, int R, int L, int LP, int S, int S16
 constexpr int kNB = 256;  // threads per block of the unknown kernel
 // cells per partial sum of the cell kernel: at most 4096 partials per launch,
 // a fixed function of the system so that the sums do not depend on the grid
-inline int cell_group(int ncell) { return std::max(1, (ncell + 4095) / 4096); }
+// (FP64 blocks, one wavefront per workgroup); with nw wavefronts per workgroup
+// (FP32 and FP16 blocks) a multiple of nw cells and at most 1024 partials
+constexpr int kNW = 4;  // wavefronts per workgroup of the reduced-precision cell kernel
+inline int cell_group(int ncell, int nw) {
+  if (nw == 1) return std::max(1, (ncell + 4095) / 4096);
+  const int per = std::max(1, (ncell + 1023) / 1024);
+  return (per + nw - 1) / nw * nw;
+}
+inline int cell_waves(int prec) { return prec == kBlocks64 ? 1 : kNW; }
 
 // ---------------------------------------------------------------------------
 // r0 = b - A (x0/q), P = A^T r0; per-cell partials of ||r0||^2 and ||b||^2
@@ -387,6 +395,188 @@ __global__ void __launch_bounds__(64) lsqr_A(int ncell, int cgroup, const T *__r
   }
 }
 
+// orders a wavefront's LDS stores before its own later LDS loads by other
+// lanes (a wavefront's LDS operations complete in order; this keeps the
+// compiler from moving them across)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The same cell work on reduced-precision blocks with NW wavefronts per
+// workgroup.  A workgroup takes a contiguous range of whole groups of cgroup
+// cells (a multiple of NW); wavefront w takes cells cb + w, cb + w + NW, ...
+// of each group, prefetching its next cell as above.  A group's ||u'||^2
+// partial is the wavefronts' sums added in wavefront order, so it depends on
+// neither the grid nor the timing; groups are NW times larger than with one
+// wavefront per workgroup, so the unknown kernel reduces NW times fewer
+// partials, and alpha is reduced once per workgroup.  LDS staging is per
+// wavefront and synchronised within it; the workgroup barrier sits at group
+// ends, which every wavefront reaches.
+template <typename T, int R, int L, int NW>
+__global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(int ncell, int cgroup, const T *__restrict__ A,
+                                                   const int *__restrict__ colmap,
+                                                   const double *__restrict__ vp, double *__restrict__ u,
+                                                   double *__restrict__ P, double *__restrict__ pa1,
+                                                   const double *__restrict__ pb1, const double *__restrict__ pb2,
+                                                   int nb, const LsqrState *__restrict__ sin,
+                                                   LsqrState *__restrict__ sout) {
+  static_assert(sizeof(T) <= 4, "multi-wavefront cell kernel: FP32 or FP16 blocks");
+  constexpr int S = block_len<T, R, L>();
+  __shared__ __align__(16) T blk_w[NW][S];
+  __shared__ __align__(16) float vsf_w[NW][64], uuf_w[NW][64];
+  __shared__ double sh[NW], gp[NW];
+  // the wavefront index, known uniform (so cell indices and their branches
+  // stay scalar)
+  const int t = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  T *blk = blk_w[w];
+  float *vsf = vsf_w[w], *uuf = uuf_w[w];
+  const int k = sin->kseq + 1;
+  if (blockIdx.x == 0) {
+    const double nvs = reduce_partials<64 * NW>(pb1, nb, sh);
+    const double dks = k >= 3 ? reduce_partials<64 * NW>(pb2, nb, sh) : 0.0;
+    if (threadIdx.x == 0) {
+      LsqrState o;
+      scalar_update(k, *sin, nvs, dks, o);
+      o.kseq = k;
+      *sout = o;
+    }
+    return;
+  }
+  if (sin->done) return;
+  const int G = gridDim.x - 1, lb = xcd_logical(blockIdx.x, 1, G + 1);
+  const int ngroup = (ncell + cgroup - 1) / cgroup;
+  const int g0 = (int)((long long)lb * ngroup / G), g1 = (int)((long long)(lb + 1) * ngroup / G);
+  if (g0 >= g1) return;
+  auto gend = [&](int g) { return min(ncell, (g + 1) * cgroup); };
+  // this wavefront's cell after c, or -1 (only the last group can be short)
+  auto next_cell = [&](int c) -> int {
+    const int g = c / cgroup;
+    if (c + NW < gend(g)) return c + NW;
+    if (g + 1 >= g1) return -1;
+    const int cn = (g + 1) * cgroup + w;
+    return cn < gend(g + 1) ? cn : -1;
+  };
+  int c = g0 * cgroup + w;
+  if (c >= gend(g0)) c = -1;
+  int cn = c >= 0 ? next_cell(c) : -1;
+  int jn = -1;
+  double uo = 0.0, vj = 0.0;
+  RegBlock<T, S> rb;
+  if (c >= 0) {
+    const int j = t < L ? colmap[(size_t)c * L + t] : -1;
+    jn = (cn >= 0 && t < L) ? colmap[(size_t)cn * L + t] : -1;
+    uo = t < R ? u[(size_t)c * R + t] : 0.0;
+    rb.load(A + (size_t)c * S);
+    vj = j >= 0 ? vp[j] : 0.0;
+  }
+  const double beta = sin->beta, alpha_prev = sin->alpha;
+  const double nvs = reduce_partials<64 * NW>(pb1, nb, sh);
+  const double alpha = (k == 1 || beta > 0.0) ? sqrt(nvs) : alpha_prev;
+  for (int g = g0; g < g1; ++g) {
+    double acc = 0.0;
+    while (c >= 0 && c < gend(g)) {
+      rb.store(blk);
+      vsf[t] = (float)(t < L ? (alpha > 0.0 ? vj / alpha : vj) : 0.0);
+      wave_sync();
+      const int cnn = cn >= 0 ? next_cell(cn) : -1;
+      double vjn = 0.0, uon = 0.0;
+      int jnn = -1;
+      if (cn >= 0) {
+        vjn = jn >= 0 ? vp[jn] : 0.0;
+        uon = t < R ? u[(size_t)cn * R + t] : 0.0;
+        rb.load(A + (size_t)cn * S);
+        if (cnn >= 0 && t < L) jnn = colmap[(size_t)cnn * L + t];
+      }
+      double un = 0.0;
+      {
+        double s = 0.0;
+        const int row = t < R ? t : R - 1;
+        if constexpr (sizeof(T) == 2) {
+          constexpr int LP = row_len<T, L>();
+          const uint2 *rp = reinterpret_cast<const uint2 *>(blk + row * LP);
+          float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+          for (int q = 0; q < LP / 4; ++q) {
+            const uint2 cw = rp[q];
+            const float4 va = *reinterpret_cast<const float4 *>(vsf + 4 * q);
+            const _Float16 *h = reinterpret_cast<const _Float16 *>(&cw);
+            f0 += (float)h[0] * va.x;
+            f1 += (float)h[1] * va.y;
+            f0 += (float)h[2] * va.z;
+            f1 += (float)h[3] * va.w;
+          }
+          s = (double)f0 + (double)f1;
+        } else {
+          float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+          for (int l = 0; l + 1 < L; l += 2) {
+            f0 += (float)blk[row * L + l] * vsf[l];
+            f1 += (float)blk[row * L + l + 1] * vsf[l + 1];
+          }
+          if (L & 1) f0 += (float)blk[row * L + L - 1] * vsf[L - 1];
+          s = (double)f0 + (double)f1;
+        }
+        const double uold = beta > 0.0 ? uo / beta : uo;
+        if (t < R) {
+          un = -alpha * uold + s;
+          u[(size_t)c * R + t] = un;
+        }
+      }
+      uuf[t] = (float)un;
+      wave_sync();
+      if constexpr (sizeof(T) == 2) {
+        constexpr int LP = row_len<T, L>(), NPAIR = (L + 1) / 2, RH = (R + 1) / 2;
+        const int p = t & 31, hf = t >> 5, i0 = hf ? RH : 0;
+        const int pc = p < NPAIR ? p : 0;
+        float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+        for (int q = 0; q < RH; ++q) {
+          const int i = i0 + q;
+          if (i < R) {
+            const unsigned int pw = *reinterpret_cast<const unsigned int *>(blk + i * LP + 2 * pc);
+            const _Float16 *h = reinterpret_cast<const _Float16 *>(&pw);
+            const float ui = uuf[i];
+            s0 += (float)h[0] * ui;
+            s1 += (float)h[1] * ui;
+          }
+        }
+        s0 += __shfl_xor(s0, 32, 64);
+        s1 += __shfl_xor(s1, 32, 64);
+        if (hf == 0 && p < NPAIR) {
+          float *Pf = reinterpret_cast<float *>(P);
+          Pf[(size_t)c * L + 2 * p] = s0;
+          if (2 * p + 1 < L) Pf[(size_t)c * L + 2 * p + 1] = s1;
+        }
+      } else {
+        const int col = t < L ? t : L - 1;
+        float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+        for (int i = 0; i + 1 < R; i += 2) {
+          f0 += (float)blk[i * L + col] * uuf[i];
+          f1 += (float)blk[(i + 1) * L + col] * uuf[i + 1];
+        }
+        if (R & 1) f0 += (float)blk[(R - 1) * L + col] * uuf[R - 1];
+        if (t < L) P[(size_t)c * L + t] = (double)f0 + (double)f1;
+      }
+      acc += un * un;
+      c = cn, cn = cnn, vj = vjn, jn = jnn, uo = uon;
+      wave_sync();  // blk, vsf and uuf are rewritten next
+    }
+    const double ws = wave_sum(acc);
+    if (t == 0) gp[w] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sg = gp[0];
+#pragma unroll
+      for (int i = 1; i < NW; ++i) sg += gp[i];
+      pa1[g] = sg;
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // iteration k, unknown part (k = 0: initialisation after r0)
 template <int W, typename PT>
@@ -482,17 +672,30 @@ void launch_r0(BlockSys &s, LsqrWork &wk, const double *b, const double *x, hipS
   hipLaunchKernelGGL((lsqr_r0<R, L>), dim3(s.ncell), dim3(64), 0, st, s.A.p, s.colmap.p, s.q.p, x, b, wk.u.p, wk.P.p,
                      wk.pa1.p, wk.pa2.p);
 }
-// resident blocks of a cell-kernel instantiation on the whole device
+// the cell kernel of a block type: one wavefront per workgroup on FP64
+// blocks (their LDS staging limits residency), kNW on FP32/FP16 blocks
 template <typename T, int R, int L>
-int persistent_grid(int ncell) {
+const void *cell_kernel() {
+  if constexpr (sizeof(T) == 8)
+    return reinterpret_cast<const void *>(lsqr_A<T, R, L>);
+  else
+    return reinterpret_cast<const void *>(lsqr_Aw<T, R, L, kNW>);
+}
+template <typename T>
+constexpr int cell_threads() {
+  return sizeof(T) == 8 ? 64 : 64 * kNW;
+}
+// resident workgroups of a cell-kernel instantiation on the whole device
+template <typename T, int R, int L>
+int persistent_grid(int ngroup) {
   static int per_cu = -1, cus = 0;
   if (per_cu < 0) {
     int dev = 0;
     TT_HIP(hipGetDevice(&dev));
     TT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lsqr_A<T, R, L>, 64, 0));
+    TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cell_kernel<T, R, L>(), cell_threads<T>(), 0));
     hipFuncAttributes fa;
-    TT_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(lsqr_A<T, R, L>)));
+    TT_HIP(hipFuncGetAttributes(&fa, cell_kernel<T, R, L>()));
     int lds_per_cu = 0;
     TT_HIP(hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     // static LDS rounded to the 512-byte allocation granule
@@ -500,26 +703,36 @@ int persistent_grid(int ncell) {
     if (lds > 0 && lds_per_cu > 0) per_cu = std::min<int>(per_cu, (int)(lds_per_cu / lds));
     if (per_cu < 1) per_cu = 1;
   }
-  return std::min(ncell, per_cu * cus);
+  return std::min(ngroup, per_cu * cus);
 }
 // e0/e1 non-null: the dispatch itself records its start and end (sampled timing)
 template <typename T, int R, int L>
 void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
-  const int ngroup = ceil_div(s.ncell, cell_group(s.ncell));
+  constexpr int NW = sizeof(T) == 8 ? 1 : kNW;
+  const int cg = cell_group(s.ncell, NW);
+  const int ngroup = ceil_div(s.ncell, cg);
   const int tuned = s.grid_a[sizeof(T) == 8 ? 0 : sizeof(T) == 4 ? 1 : 2];
-  const int g = std::min(ngroup, tuned > 0 ? tuned : persistent_grid<T, R, L>(s.ncell));
+  const int g = std::min(ngroup, tuned > 0 ? tuned : persistent_grid<T, R, L>(ngroup));
   wk.na = ngroup;
-  if (e0)
-    hipExtLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, e0, e1, 0, s.ncell,
-                          cell_group(s.ncell), A, s.colmap.p,
-                          (const double *)wk.v.p, wk.u.p, wk.P.p, wk.pa1.p, (const double *)wk.pb1.p,
-                          (const double *)wk.pb2.p, (int)ceil_div(s.n, kNB),
-                          (const LsqrState *)(wk.state.p + ((k - 1) & 1)), wk.state.p + (k & 1));
-  else
-    hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, s.ncell, cell_group(s.ncell), A,
-                       s.colmap.p, wk.v.p, wk.u.p,
-                       wk.P.p, wk.pa1.p, wk.pb1.p, wk.pb2.p, (int)ceil_div(s.n, kNB), wk.state.p + ((k - 1) & 1),
-                       wk.state.p + (k & 1));
+  const double *vp = wk.v.p, *pb1 = wk.pb1.p, *pb2 = wk.pb2.p;
+  const LsqrState *sin = wk.state.p + ((k - 1) & 1);
+  LsqrState *sout = wk.state.p + (k & 1);
+  const int nb = (int)ceil_div(s.n, kNB);
+  if constexpr (NW == 1) {
+    if (e0)
+      hipExtLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, e0, e1, 0, s.ncell, cg, A, s.colmap.p,
+                            vp, wk.u.p, wk.P.p, wk.pa1.p, pb1, pb2, nb, sin, sout);
+    else
+      hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, s.ncell, cg, A, s.colmap.p, vp, wk.u.p,
+                         wk.P.p, wk.pa1.p, pb1, pb2, nb, sin, sout);
+  } else {
+    if (e0)
+      hipExtLaunchKernelGGL((lsqr_Aw<T, R, L, NW>), dim3(g + 1), dim3(64 * NW), 0, st, e0, e1, 0, s.ncell, cg, A,
+                            s.colmap.p, vp, wk.u.p, wk.P.p, wk.pa1.p, pb1, pb2, nb, sin, sout);
+    else
+      hipLaunchKernelGGL((lsqr_Aw<T, R, L, NW>), dim3(g + 1), dim3(64 * NW), 0, st, s.ncell, cg, A, s.colmap.p, vp,
+                         wk.u.p, wk.P.p, wk.pa1.p, pb1, pb2, nb, sin, sout);
+  }
 }
 // iteration 0 gathers the initial residual's FP64 partials; later ones the
 // cell kernel's partials in its block precision's type
@@ -785,16 +998,16 @@ void block_tune(BlockSys &s, int prec, hipStream_t st) {
       return;
     }
   }
-  const int m = s.ncell * s.R, n = s.n, ngroup = ceil_div(s.ncell, cell_group(s.ncell));
+  const int m = s.ncell * s.R, n = s.n, ngroup = ceil_div(s.ncell, cell_group(s.ncell, cell_waves(prec)));
   int resident = 0;
   if (s.R == 51)
-    resident = prec == kBlocks16 ? persistent_grid<_Float16, 51, 51>(s.ncell)
-               : prec == kBlocks32 ? persistent_grid<float, 51, 51>(s.ncell)
-                                   : persistent_grid<double, 51, 51>(s.ncell);
+    resident = prec == kBlocks16 ? persistent_grid<_Float16, 51, 51>(ngroup)
+               : prec == kBlocks32 ? persistent_grid<float, 51, 51>(ngroup)
+                                   : persistent_grid<double, 51, 51>(ngroup);
   else
-    resident = prec == kBlocks16 ? persistent_grid<_Float16, 27, 27>(s.ncell)
-               : prec == kBlocks32 ? persistent_grid<float, 27, 27>(s.ncell)
-                                   : persistent_grid<double, 27, 27>(s.ncell);
+    resident = prec == kBlocks16 ? persistent_grid<_Float16, 27, 27>(ngroup)
+               : prec == kBlocks32 ? persistent_grid<float, 27, 27>(ngroup)
+                                   : persistent_grid<double, 27, 27>(ngroup);
   std::vector<int> cand;
   for (int d = 1; d <= 8; d *= 2) {
     const int g = std::min(ngroup, std::max(1, resident / d));
