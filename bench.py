@@ -423,7 +423,7 @@ def main():
 
     # roofline of the dominant kernel: lsqr_A on the 51-row systems (VTE x, y, z and T)
     half = args.mixed == "fp16"
-    kname = "lsqr_A<_Float16,51,51>" if half else "lsqr_A<float,51,51>"
+    kname = "lsqr_Aw<_Float16,51,51,4>" if half else "lsqr_Aw<float,51,51,4>"
     vbytes = 2 if half else 4
     infos = [case.system_info(s) for s in range(4)]
     tuned = {"vorticity_energy": {k: infos[0][k] for k in ("grid_fp64", "grid_fp32", "grid_fp16")},
