@@ -207,6 +207,20 @@ MODULE tritok_amd_bind
       INTEGER(C_INT32_T), VALUE :: on
       INTEGER(C_INT) :: tt_case_set_graphs
     END FUNCTION
+    FUNCTION tt_case_probe(h, stride) BIND(C, NAME="tt_case_probe")
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: stride
+      INTEGER(C_INT) :: tt_case_probe
+    END FUNCTION
+    FUNCTION tt_case_probe_stats(h, rows, avg_ms, sampled, launches) BIND(C, NAME="tt_case_probe_stats")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: rows
+      REAL(C_DOUBLE) :: avg_ms
+      INTEGER(C_INT64_T) :: sampled, launches
+      INTEGER(C_INT) :: tt_case_probe_stats
+    END FUNCTION
     FUNCTION tt_case_set_anchor_period(h, period) BIND(C, NAME="tt_case_set_anchor_period")
       IMPORT :: C_INT, C_INT32_T, C_PTR
       TYPE(C_PTR), VALUE :: h
