@@ -421,34 +421,40 @@ struct SdOut {
   double *H, *Htx, *Hty, *Htz, *Dx, *Dy, *Dz;
 };
 
-__global__ void __launch_bounds__(64) sd_bnd_kernel(MeshPtrs m, const QuadDev *__restrict__ qp, int row0, int nrows,
-                                                    SdOut o) {
-  const int r = blockIdx.x * 64 + threadIdx.x;
-  const bool active = r < nrows;
-  const int i = active ? row0 + r : row0;
-  const int dn = m.gbn[i];
+// Boundary integrals of the single-domain rows (H and the Ht cross products,
+// the boundary part of FMATsdkm, lib_sdkm.f:1286-1530), spread over (row,
+// face) pairs: one launch per colour of boundary faces that share no node,
+// one thread per pair (rows fastest, so the column-major output is written
+// coalesced).  Within a launch
+// no two threads add into the same (row, node) entry; the colours run in a
+// fixed order, so the sums are deterministic (they add a row's faces in colour
+// order rather than face order: a rounding-level difference).
+__global__ void __launch_bounds__(256) sd_bnd_pairs_kernel(MeshPtrs m, const QuadDev *__restrict__ qp, int row0,
+                                                           int nrows, const int *__restrict__ faces, int nf,
+                                                           SdOut o) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (size_t)nrows * nf) return;
+  const int r = (int)(t % nrows), je = faces[t / nrows];
+  const int dn = m.gbn[row0 + r];
   const double p[3] = {m.x[dn - 1], m.x[dn - 1 + m.nn], m.x[dn - 1 + 2 * (size_t)m.nn]};
   const size_t ne = m.ne, ld = o.ld;
-  for (int je = 0; je < m.ne; ++je) {
-    double c[4][3];
-    int nodes[9];
-    for (int k = 0; k < 9; ++k) nodes[k] = m.ibc[je + ne * k];
-    for (int k = 0; k < 4; ++k)
-      for (int d = 0; d < 3; ++d) c[k][d] = m.x[nodes[2 * k] - 1 + (size_t)m.nn * d];
-    int isrc = 0;
-    for (int s = 0; s < 9; ++s)
-      if (nodes[s] == dn) isrc = s + 1;
-    BndAcc<false> acc;
-    acc.zero();
-    integrate_face<false>(*qp, p, c, isrc, 0.0, acc);
-    if (!active) continue;
-    for (int s = 0; s < 9; ++s) {
-      size_t j = m.lbn[nodes[s] - 1] - 1;
-      o.H[r + ld * j] += acc.he[s];
-      o.Htx[r + ld * j] -= acc.heyz[s];
-      o.Hty[r + ld * j] -= acc.hezx[s];
-      o.Htz[r + ld * j] -= acc.hexy[s];
-    }
+  double c[4][3];
+  int nodes[9];
+  for (int k = 0; k < 9; ++k) nodes[k] = m.ibc[je + ne * k];
+  for (int k = 0; k < 4; ++k)
+    for (int d = 0; d < 3; ++d) c[k][d] = m.x[nodes[2 * k] - 1 + (size_t)m.nn * d];
+  int isrc = 0;
+  for (int q = 0; q < 9; ++q)
+    if (nodes[q] == dn) isrc = q + 1;
+  BndAcc<false> acc;
+  acc.zero();
+  integrate_face<false>(*qp, p, c, isrc, 0.0, acc);
+  for (int q = 0; q < 9; ++q) {
+    const size_t j = m.lbn[nodes[q] - 1] - 1;
+    o.H[r + ld * j] += acc.he[q];
+    o.Htx[r + ld * j] -= acc.heyz[q];
+    o.Hty[r + ld * j] -= acc.hezx[q];
+    o.Htz[r + ld * j] -= acc.hexy[q];
   }
 }
 
@@ -722,8 +728,39 @@ void sd_integrals_dev(const DevMesh &m, const tt_mesh_view &hv, const QuadDev &q
   DBuf<QuadDev> dq(1);
   dq.upload(&qh, 1, s);
   SdOut out{nr, o.H.p, o.Htx.p, o.Hty.p, o.Htz.p, o.Dx.p, o.Dy.p, o.Dz.p};
-  hipLaunchKernelGGL(sd_bnd_kernel, dim3(ceil_div(nr, 64)), dim3(64), 0, s, ptrs(m), dq.p, row0, nr, out);
-  TT_LAUNCH_CHECK();
+  // boundary faces: colour them so that no two of one colour share a node
+  // (greedy, first free colour), then one launch per colour over (row, face)
+  {
+    const int ne = m.ne;
+    std::vector<uint64_t> used(m.nn, 0);
+    std::vector<int> colour(ne);
+    int ncol = 0;
+    for (int je = 0; je < ne; ++je) {
+      uint64_t mask = 0;
+      for (int k = 0; k < 9; ++k) mask |= used[hv.ibc[je + (size_t)ne * k] - 1];
+      if (~mask == 0) throw std::runtime_error("face colouring needs more than 64 colours");
+      const int col = __builtin_ctzll(~mask);
+      for (int k = 0; k < 9; ++k) used[hv.ibc[je + (size_t)ne * k] - 1] |= 1ull << col;
+      colour[je] = col, ncol = std::max(ncol, col + 1);
+    }
+    std::vector<int> fstart(ncol + 1, 0), flist(ne);
+    for (int je = 0; je < ne; ++je) ++fstart[colour[je] + 1];
+    for (int k = 0; k < ncol; ++k) fstart[k + 1] += fstart[k];
+    {
+      std::vector<int> fill(fstart.begin(), fstart.end() - 1);
+      for (int je = 0; je < ne; ++je) flist[fill[colour[je]]++] = je;
+    }
+    DBuf<int> dfaces;
+    dfaces.upload(flist, s);
+    for (int k = 0; k < ncol; ++k) {
+      const int nf = fstart[k + 1] - fstart[k];
+      if (nf == 0) continue;
+      hipLaunchKernelGGL(sd_bnd_pairs_kernel, dim3(ceil_div((size_t)nr * nf, 256)), dim3(256), 0, s, ptrs(m), dq.p,
+                         row0, nr, dfaces.p + fstart[k], nf, out);
+    }
+    TT_LAUNCH_CHECK();
+    TT_HIP(hipStreamSynchronize(s));  // dfaces is released on return
+  }
   // regular cells: colour the cells so that no two of one colour share a node
   // (greedy, first free colour), then one product launch per colour
   {
