@@ -613,20 +613,23 @@ __global__ void __launch_bounds__(64) sd_dom_singular_kernel(MeshPtrs m, const Q
                                                              const int *__restrict__ prow, const int *__restrict__ pcell,
                                                              const int *__restrict__ psrc, int npairs,
                                                              double *__restrict__ staging) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
+  // one wavefront per (row, cell) pair: the pair's quadrature points are
+  // evaluated 64 at a time (WaveCell, as for the macro domain integrals)
+  __shared__ double lds[wave_cell_lds<3>()];
+  const int k = blockIdx.x;
   if (k >= npairs) return;
   const size_t nc = m.nc;
-  int i = prow[k], ic = pcell[k];
-  int dn = m.gbn[i];
+  const int i = prow[k], ic = pcell[k];
+  const int dn = m.gbn[i];
   const double p[3] = {m.x[dn - 1], m.x[dn - 1 + m.nn], m.x[dn - 1 + 2 * (size_t)m.nn]};
   double c[8][3];
   for (int t = 0; t < 8; ++t)
     for (int d = 0; d < 3; ++d) c[t][d] = m.x[m.idc[ic + nc * (kGeo8[t] - 1)] - 1 + (size_t)m.nn * d];
-  DomAcc<3> acc;
-  acc.zero();
-  integrate_cell(*qp, p, c, psrc[k], acc);
-  for (int t = 0; t < 3; ++t)
-    for (int s = 0; s < 27; ++s) staging[(size_t)k * 81 + t * 27 + s] = acc.a[t][s];
+  WaveCell<3, false> w;
+  w.run(*qp, p, c, psrc[k], 0.0, lds);
+  const int lane = threadIdx.x;
+  if (lane < 27)
+    for (int t = 0; t < 3; ++t) staging[(size_t)k * 81 + t * 27 + lane] = w.acc[t];
 }
 
 // adds each row's singular pairs (cell order) into D
@@ -839,7 +842,7 @@ void sd_integrals_dev(const DevMesh &m, const tt_mesh_view &hv, const QuadDev &q
   dprow.upload(prow, s), dpcell.upload(pcell, s), dpsrc.upload(psrc, s), doff.upload(off, s);
   DBuf<double> staging((size_t)np * 81);
   if (np) {
-    hipLaunchKernelGGL(sd_dom_singular_kernel, dim3(ceil_div(np, 64)), dim3(64), 0, s, ptrs(m), dq.p, dprow.p,
+    hipLaunchKernelGGL(sd_dom_singular_kernel, dim3(np), dim3(64), 0, s, ptrs(m), dq.p, dprow.p,
                        dpcell.p, dpsrc.p, np, staging.p);
     TT_LAUNCH_CHECK();
     hipLaunchKernelGGL(sd_dom_singular_apply, dim3(ceil_div(nr, 64)), dim3(64), 0, s, ptrs(m), row0, nr, doff.p,
