@@ -368,6 +368,8 @@ def main():
                     help="n: mixed-precision kinematics solve on an n^3 cube (0: skip)")
     ap.add_argument("--assembly-only", type=int, default=0,
                     help="n: only the macro and single-domain integrals of an n^3 cube, then exit (counter runs)")
+    ap.add_argument("--mixed-kin-only", action="store_true",
+                    help="only the mixed-precision kinematics leg (--mixed-kin n), then exit (profiler runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="MiB of a device-to-device copy issued first (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -378,6 +380,10 @@ def main():
 
     if args.assembly_only:
         assembly_only(args.assembly_only)
+        return
+    if args.mixed_kin_only:
+        mk = mixed_kinematics_leg(args.mixed_kin)
+        log("mixed kinematics only: " + json.dumps({p: mk[p] for p in ("fp64", "fp32", "bf16", "fp16")}))
         return
 
     from tritok_amd import workloads
