@@ -552,25 +552,31 @@ void dense_setup(DenseKin &d, const DevMesh &m, const int *kode_host, SdPrec pre
   rocblas_handle h;
   if (rocblas_create_handle(&h) != rocblas_status_success) throw std::runtime_error("rocblas_create_handle failed");
   rocblas_set_stream(h, s);
-  DBuf<int> ipiv(nb), info(1);
+  // the three equations' systems factorised and inverted together (one
+  // batched call each: every panel step launches once for all three)
+  DBuf<int> ipiv((size_t)3 * nb), info(3);
+  DBuf<double *> aptr(3);
   hipEvent_t e0, e1;
   TT_HIP(hipEventCreate(&e0));
   TT_HIP(hipEventCreate(&e1));
   TT_HIP(hipEventRecord(e0, s));
+  double *ap[3];
   for (int e = 0; e < 3; ++e) {
     d.Ainv[e].alloc((size_t)nb * nb);
     hipLaunchKernelGGL(build_sys_kernel, dim3(ceil_div((size_t)nb * nb, 256)), dim3(256), 0, s, nb, e, d.kode.p,
                        m.nx.p, m.ny.p, m.nz.p, m.gbn.p, d.mats.Dx.p, d.mats.Dy.p, d.mats.Dz.p, d.Ainv[e].p);
     TT_LAUNCH_CHECK();
-    if (rocsolver_dgetrf(h, nb, nb, d.Ainv[e].p, nb, ipiv.p, info.p) != rocblas_status_success)
-      throw std::runtime_error("rocsolver_dgetrf failed");
-    int inf = 0;
-    info.download(&inf, 1, s);
-    TT_HIP(hipStreamSynchronize(s));
-    if (inf != 0) throw std::runtime_error("single-domain system matrix is singular");
-    if (rocsolver_dgetri(h, nb, d.Ainv[e].p, nb, ipiv.p, info.p) != rocblas_status_success)
-      throw std::runtime_error("rocsolver_dgetri failed");
+    ap[e] = d.Ainv[e].p;
   }
+  aptr.upload(ap, 3, s);
+  if (rocsolver_dgetrf_batched(h, nb, nb, aptr.p, nb, ipiv.p, nb, info.p, 3) != rocblas_status_success)
+    throw std::runtime_error("rocsolver_dgetrf_batched failed");
+  int inf[3] = {0, 0, 0};
+  info.download(inf, 3, s);
+  TT_HIP(hipStreamSynchronize(s));
+  if (inf[0] != 0 || inf[1] != 0 || inf[2] != 0) throw std::runtime_error("single-domain system matrix is singular");
+  if (rocsolver_dgetri_batched(h, nb, aptr.p, nb, ipiv.p, nb, info.p, 3) != rocblas_status_success)
+    throw std::runtime_error("rocsolver_dgetri_batched failed");
   TT_HIP(hipEventRecord(e1, s));
   TT_HIP(hipEventSynchronize(e1));
   float ms = 0;
