@@ -70,18 +70,26 @@ __global__ void scatter_kernel(int n, int nn, double ur, const int *__restrict__
     fflux[f - nn] = x[u];
 }
 
-// known-column part of one cell row: sum over known columns of (-Afull)*value
+// Known-column part of one cell row: the sum over the known columns of
+// (-Afull) * value. The cell's known values are staged in LDS by its
+// workgroup (thread l < L stages column l, with its known flag); every row of
+// the cell reads them from there.
+__device__ __forceinline__ void stage_known(int c, int L, const int *__restrict__ colknown,
+                                            const int *__restrict__ colfield, const double *__restrict__ fnode,
+                                            const double *__restrict__ fflux, double *kval, int *kflag) {
+  const int l = threadIdx.x;
+  if (l < L) {
+    const int kn = colknown[(size_t)c * L + l];
+    kflag[l] = kn;
+    kval[l] = kn ? (l < 27 ? fnode[colfield[(size_t)c * L + l]] : fflux[colfield[(size_t)c * L + l]]) : 0.0;
+  }
+}
 __device__ __forceinline__ double known_part(int c, int r, int R, int L, const double *__restrict__ Af,
-                                             const int *__restrict__ colknown, const int *__restrict__ colfield,
-                                             const double *__restrict__ fnode, const double *__restrict__ fflux) {
+                                             const double *kval, const int *kflag) {
   const double *a = Af + ((size_t)c * R + r) * L;
   double s = 0.0;
-  for (int l = 0; l < L; ++l) {
-    if (!colknown[(size_t)c * L + l]) continue;
-    int f = colfield[(size_t)c * L + l];
-    double val = l < 27 ? fnode[f] : fflux[f];
-    s = s + (-a[l]) * val;
-  }
+  for (int l = 0; l < L; ++l)
+    if (kflag[l]) s = s + (-a[l]) * kval[l];
   return s;
 }
 
@@ -97,18 +105,34 @@ __global__ void adv_diff_kernel(size_t n, double *__restrict__ a, const double *
   if (i < n) a[i] = a[i] - d[i];
 }
 
-// SolveFDwTE right-hand side (lib_solvFDwTE.f:245-285) incl. BouyancyRHS
-__global__ void vte_rhs_kernel(int nc, int nn, int e, MacroP mp, const int *__restrict__ idc, const double *__restrict__ Af,
-                               const int *__restrict__ colknown, const int *__restrict__ colfield,
-                               const double *__restrict__ vel, const double *__restrict__ vort,
-                               const double *__restrict__ qvort_e, const double *__restrict__ rhsv_e,
-                               const double *__restrict__ temp, double ren, int buoy, double fac, double gx, double gy,
-                               double gz, double *__restrict__ r) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)nc * 51) return;
-  int c = (int)(i / 51), it = (int)(i % 51);
-  const size_t m1 = (size_t)nc * 51;
-  double v = known_part(c, it, 51, 51, Af, colknown, colfield, vort + (size_t)nn * e, qvort_e);
+// SolveFDwTE right-hand side (lib_solvFDwTE.f:245-285) incl. BouyancyRHS; one
+// 64-thread workgroup per cell: the cell's 27 nodal values and its known
+// columns are staged in LDS once, each of the 51 rows then streams only its
+// matrix entries
+__global__ void __launch_bounds__(64) vte_rhs_kernel(int nc, int nn, int e, MacroP mp, const int *__restrict__ idc,
+                                                     const double *__restrict__ Af, const int *__restrict__ colknown,
+                                                     const int *__restrict__ colfield, const double *__restrict__ vel,
+                                                     const double *__restrict__ vort,
+                                                     const double *__restrict__ qvort_e,
+                                                     const double *__restrict__ rhsv_e,
+                                                     const double *__restrict__ temp, double ren, int buoy,
+                                                     double fac, double gx, double gy, double gz,
+                                                     double *__restrict__ r) {
+  __shared__ double sv[27][8];  // V0 V1 V2 W0 W1 W2 rhsv_e temp
+  __shared__ double kval[64];
+  __shared__ int kflag[64];
+  const int c = blockIdx.x, it = threadIdx.x;
+  if (it < 27) {
+    const size_t node = idc[c + (size_t)nc * it] - 1;
+    for (int d = 0; d < 3; ++d) sv[it][d] = vel[node + (size_t)nn * d], sv[it][3 + d] = vort[node + (size_t)nn * d];
+    sv[it][6] = rhsv_e[node];
+    sv[it][7] = buoy ? temp[node] : 0.0;
+  }
+  stage_known(c, 51, colknown, colfield, vort + (size_t)nn * e, qvort_e, kval, kflag);
+  __syncthreads();
+  if (it >= 51) return;
+  const size_t m1 = (size_t)nc * 51, i = (size_t)c * 51 + it;
+  double v = known_part(c, it, 51, 51, Af, kval, kflag);
   const double rn = ren / 1.0;
   // one pass over the matrices: the buoyancy terms (BouyancyRHS, added after
   // the convection sum as there) are summed beside the convection terms and
@@ -117,15 +141,13 @@ __global__ void vte_rhs_kernel(int nc, int nn, int e, MacroP mp, const int *__re
   double bsum = 0.0;
   const double Rro = 1.0e20;
   for (int j = 0; j < 27; ++j) {
-    size_t node = idc[c + (size_t)nc * j] - 1;
-    double V0 = vel[node], V1 = vel[node + nn], V2 = vel[node + 2 * (size_t)nn];
-    double W0 = vort[node], W1 = vort[node + nn], W2 = vort[node + 2 * (size_t)nn];
-    double we = vort[node + (size_t)nn * e], ve = vel[node + (size_t)nn * e];
-    size_t a = i + m1 * j;
+    const double V0 = sv[j][0], V1 = sv[j][1], V2 = sv[j][2], W0 = sv[j][3], W1 = sv[j][4], W2 = sv[j][5];
+    const double we = sv[j][3 + e], ve = sv[j][e];
+    const size_t a = i + m1 * j;
     const double ax = mp.Ex[a], ay = mp.Ey[a], az = mp.Ez[a];
-    v = v - mp.B[a] * rhsv_e[node] * ren / 1.0 -
+    v = v - mp.B[a] * sv[j][6] * ren / 1.0 -
         rn * (+ax * (V0 * we - ve * W0) + ay * (V1 * we - ve * W1) + az * (V2 * we - ve * W2));
-    const double tt = buoy ? temp[node] - 0.0 / Rro : 0.0;
+    const double tt = buoy ? sv[j][7] - 0.0 / Rro : 0.0;
     if (e == 0)
       bsum += fac * (+gz * ay * tt - gy * az * tt);
     else if (e == 1)
@@ -137,40 +159,65 @@ __global__ void vte_rhs_kernel(int nc, int nn, int e, MacroP mp, const int *__re
   r[i] = v;
 }
 
-// SolveFDdae right-hand side (lib_solvFDdae.f:209-254)
-__global__ void t_rhs_kernel(int nc, int nn, MacroP mp, const int *__restrict__ idc, const double *__restrict__ Af,
-                             const int *__restrict__ colknown, const int *__restrict__ colfield,
-                             const double *__restrict__ vel, const double *__restrict__ temp,
-                             const double *__restrict__ qtemp, const double *__restrict__ ttime, double enazalfa,
-                             double *__restrict__ r) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)nc * 51) return;
-  int c = (int)(i / 51), it = (int)(i % 51);
-  const size_t m1 = (size_t)nc * 51;
-  double v = known_part(c, it, 51, 51, Af, colknown, colfield, temp, qtemp);
+// SolveFDdae right-hand side (lib_solvFDdae.f:209-254); one 64-thread
+// workgroup per cell with the nodal and known values staged in LDS
+__global__ void __launch_bounds__(64) t_rhs_kernel(int nc, int nn, MacroP mp, const int *__restrict__ idc,
+                                                   const double *__restrict__ Af, const int *__restrict__ colknown,
+                                                   const int *__restrict__ colfield, const double *__restrict__ vel,
+                                                   const double *__restrict__ temp, const double *__restrict__ qtemp,
+                                                   const double *__restrict__ ttime, double enazalfa,
+                                                   double *__restrict__ r) {
+  __shared__ double sv[27][5];  // V0 V1 V2 temp ttime
+  __shared__ double kval[64];
+  __shared__ int kflag[64];
+  const int c = blockIdx.x, it = threadIdx.x;
+  if (it < 27) {
+    const size_t node = idc[c + (size_t)nc * it] - 1;
+    for (int d = 0; d < 3; ++d) sv[it][d] = vel[node + (size_t)nn * d];
+    sv[it][3] = temp[node];
+    sv[it][4] = ttime[node];
+  }
+  stage_known(c, 51, colknown, colfield, temp, qtemp, kval, kflag);
+  __syncthreads();
+  if (it >= 51) return;
+  const size_t m1 = (size_t)nc * 51, i = (size_t)c * 51 + it;
+  double v = known_part(c, it, 51, 51, Af, kval, kflag);
   for (int j = 0; j < 27; ++j) {
-    size_t node = idc[c + (size_t)nc * j] - 1;
-    double V0 = vel[node], V1 = vel[node + nn], V2 = vel[node + 2 * (size_t)nn], u = temp[node];
-    size_t a = i + m1 * j;
-    v = v - mp.B[a] * ttime[node] * enazalfa - enazalfa * (+mp.Ex[a] * V0 * u + mp.Ey[a] * V1 * u + mp.Ez[a] * V2 * u);
+    const double V0 = sv[j][0], V1 = sv[j][1], V2 = sv[j][2], u = sv[j][3];
+    const size_t a = i + m1 * j;
+    v = v - mp.B[a] * sv[j][4] * enazalfa - enazalfa * (+mp.Ex[a] * V0 * u + mp.Ey[a] * V1 * u + mp.Ez[a] * V2 * u);
   }
   r[i] = v;
 }
 
 // rhsKM3 (lib_km.f:304-389) incl. the rhsm*knv part
-__global__ void km_rhs_kernel(int nc, int nn, int e, MacroP mp, const int *__restrict__ idc, const double *__restrict__ Af,
-                              const int *__restrict__ colknown, const int *__restrict__ colfield,
-                              const double *__restrict__ vel, const double *__restrict__ vort, double *__restrict__ r) {
-  size_t ii = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (ii >= (size_t)nc * 27) return;
-  int c = (int)(ii / 27), it = (int)(ii % 27);
-  const size_t m1 = (size_t)nc * 51, m2 = (size_t)nc * 27, i = (size_t)c * 51 + it;
-  double v = known_part(c, it, 27, 27, Af, colknown, colfield, vel + (size_t)nn * e, nullptr);
+// Two cells per 64-thread workgroup (lanes 0-26 and 32-58, one row each); each
+// cell's nodal velocities and vorticities and its known columns are staged in
+// LDS once, the rows then stream only their matrix entries
+__global__ void __launch_bounds__(64) km_rhs_kernel(int nc, int nn, int e, MacroP mp, const int *__restrict__ idc,
+                                                    const double *__restrict__ Af, const int *__restrict__ colknown,
+                                                    const int *__restrict__ colfield, const double *__restrict__ vel,
+                                                    const double *__restrict__ vort, double *__restrict__ r) {
+  __shared__ double sv[2][27][6];  // V0 V1 V2 W0 W1 W2
+  __shared__ double kval[2][32];
+  __shared__ int kflag[2][32];
+  const int h = threadIdx.x >> 5, it = threadIdx.x & 31, c = blockIdx.x * 2 + h;
+  const bool row = it < 27 && c < nc;
+  if (row) {
+    const size_t node = idc[c + (size_t)nc * it] - 1;
+    for (int d = 0; d < 3; ++d) sv[h][it][d] = vel[node + (size_t)nn * d], sv[h][it][3 + d] = vort[node + (size_t)nn * d];
+    const int kn = colknown[(size_t)c * 27 + it];
+    kflag[h][it] = kn;
+    kval[h][it] = kn ? vel[(size_t)nn * e + colfield[(size_t)c * 27 + it]] : 0.0;
+  }
+  __syncthreads();
+  if (!row) return;
+  const size_t m1 = (size_t)nc * 51, m2 = (size_t)nc * 27, ii = (size_t)c * 27 + it, i = (size_t)c * 51 + it;
+  double v = known_part(c, it, 27, 27, Af, kval[h], kflag[h]);
   for (int j = 0; j < 27; ++j) {
-    size_t node = idc[c + (size_t)nc * j] - 1;
-    size_t a2 = ii + m2 * j, a1 = i + m1 * j;
-    double V0 = vel[node], V1 = vel[node + nn], V2 = vel[node + 2 * (size_t)nn];
-    double W0 = vort[node], W1 = vort[node + nn], W2 = vort[node + 2 * (size_t)nn];
+    const size_t a2 = ii + m2 * j, a1 = i + m1 * j;
+    const double V0 = sv[h][j][0], V1 = sv[h][j][1], V2 = sv[h][j][2];
+    const double W0 = sv[h][j][3], W1 = sv[h][j][4], W2 = sv[h][j][5];
     if (e == 0)
       v = v - mp.Htz[a2] * V1 + mp.Hty[a2] * V2 + mp.Dz[a1] * W1 - mp.Dy[a1] * W2;
     else if (e == 1)
@@ -505,7 +552,7 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
     {
       EvTimer t(s);
       for (int e = 0; e < 3; ++e) {
-        hipLaunchKernelGGL(km_rhs_kernel, dim3(ceil_div((size_t)nc * 27, 128)), dim3(128), 0, s, nc, nn, e, mp,
+        hipLaunchKernelGGL(km_rhs_kernel, dim3(ceil_div((size_t)nc, 2)), dim3(64), 0, s, nc, nn, e, mp,
                            c->mesh.idc.p, c->Af_km.p, c->km.colknown.p, c->km.colfield.p, c->vel.p, c->vort.p, c->b.p);
         TT_LAUNCH_CHECK();
         its[3 + e] = solve_subsys(*c, c->km, c->vel.p + (size_t)nn * e, nullptr, c->dlse[e], e);
@@ -522,7 +569,7 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
     if (cv.iDT == 1) {
       EvTimer t(s);
       double enazalfa = c->ren * cv.PrN / 1.0;
-      hipLaunchKernelGGL(t_rhs_kernel, dim3(ceil_div((size_t)nc * 51, 128)), dim3(128), 0, s, nc, nn, mp, c->mesh.idc.p,
+      hipLaunchKernelGGL(t_rhs_kernel, dim3(nc), dim3(64), 0, s, nc, nn, mp, c->mesh.idc.p,
                          c->Af_t.p, c->tsys.colknown.p, c->tsys.colfield.p, c->vel.p, c->temp.p, c->qtemp.p,
                          c->ttime.p, enazalfa, c->b.p);
       TT_LAUNCH_CHECK();
@@ -537,7 +584,7 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
       EvTimer t(s);
       double fac = cv.RaN / cv.PrN / c->ren * 1.0 / 1.0;
       for (int e = 0; e < 3; ++e) {
-        hipLaunchKernelGGL(vte_rhs_kernel, dim3(ceil_div((size_t)nc * 51, 128)), dim3(128), 0, s, nc, nn, e, mp,
+        hipLaunchKernelGGL(vte_rhs_kernel, dim3(nc), dim3(64), 0, s, nc, nn, e, mp,
                            c->mesh.idc.p, c->Af_vte.p, c->vte[e].colknown.p, c->vte[e].colfield.p, c->vel.p,
                            c->vort.p, c->qvort.p + (size_t)nq * e, c->rhsv.p + (size_t)nn * e, c->temp.p, c->ren,
                            cv.iDT > 0 ? 1 : 0, fac, cv.gx, cv.gy, cv.gz, c->b.p);
