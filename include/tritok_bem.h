@@ -113,6 +113,18 @@ int tt_lsqr_setup(int32_t m, int32_t n, int32_t nnz, const int32_t *ia, const in
  * Returns istop (0..5) in *istop and the iteration count in *nit. */
 int tt_lsqr_solve(tt_lsqr *s, const double *b, double *x, int32_t maxit, float eps4, int32_t *nit,
                   int32_t *istop);
+/* No counterpart in the reference: tt_lsqr_solve for callers that cannot
+ * allocate device memory (code under HIP stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  b_dev (m) and x_dev (n) are device
+ * pointers on the handle's device; x_dev holds the hot start on entry and the
+ * solution on exit.  The work vectors (u, v, w, the correction, partial sums,
+ * LSQR state) live in a library-owned scratch array of 32 MiB per device,
+ * held until the process ends: systems with m + 3n above about 4.19 million
+ * are rejected with an error.  Every call on one device shares that scratch,
+ * so calls that overlap on one GPU race.  Same kernels and results as
+ * tt_lsqr_solve; nothing is allocated or freed. */
+int tt_lsqr_solve_noalloc(tt_lsqr *s, const double *b_dev, double *x_dev, int32_t maxit, float eps4, int32_t *nit,
+                          int32_t *istop);
 void tt_lsqr_free(tt_lsqr *s);
 
 /* Many independent systems in one persistent launch: prelsqr2 for each at

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "tritok_bem.h"
@@ -247,7 +248,7 @@ __global__ void gather_kernel(int nnz, const int *__restrict__ perm, const doubl
 using namespace tt;
 
 struct tt_lsqr {
-  int m = 0, n = 0, nnz = 0, grow = 0, gcol = 0;
+  int m = 0, n = 0, nnz = 0, grow = 0, gcol = 0, dev = 0;
   hipStream_t s = nullptr;
   DBuf<int> ia, ja, cp, ci;
   DBuf<double> a, at, q;
@@ -309,6 +310,7 @@ extern "C" int tt_lsqr_setup(int32_t m, int32_t n, int32_t nnz, const int32_t *i
     int dev = 0, cus = 0;
     TT_HIP(hipGetDevice(&dev));
     TT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    s->dev = dev;
     s->grow = std::max(1, std::min((int)ceil_div(m, kRPB), 8 * cus));
     s->gcol = std::max(1, std::min((int)ceil_div(n, kRPB), 8 * cus));
     s->u.alloc(m), s->b.alloc(m), s->v.alloc(n), s->w.alloc(n), s->x.alloc(n), s->dx.alloc(n);
@@ -322,52 +324,155 @@ extern "C" int tt_lsqr_setup(int32_t m, int32_t n, int32_t nnz, const int32_t *i
   });
 }
 
+namespace {
+
+// Work vectors of one solve: u (m), v, w, dx (n), the row and column partials
+// and the two state copies.
+struct CsrWork {
+  double *u, *v, *w, *dx, *pa1, *pa2, *pb1, *pb2;
+  LsqrState *state;
+};
+
+// slvlsqr2's initial state (both copies): atol = eps, conlim = 1e8, itnlim
+__global__ void csr_init_state_kernel(LsqrState *st, double atol, int itnlim) {
+  const int i = threadIdx.x;
+  if (i >= 2) return;
+  LsqrState o{};
+  o.cs2 = -1.0;
+  o.atol = atol;
+  o.ctol = 1.0 / 1.0e8;
+  o.itnlim = itnlim;
+  st[i] = o;
+}
+
+__global__ void csr_zero_kernel(int n, double *__restrict__ p) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) p[j] = 0.0;
+}
+
+// The solve of slvlsqr2 (lib_Solve_v11.f:4401-4500) on device b and x (x is
+// the hot start on entry and the solution on exit).  The state is read back
+// into *rb (host memory) once per batch of iterations.
+void csr_solve(tt_lsqr *s, hipStream_t st, const double *b, double *x, const CsrWork &wk, int32_t maxit, float eps4,
+               LsqrState *rb, int32_t *nit, int32_t *istop) {
+  const int m = s->m, n = s->n;
+  const int itnlim = maxit == 0 ? 10 * (m + n + 50) : maxit;
+  hipLaunchKernelGGL(csr_init_state_kernel, dim3(1), dim3(64), 0, st, wk.state, (double)eps4, itnlim);
+  hipLaunchKernelGGL(csr_zero_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, wk.dx);
+  hipLaunchKernelGGL(csr_r0_kernel, dim3(s->grow), dim3(kTPB), 0, st, m, s->ia.p, s->ja.p, s->a.p, s->q.p, x, b,
+                     wk.u, wk.pa1, wk.pa2);
+  hipLaunchKernelGGL(csr_col_kernel, dim3(s->gcol), dim3(kTPB), 0, st, n, 0, s->cp.p, s->ci.p, s->at.p, wk.u, wk.v,
+                     wk.w, wk.dx, wk.pa1, wk.pa2, s->grow, wk.pb1, wk.pb2, wk.state);
+  TT_LAUNCH_CHECK();
+  int k = 1, batch = s->last_nit > 0 ? s->last_nit + 2 : 8;
+  for (;;) {
+    for (int i = 0; i < batch; ++i, ++k) {
+      hipLaunchKernelGGL(csr_row_kernel, dim3(s->grow + 1), dim3(kTPB), 0, st, k, m, s->ia.p, s->ja.p, s->a.p, wk.v,
+                         wk.u, wk.pa1, wk.pb1, wk.pb2, s->gcol, wk.state + ((k - 1) & 1), wk.state + (k & 1));
+      hipLaunchKernelGGL(csr_col_kernel, dim3(s->gcol), dim3(kTPB), 0, st, n, k, s->cp.p, s->ci.p, s->at.p, wk.u,
+                         wk.v, wk.w, wk.dx, wk.pa1, wk.pa2, s->grow, wk.pb1, wk.pb2, wk.state + (k & 1));
+    }
+    TT_LAUNCH_CHECK();
+    TT_HIP(hipMemcpyAsync(rb, wk.state + ((k - 1) & 1), sizeof(LsqrState), hipMemcpyDeviceToHost, st));
+    TT_HIP(hipStreamSynchronize(st));
+    if (rb->done) break;
+    if (k > itnlim + 3) throw std::runtime_error("LSQR did not signal termination");
+    batch = std::max(4, std::min(64, k / 4));
+  }
+  if (nit) *nit = rb->itn;
+  if (istop) *istop = rb->istop;
+  s->last_nit = rb->itn;
+  hipLaunchKernelGGL(csr_finish_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, s->q.p, wk.dx, x);
+  TT_LAUNCH_CHECK();
+}
+
+// Scratch of the allocation-free solve: one array per device, held until the
+// process ends.  In host code `scratch` only names the symbol; its device
+// address comes from hipGetSymbolAddress, once per device (scratch_base).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+constexpr size_t kScratchDoubles = kScratchBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_base[kMaxDevices] = {};
+
+// device address of `scratch` on the current device
+double *scratch_base() {
+  int dev = 0;
+  TT_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index outside the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_base[dev]) {
+    void *p = nullptr;
+    TT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_base[dev] = static_cast<double *>(p);
+  }
+  return g_scratch_base[dev];
+}
+
+// makes dev current for the lifetime of the guard
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    TT_HIP(hipGetDevice(&prev));
+    if (prev != dev) TT_HIP(hipSetDevice(dev));
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+}  // namespace
+
 extern "C" int tt_lsqr_solve(tt_lsqr *s, const double *b, double *x, int32_t maxit, float eps4, int32_t *nit,
                              int32_t *istop) {
   return guarded("tt_lsqr_solve", [&] {
     if (!s || !b || !x) throw std::runtime_error("NULL argument");
     hipStream_t st = s->s;
-    const int m = s->m, n = s->n;
-    LsqrState init;
-    std::memset(&init, 0, sizeof(init));
-    init.cs2 = -1.0;
-    init.atol = (double)eps4;
-    init.ctol = 1.0 / 1.0e8;
-    init.itnlim = maxit == 0 ? 10 * (m + n + 50) : maxit;
-    LsqrState init2[2] = {init, init};
-    TT_HIP(hipMemcpyAsync(s->state.p, init2, sizeof(init2), hipMemcpyHostToDevice, st));
-    s->b.upload(b, m, st);
-    s->x.upload(x, n, st);
-    TT_HIP(hipMemsetAsync(s->dx.p, 0, sizeof(double) * n, st));
-    hipLaunchKernelGGL(csr_r0_kernel, dim3(s->grow), dim3(kTPB), 0, st, m, s->ia.p, s->ja.p, s->a.p, s->q.p, s->x.p,
-                       s->b.p, s->u.p, s->pa1.p, s->pa2.p);
-    hipLaunchKernelGGL(csr_col_kernel, dim3(s->gcol), dim3(kTPB), 0, st, n, 0, s->cp.p, s->ci.p, s->at.p, s->u.p,
-                       s->v.p, s->w.p, s->dx.p, s->pa1.p, s->pa2.p, s->grow, s->pb1.p, s->pb2.p, s->state.p);
-    TT_LAUNCH_CHECK();
-    int k = 1, batch = s->last_nit > 0 ? s->last_nit + 2 : 8;
-    for (;;) {
-      for (int i = 0; i < batch; ++i, ++k) {
-        hipLaunchKernelGGL(csr_row_kernel, dim3(s->grow + 1), dim3(kTPB), 0, st, k, m, s->ia.p, s->ja.p, s->a.p,
-                           s->v.p, s->u.p, s->pa1.p, s->pb1.p, s->pb2.p, s->gcol, s->state.p + ((k - 1) & 1),
-                           s->state.p + (k & 1));
-        hipLaunchKernelGGL(csr_col_kernel, dim3(s->gcol), dim3(kTPB), 0, st, n, k, s->cp.p, s->ci.p, s->at.p, s->u.p,
-                           s->v.p, s->w.p, s->dx.p, s->pa1.p, s->pa2.p, s->grow, s->pb1.p, s->pb2.p,
-                           s->state.p + (k & 1));
-      }
-      TT_LAUNCH_CHECK();
-      TT_HIP(hipMemcpyAsync(s->host_state, s->state.p + ((k - 1) & 1), sizeof(LsqrState), hipMemcpyDeviceToHost, st));
-      TT_HIP(hipStreamSynchronize(st));
-      if (s->host_state->done) break;
-      if (k > init.itnlim + 3) throw std::runtime_error("LSQR did not signal termination");
-      batch = std::max(4, std::min(64, k / 4));
-    }
-    if (nit) *nit = s->host_state->itn;
-    if (istop) *istop = s->host_state->istop;
-    s->last_nit = s->host_state->itn;
-    hipLaunchKernelGGL(csr_finish_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, s->q.p, s->dx.p, s->x.p);
-    TT_LAUNCH_CHECK();
-    s->x.download(x, n, st);
+    s->b.upload(b, s->m, st);
+    s->x.upload(x, s->n, st);
+    const CsrWork wk{s->u.p, s->v.p, s->w.p, s->dx.p, s->pa1.p, s->pa2.p, s->pb1.p, s->pb2.p, s->state.p};
+    csr_solve(s, st, s->b.p, s->x.p, wk, maxit, eps4, s->host_state, nit, istop);
+    s->x.download(x, s->n, st);
     TT_HIP(hipStreamSynchronize(st));
+    return 0;
+  });
+}
+
+extern "C" int tt_lsqr_solve_noalloc(tt_lsqr *s, const double *b_dev, double *x_dev, int32_t maxit, float eps4,
+                                     int32_t *nit, int32_t *istop) {
+  return guarded("tt_lsqr_solve_noalloc", [&] {
+    if (!s || !b_dev || !x_dev) throw std::runtime_error("NULL argument");
+    DeviceGuard dg(s->dev);
+    double *base = scratch_base();
+    // carve the work vectors out of the scratch, each on a 256-byte boundary
+    const size_t align = 256 / sizeof(double);
+    auto up = [&](size_t c) { return (c + align - 1) / align * align; };
+    const size_t n_state = up((2 * sizeof(LsqrState) + sizeof(double) - 1) / sizeof(double));
+    const size_t need = n_state + up(s->m) + 3 * up(s->n) + 2 * up(s->grow) + 2 * up(s->gcol);
+    if (need > kScratchDoubles) {
+      char msg[256];
+      std::snprintf(msg, sizeof(msg), "system needs %zu bytes of work space, the scratch holds %zu (m + 3n too large)",
+                    need * sizeof(double), kScratchBytes);
+      throw std::runtime_error(msg);
+    }
+    CsrWork wk;
+    size_t off = 0;
+    wk.state = reinterpret_cast<LsqrState *>(base);
+    off += n_state;
+    auto take = [&](size_t c) {
+      double *p = base + off;
+      off += up(c);
+      return p;
+    };
+    wk.u = take(s->m), wk.v = take(s->n), wk.w = take(s->n), wk.dx = take(s->n);
+    wk.pa1 = take(s->grow), wk.pa2 = take(s->grow), wk.pb1 = take(s->gcol), wk.pb2 = take(s->gcol);
+    LsqrState rb;
+    csr_solve(s, s->s, b_dev, x_dev, wk, maxit, eps4, &rb, nit, istop);
+    TT_HIP(hipStreamSynchronize(s->s));
     return 0;
   });
 }

@@ -13,6 +13,7 @@ PD, PI = _lib.PD, _lib.PI
 _csr_mv = _lib._sig("tt_csr_mv", i32, i32, i32, i32, PI, PI, PD, PD, PD)
 _lsqr_setup = _lib._sig("tt_lsqr_setup", i32, i32, i32, i32, PI, PI, PD, PD, i32, C.POINTER(P))
 _lsqr_solve = _lib._sig("tt_lsqr_solve", i32, P, PD, PD, i32, C.c_float, PI, PI)
+_lsqr_solve_noalloc = _lib._sig("tt_lsqr_solve_noalloc", i32, P, P, P, i32, C.c_float, PI, PI)
 _lsqr_free = _lib._sig("tt_lsqr_free", None, P)
 _lu = _lib._sig("tt_dense_lu", i32, i32, PD, PD)
 _lu_solve = _lib._sig("tt_dense_lu_solve", i32, i32, PD, PD, PD)
@@ -62,6 +63,15 @@ class Lsqr:
         check(_lsqr_solve(self._h, b.ctypes.data_as(PD), x.ctypes.data_as(PD), int(maxit), float(eps4),
                           C.byref(nit), C.byref(istop)), "tt_lsqr_solve")
         return x, istop.value, nit.value
+
+    def solve_noalloc(self, b_dev, x_dev, maxit, eps4):
+        """tt_lsqr_solve_noalloc on device pointers (integers): b_dev (m
+        doubles) and x_dev (n doubles, hot start in, solution out).  Returns
+        (istop, nit)."""
+        nit, istop = i32(), i32()
+        check(_lsqr_solve_noalloc(self._h, P(b_dev), P(x_dev), int(maxit), float(eps4), C.byref(nit),
+                                  C.byref(istop)), "tt_lsqr_solve_noalloc")
+        return istop.value, nit.value
 
     def close(self):
         if getattr(self, "_h", None):
