@@ -328,10 +328,29 @@ def assembly_only(n):
 
 
 def lsqr_matrix_bytes(info, value_bytes):
-    """Algorithmic bytes of the two sparse products of one LSQR iteration for a
-    CSR system (values + int32 column indices, row pointers), per the
-    reference's CSR form: 2 * ((value_bytes + 4) * nnz + 4 * (m + 1))."""
+    """Bytes the two sparse products of one LSQR iteration would move in the
+    reference's CSR form (values + int32 column indices, row pointers):
+    2 * ((value_bytes + 4) * nnz + 4 * (m + 1)).  Reported beside the roofline
+    as a CSR-equivalent rate; the block format moves less."""
     return 2.0 * ((value_bytes + 4) * info["nnz"] + 4 * (info["m"] + 1))
+
+
+def cell_kernel_bytes(info, value_bytes, partial_bytes):
+    """Algorithmic HBM bytes of one launch of the least-squares cell kernel on
+    block storage: every cell's R x L block once (value_bytes per entry,
+    padding not counted), its column map (int32), u read and u' written (FP64,
+    m each), v read once (FP64, n) and the A^T u' partials written
+    (partial_bytes per cell column)."""
+    c, R, L = info["cells"], info["rows_per_cell"], info["cols_per_cell"]
+    return float(c * R * L * value_bytes + c * L * 4 + 16 * info["m"] + 8 * info["n"] + c * L * partial_bytes)
+
+
+def unknown_kernel_bytes(info, partial_bytes):
+    """Algorithmic HBM bytes of one launch of the unknown kernel: the cell
+    partials gathered (partial_bytes per cell column), the gather table (8
+    int32 per unknown), x and w read and written, v' written (FP64)."""
+    c, L, n = info["cells"], info["cols_per_cell"], info["n"]
+    return float(c * L * partial_bytes + 32 * n + 40 * n)
 
 
 def main():
@@ -436,20 +455,38 @@ def main():
              "kinematics": {k: case.system_info(4)[k] for k in ("grid_fp64", "grid_fp32", "grid_fp16")}}
     nit = mixed["nits"].sum(0)
     w = np.array([nit[6], nit[7], nit[8], nit[9]], float)
-    alg = np.array([lsqr_matrix_bytes(i, vbytes) for i in infos])
-    alg_bytes = float((w * alg).sum() / max(w.sum(), 1.0))
+    pbytes = 4  # FP16 and FP32 blocks store the A^T u' partials in FP32
+    cell_bytes = float((w * np.array([cell_kernel_bytes(i, vbytes, pbytes) for i in infos])).sum() / max(w.sum(), 1.0))
+    unk_bytes = float((w * np.array([unknown_kernel_bytes(i, pbytes) for i in infos])).sum() / max(w.sum(), 1.0))
+    csr_bytes = float((w * np.array([lsqr_matrix_bytes(i, vbytes) for i in infos])).sum() / max(w.sum(), 1.0))
     avg_ms = mixed["probe51"]["avg_ms"]
     traffic = read_traffic(kname)  # measured HBM bytes per launch (profiles/)
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
+    achieved = cell_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None
+    # the whole LSQR iteration of the 51-row systems: the vorticity and energy
+    # phases' device time over their iterations (includes their right-hand
+    # sides, hot-start residuals and the host's batch synchronisations)
+    ph_mean = {k: float(np.mean([p[k] for p in mixed["phases"]])) for k in ("vte", "temperature")}
+    it_ms = (ph_mean["vte"] + ph_mean["temperature"]) / max(w.sum() / args.steps, 1.0)
+    it_gbs = (cell_bytes + unk_bytes) / (it_ms * 1e-3) / 1e9
     roofline = {"kernel": "%s (51-row blocks)" % kname, "bound": "hbm",
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None,
                 "traffic": traffic,
                 "traffic_GBps": traffic / (avg_ms * 1e-3) / 1e9 if traffic and avg_ms > 0 else None,
                 "traffic_frac": traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic and avg_ms > 0 else None,
+                "algorithmic_bytes_per_launch": cell_bytes,
+                "bytes": "cells*R*L*2 (FP16 block values, unpadded) + cells*L*4 (column maps) + 16m (u read, u' "
+                         "written) + 8n (v) + cells*L*4 (FP32 A^T u' partials)",
                 "avg_launch_ms": avg_ms, "launches_timed": mixed["probe51"]["sampled"], "tuned_grid": tuned,
-                "launches": mixed["probe51"]["launches"], "algorithmic_bytes_per_launch": alg_bytes,
-                "block_bytes_per_launch": float(infos[0]["cells"] * (2608 * 2 if half else 2604 * 4))}
+                "launches": mixed["probe51"]["launches"],
+                "block_bytes_per_launch": float(infos[0]["cells"] * (2608 * 2 if half else 2604 * 4)),
+                "csr_equivalent_bytes_per_launch": csr_bytes,
+                "csr_equivalent_GBps": csr_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None,
+                "lsqr_iteration": {"algorithmic_bytes": cell_bytes + unk_bytes,
+                                   "unknown_kernel_bytes": unk_bytes, "ms": it_ms, "GBps": it_gbs,
+                                   "frac": it_gbs / HBM_PEAK_GBS,
+                                   "time": "vorticity + energy phase device time per step / their LSQR "
+                                           "iterations per step"}}
 
     cpu = None
     if not args.no_cpu:

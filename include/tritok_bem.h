@@ -370,7 +370,10 @@ int tt_case_system_info(tt_case *c, int32_t sys, int64_t info[9]);
  * changes (tri.sta), nit[12] = solver iterations (tri.ite).  Returns 1 while
  * the loop should continue, 0 when converged or at nnlit, < 0 on error --
  * including a diverged iteration (a non-finite relative change, where the
- * reference would stop as if converged). */
+ * reference would stop as if converged), and a vorticity or energy LSQR solve
+ * that stops with istop > 3 (WarnErr level 4 in the reference).  A kinematics
+ * solve that stops with istop > 3 keeps its updated velocities and the step
+ * goes on, as lib_km.f:274 does. */
 int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]);
 /* Runs Picard iterations until convergence; returns the iteration count. */
 int tt_case_run(tt_case *c, int32_t max_iter, double *sta, int32_t *ite);

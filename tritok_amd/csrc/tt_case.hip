@@ -307,6 +307,7 @@ struct tt_case {
   double beta = 0, beta2 = 0, beta3 = 0, ren = 0;
   int it = 0;
   bool finished = false;
+  int64_t km_solver_warnings = 0;  // kinematics solves that ended with istop > 3 (the case continues)
   double t_phase[11] = {0};
   hipEvent_t ev[8];
 };
@@ -371,15 +372,23 @@ struct EvTimer {
   }
 };
 
-// slot: the field component a system shared by components is solved for
-int solve_subsys(tt_case &c, SubSys &S, const double *fnode, const double *fflux, float eps4, int slot) {
+// slot: the field component a system shared by components is solved for.
+// fatal: istop > 3 (condition limit or iteration limit) stops the case, as
+// WarnErr level 4 does after the vorticity and energy solves
+// (lib_solvFDwTE.f:297, lib_solvFDdae.f:265); the kinematics solves only
+// report it and keep the updated x (lib_km.f:274).
+int solve_subsys(tt_case &c, SubSys &S, const double *fnode, const double *fflux, float eps4, int slot,
+                 bool fatal) {
   const int n = S.n;
   hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, c.s, n, c.cv.mesh.nnodes, S.unk_field.p,
                      fnode, fflux, c.x.p);
   TT_LAUNCH_CHECK();
   LsqrResult r = block_lsqr(S.bs, c.work, c.b.p, c.x.p, c.cv.lsqs_maxit, eps4, c.lsqr_prec, c.s,
                             S.R == 51 ? &c.probe51 : &c.probe27, c.graphs, slot);
-  if (r.istop > 3) throw std::runtime_error("LSQR stopped with istop > 3 (ill-conditioned or iteration limit)");
+  if (r.istop > 3) {
+    if (fatal) throw std::runtime_error("LSQR stopped with istop > 3 (ill-conditioned or iteration limit)");
+    ++c.km_solver_warnings;
+  }
   return r.nit;
 }
 
@@ -555,7 +564,7 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
         hipLaunchKernelGGL(km_rhs_kernel, dim3(ceil_div((size_t)nc, 2)), dim3(64), 0, s, nc, nn, e, mp,
                            c->mesh.idc.p, c->Af_km.p, c->km.colknown.p, c->km.colfield.p, c->vel.p, c->vort.p, c->b.p);
         TT_LAUNCH_CHECK();
-        its[3 + e] = solve_subsys(*c, c->km, c->vel.p + (size_t)nn * e, nullptr, c->dlse[e], e);
+        its[3 + e] = solve_subsys(*c, c->km, c->vel.p + (size_t)nn * e, nullptr, c->dlse[e], e, false);
         hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c->km.n, 256)), dim3(256), 0, s, c->km.n, nn, cv.urDv[e],
                            c->km.unk_field.p, c->x.p, c->vel.p + (size_t)nn * e, nullptr);
         if (c->kobc.p)
@@ -573,7 +582,7 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
                          c->Af_t.p, c->tsys.colknown.p, c->tsys.colfield.p, c->vel.p, c->temp.p, c->qtemp.p,
                          c->ttime.p, enazalfa, c->b.p);
       TT_LAUNCH_CHECK();
-      its[9] = solve_subsys(*c, c->tsys, c->temp.p, c->qtemp.p, c->dlse[6], 0);
+      its[9] = solve_subsys(*c, c->tsys, c->temp.p, c->qtemp.p, c->dlse[6], 0, true);
       hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c->tsys.n, 256)), dim3(256), 0, s, c->tsys.n, nn, cv.urDT,
                          c->tsys.unk_field.p, c->x.p, c->temp.p, c->qtemp.p);
       TT_LAUNCH_CHECK();
@@ -590,7 +599,7 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
                            cv.iDT > 0 ? 1 : 0, fac, cv.gx, cv.gy, cv.gz, c->b.p);
         TT_LAUNCH_CHECK();
         its[6 + e] = solve_subsys(*c, c->vte[e], c->vort.p + (size_t)nn * e, c->qvort.p + (size_t)nq * e, c->dlse[3 + e],
-                                  0);
+                                  0, true);
         hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c->vte[e].n, 256)), dim3(256), 0, s, c->vte[e].n, nn,
                            cv.urDw[e], c->vte[e].unk_field.p, c->x.p, c->vort.p + (size_t)nn * e,
                            c->qvort.p + (size_t)nq * e);

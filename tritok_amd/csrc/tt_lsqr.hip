@@ -685,25 +685,31 @@ template <typename T>
 constexpr int cell_threads() {
   return sizeof(T) == 8 ? 64 : 64 * kNW;
 }
-// resident workgroups of a cell-kernel instantiation on the whole device
+// resident workgroups of a cell-kernel instantiation on the whole device,
+// computed once per device and published under a lock (serving threads build
+// cases concurrently)
 template <typename T, int R, int L>
 int persistent_grid(int ngroup) {
-  static int per_cu = -1, cus = 0;
-  if (per_cu < 0) {
-    int dev = 0;
-    TT_HIP(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::map<int, int> resident;  // device -> workgroups resident on the whole device
+  int dev = 0;
+  TT_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = resident.find(dev);
+  if (it == resident.end()) {
+    int cus = 0, per_cu = 0, lds_per_cu = 0;
     TT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cell_kernel<T, R, L>(), cell_threads<T>(), 0));
     hipFuncAttributes fa;
     TT_HIP(hipFuncGetAttributes(&fa, cell_kernel<T, R, L>()));
-    int lds_per_cu = 0;
     TT_HIP(hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     // static LDS rounded to the 512-byte allocation granule
     const size_t lds = (fa.sharedSizeBytes + 511) / 512 * 512;
     if (lds > 0 && lds_per_cu > 0) per_cu = std::min<int>(per_cu, (int)(lds_per_cu / lds));
     if (per_cu < 1) per_cu = 1;
+    it = resident.emplace(dev, per_cu * cus).first;
   }
-  return std::min(ngroup, per_cu * cus);
+  return std::min(ngroup, it->second);
 }
 // e0/e1 non-null: the dispatch itself records its start and end (sampled timing)
 template <typename T, int R, int L>
