@@ -20,7 +20,11 @@ the reference, and converts its outputs into ``tests/golden/<case>.npz``:
   diffusivities fed to the reference are stored with the matrices;
 * ``gauss_ref.npz``  -> the reference's Gauss-Legendre table as its own
   GaussPosWeights (src/lib_int.f:2673-3173) fills it, written by the small
-  Fortran harness dump_gauss.f90 linked against the reference's objects.
+  Fortran harness dump_gauss.f90 linked against the reference's objects;
+* ``compress_ref.npz`` -> the reference's cross approximations (aca_CrossAprox,
+  ACA_AB, aca_RankCrossAprox; src/lib_ACA.f) and wavelet compressions
+  (Che_WcomMatrix, src/lib_WT.f:148-210) of the matrices in COMPRESS_CASES,
+  written by the harness dump_compress.f90 linked against the same objects.
 
 The mesh arrays (coordinates and connectivity, as read from the .geo) and the
 boundary-condition file text are stored too, so the tests can feed the very same
@@ -344,6 +348,97 @@ def dump_gauss():
     print("gauss_ref ->", os.path.join(HERE, "gauss_ref.npz"))
 
 
+def _kernel_block(m, n, sep, seed):
+    """Smooth BEM-like block: 1/|x - y| between two separated point clouds."""
+    rng = np.random.default_rng(seed)
+    x = rng.random((m, 3))
+    y = rng.random((n, 3)) + np.array([sep, 0.0, 0.0])
+    return 1.0 / np.linalg.norm(x[:, None, :] - y[None, :, :], axis=2)
+
+
+def _compress_matrices():
+    """(kind, eps or kappa, seed, matrix) of every case of compress_ref.npz.
+    kind: 1 aca_CrossAprox, 2 ACA_AB, 3 aca_RankCrossAprox, 4 Che_WcomMatrix."""
+    rng = np.random.default_rng(20251204)
+    cases = []
+    rnd = rng.standard_normal((23, 17))
+    k1 = _kernel_block(37, 90, 2.0, 1)
+    k2 = _kernel_block(150, 120, 1.5, 2)
+    k3 = _kernel_block(120, 90, 3.0, 0)
+    k4 = _kernel_block(80, 70, 3.0, 1)
+    k5 = _kernel_block(64, 64, 1.2, 3)
+    near = _kernel_block(45, 33, 0.3, 4)  # close clusters: slow decay, higher ranks
+    cases += [(1, 0.3, 11, rnd), (1, 0.2, 12, k1), (1, 0.5, 13, k5)]
+    for eps in (1e-3, 1e-6, 1e-9):
+        cases.append((2, eps, 21, k3))
+    cases += [(2, 1e-6, 22, k1), (2, 1e-8, 23, k2), (2, 1e-4, 24, near), (2, 1e-10, 25, k5)]
+    cases += [(3, 1e-8, 31, k4), (3, 1e-6, 32, k1), (3, 1e-5, 33, near)]
+    # wavelets: smooth blocks of stretched (non-power-of-two) and 2^k sizes,
+    # and rows that are exactly constant after the transform's first level, so
+    # every interior coefficient of rows 2.. falls under the threshold and the
+    # last-column rule (kept, or set to 1e-14 when exactly zero) applies
+    v = rng.standard_normal(40)
+    const_rows = np.outer(np.ones(24), v)
+    mixed = np.outer(np.ones(33), rng.standard_normal(20))
+    mixed[::5] += 0.3 * np.outer(np.ones(7), np.sin(np.arange(20)))
+    cases += [(4, 0.05, 41, k1), (4, 0.1, 42, k2), (4, 0.02, 43, k5), (4, 0.5, 44, const_rows),
+              (4, 0.2, 45, mixed), (4, 0.0, 46, rnd)]
+    return cases
+
+
+def dump_compress():
+    objdir = os.path.join(REPO, "oracle", "_ref", "obj")
+    objs = sorted(os.path.join(objdir, f) for f in os.listdir(objdir)
+                  if f.endswith(".o") and f != "main.o")
+    work = tempfile.mkdtemp(prefix="tt_compress_")
+    exe = os.path.join(work, "dump_compress")
+    subprocess.run(["amdflang", "-O2", "-ffp-contract=off", "-I" + objdir, os.path.join(HERE, "dump_compress.f90")] +
+                   objs + ["-L/opt/conda/lib", "-lmpifort", "-lmpi", "-Wl,-rpath,/opt/conda/lib", "-o", exe],
+                   check=True)
+    cases = _compress_matrices()
+    with open(os.path.join(work, "compress_in.bin"), "wb") as f:
+        f.write(struct.pack("=i", len(cases)))
+        for kind, eps, seed, A in cases:
+            m, n = A.shape
+            f.write(struct.pack("=iiidi", kind, m, n, eps, seed))
+            f.write(np.asfortranarray(A, np.float64).tobytes(order="F"))
+    subprocess.run([exe], cwd=work, check=True, stdout=subprocess.DEVNULL)
+    b = open(os.path.join(work, "compress_out.bin"), "rb").read()
+    pos = 0
+
+    def take(dtype, count):
+        nonlocal pos
+        a = np.frombuffer(b, dtype, count, pos).copy()
+        pos += a.nbytes
+        return a
+
+    out = {"ncase": np.int32(len(cases))}
+    for i, (kind, eps, seed, A) in enumerate(cases):
+        m, n = A.shape
+        p = "c%02d_" % i
+        out.update({p + "kind": np.int32(kind), p + "eps": np.float64(eps), p + "A": np.asfortranarray(A)})
+        if kind <= 3:
+            r = int(take(np.int32, 1)[0])
+            out[p + "rank"] = np.int32(r)
+            out[p + "norm"] = take(np.float64, 1)[0]
+            out[p + "vec"] = take(np.float64, n)
+            out[p + "a"] = take(np.float64, m * r).reshape((m, r), order="F")
+            out[p + "b"] = take(np.float64, r * n).reshape((r, n), order="F")
+            out[p + "residual"] = take(np.float64, m * n).reshape((m, n), order="F")
+        else:
+            nnz, nrow, st = (int(x) for x in take(np.int32, 3))
+            out[p + "nnz"] = np.int32(nnz)
+            out[p + "zvr"] = take(np.int32, m + 1)
+            out[p + "c"] = take(np.int32, st)
+            out[p + "v"] = take(np.float64, st)
+            out[p + "norm"] = take(np.float64, 1)[0]
+            out[p + "vec"] = take(np.float64, n)
+    assert pos == len(b), (pos, len(b))
+    np.savez_compressed(os.path.join(HERE, "compress_ref.npz"), **out)
+    shutil.rmtree(work)
+    print("compress_ref ->", os.path.join(HERE, "compress_ref.npz"), len(cases), "cases")
+
+
 def run_case(name, spec, keep=None):
     geo_src = os.path.join(REFMESH, spec["geo"].replace("jit", "kva"))
     g = read_geo(geo_src)
@@ -400,11 +495,14 @@ def run_case(name, spec, keep=None):
 
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or ["gauss"] + list(CASES)
+    names = sys.argv[1:] or ["gauss", "compress"] + list(CASES)
     keep = os.environ.get("TT_GOLDEN_KEEP")
     if "gauss" in names:
         dump_gauss()
         names = [n for n in names if n != "gauss"]
+    if "compress" in names:
+        dump_compress()
+        names = [n for n in names if n != "compress"]
     for n in names:
         run = run_mh_case if "mh" in CASES[n] else run_case
         run(n, CASES[n], keep=os.path.join(keep, n) if keep else None)

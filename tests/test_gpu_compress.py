@@ -41,6 +41,44 @@ def test_aca_matches_oracle(t, eps):
         assert abs(h.norm - onorm) <= 1e-6 * max(onorm, 1e-300) + 1e-15
 
 
+def _ref_cases(kinds):
+    from conftest import load_golden
+    g = load_golden("compress_ref")
+    return g, [i for i in range(int(g["ncase"])) if int(g["c%02d_kind" % i]) in kinds]
+
+
+_CREF, _CACA = _ref_cases((1, 2, 3))
+_, _CWT = _ref_cases((4,))
+
+
+@pytest.mark.parametrize("i", _CACA)
+def test_aca_matches_reference_routines(i):
+    """The device cross approximation against the reference's own routines'
+    ranks and factors (compress_ref.npz)."""
+    from tritok_amd import compress as K
+    p = "c%02d_" % i
+    kind, eps, A = int(_CREF[p + "kind"]), float(_CREF[p + "eps"]), _CREF[p + "A"]
+    vec = _CREF[p + "vec"] if kind != 2 else None
+    with K.Aca(A, kind, eps, vec=vec) as h:
+        a, b = h.factors()
+        print("case", i, "type", kind, "rank", h.rank, int(_CREF[p + "rank"]), flush=True)
+        assert h.rank == int(_CREF[p + "rank"])
+        assert _rel(a, _CREF[p + "a"]) < 1e-13 and _rel(b, _CREF[p + "b"]) < 1e-13
+
+
+@pytest.mark.parametrize("i", _CWT)
+def test_wavelet_matches_reference_routines(i):
+    from tritok_amd import compress as K
+    p = "c%02d_" % i
+    A, kappa = _CREF[p + "A"], float(_CREF[p + "eps"])
+    with K.Wavelet(A, kappa) as h:
+        c = h.csr()
+        print("case", i, A.shape, kappa, "stored", len(c["v"]), len(_CREF[p + "v"]), flush=True)
+        assert np.array_equal(c["zvr"], _CREF[p + "zvr"]) and np.array_equal(c["c"], _CREF[p + "c"])
+        assert _rel(c["v"], _CREF[p + "v"]) < 1e-13
+        assert c["nnz_ref"] == int(_CREF[p + "nnz"])
+
+
 @pytest.mark.parametrize("m,n,kappa", [(150, 120, 1e-3), (64, 64, 0.0), (37, 90, 0.1), (1, 5, 1e-2)])
 def test_wavelet_matches_oracle(m, n, kappa):
     from tritok_amd import compress as K
