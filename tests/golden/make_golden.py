@@ -90,6 +90,18 @@ CASES = {
                  phys="REYN 1.0", idt=0, nnlit=10000, tight=True),
     "nc2t": dict(geo="kva2x2x2.geo", bic="NaturalConvection-X.bic", urdw=0.3, urdt=0.1,
                  phys="RAPR 1000.0 0.71\nREYN 1.0", idt=1, nnlit=10000, tight=True),
+    # the 8^3 cavity of dc8 at tight tolerances: converged fields fixed by the
+    # discrete equations, with many cell groups in the least-squares kernels
+    "dc8t": dict(geo="kva8x8x8.geo", bic="3Dldc-X.bic", urdw=0.1, urdt=0.1,
+                 phys="REYN 100.0", idt=0, nnlit=10000, tight=True),
+    # requests of the serving stream (workloads.serving_stream, seed 20251203)
+    # at its smallest size, 6^3 cells, on the generated cube mesh with the
+    # package's own .bic texts: the first cavity (Re = 239.13...) and the
+    # first convection case with Ra <= 1e5 (Ra = 5.47e4), tight tolerances
+    "sv6c": dict(cube=6, geo="kva6x6x6.geo", bic="cavity.bic", bic_gen="cavity", urdw=0.1, urdt=0.1,
+                 phys="REYN 239.131078715", idt=0, nnlit=10000, tight=True),
+    "sv6n": dict(cube=6, geo="kva6x6x6.geo", bic="heated.bic", bic_gen="heated", urdw=0.1, urdt=0.05,
+                 phys="RAPR 54746.529324582385 0.71\nREYN 1.0", idt=1, nnlit=10000, tight=True),
     # modified-Helmholtz macro integrals: uniform mu = 20 on the 2^3 cube,
     # mu = 1 + xyz on the jittered cube, uniform mu = 100 on the jittered cube
     "mh2u": dict(geo="kva2x2x2.geo", mh=(20.0, 1)),
@@ -440,16 +452,25 @@ def dump_compress():
 
 
 def run_case(name, spec, keep=None):
-    geo_src = os.path.join(REFMESH, spec["geo"].replace("jit", "kva"))
-    g = read_geo(geo_src)
     work = keep or tempfile.mkdtemp(prefix="tt_golden_%s_" % name)
     os.makedirs(work, exist_ok=True)
-    if "jitter" in spec:
-        g = jitter_corners(g, *spec["jitter"])
-        write_geo(os.path.join(work, spec["geo"]), g)
+    if "cube" in spec:  # the package's generated cube mesh and .bic text
+        sys.path.insert(0, REPO)
+        from tritok_amd import Host, workloads
+        Host.cube(spec["cube"]).write_geo(os.path.join(work, spec["geo"]))
+        g = read_geo(os.path.join(work, spec["geo"]))
+        text = workloads.bic_cavity() if spec["bic_gen"] == "cavity" else workloads.bic_heated()
+        with open(os.path.join(work, spec["bic"]), "w") as f:
+            f.write(text)
     else:
-        shutil.copy(geo_src, os.path.join(work, spec["geo"]))
-    shutil.copy(os.path.join(REFMESH, spec["bic"]), os.path.join(work, spec["bic"]))
+        geo_src = os.path.join(REFMESH, spec["geo"].replace("jit", "kva"))
+        g = read_geo(geo_src)
+        if "jitter" in spec:
+            g = jitter_corners(g, *spec["jitter"])
+            write_geo(os.path.join(work, spec["geo"]), g)
+        else:
+            shutil.copy(geo_src, os.path.join(work, spec["geo"]))
+        shutil.copy(os.path.join(REFMESH, spec["bic"]), os.path.join(work, spec["bic"]))
     inp = BASE_INP.format(**spec)
     if spec.get("tight"):
         inp = (inp.replace("RELA 10000 1.0 1.0D-6", "RELA 10000 1.0 1.0D-10")

@@ -6,7 +6,8 @@ fields too.  Parity bars:
   * tight cases (RELA eps 1e-10, LSQR eps 1e-12, fixed tolerance): converged
     fields within 1e-8 relative L2, in FP64 and in every reduced-precision
     mode (FP32 least-squares blocks; kinematics products in FP32, FP16 or BF16
-    anchored by FP64 steps);
+    anchored by FP64 steps) on the 2^3 cubes, in FP64 and mixed (FP16) on the
+    8^3 cavity and two 6^3 serving-stream requests;
   * default cases (RELA eps 1e-6, LSQR eps 1e-7 with the dynamic tolerance):
     the iteration trajectory follows the reference (same iteration count
     within 2, per-iteration changes within 5e-2 relative over the first ten
@@ -43,9 +44,15 @@ def _compare(g, f, tol, tag):
         assert e < tol, (tag, k, e)
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16", "bf16"])
-@pytest.mark.parametrize("case", ["dc2t", "nc2t"])
+TIGHT = ([(c, p) for c in ("dc2t", "nc2t") for p in ("fp64", "fp32", "fp16", "bf16")] +
+         [(c, p) for c in ("dc8t", "sv6c", "sv6n") for p in ("fp64", "fp16")])
+
+
+@pytest.mark.parametrize("case,prec", TIGHT)
 def test_tight_case_matches_reference(case, prec, gauss_table):
+    """dc8t: the 8^3 cavity (512 cells, 64 groups of the FP16 cell kernel);
+    sv6c / sv6n: a cavity and a convection request of the serving stream on
+    the generated 6^3 cube."""
     g = load_golden(case)
     print("case", case, prec, flush=True)
     h, c = _case(g, gauss_table, prec)
@@ -53,7 +60,8 @@ def test_tight_case_matches_reference(case, prec, gauss_table):
         sta, ite = c.run()
         f = c.fields()
     print(case, prec, "iterations", len(sta), "reference", len(g["sta"]), flush=True)
-    assert abs(len(sta) - len(g["sta"])) <= (3 if prec == "fp64" else 30)
+    nref = len(g["sta"])
+    assert abs(len(sta) - nref) <= (3 if prec == "fp64" else max(30, nref // 20))
     _compare(g, f, 1e-8, case + "/" + prec)
 
 

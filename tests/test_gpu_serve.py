@@ -1,9 +1,13 @@
 """The request loop: each case's converged fields are the same when it runs
 alone and when it runs concurrently with others (slots sharing the device),
 in FP64 and in mixed precision; admission accounting matches the device
-memory the cases hold."""
+memory the cases hold; two requests of the serving stream (6^3 cavity and
+convection, tight tolerances) match the reference's converged fields within
+1e-8 relative L2 whether served alone or among other stream requests."""
 import numpy as np
 import pytest
+
+from conftest import host_for, load_golden, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -59,3 +63,42 @@ def test_server_matches_direct_case_run():
     assert info["picard_iterations"] == len(sta)
     for k in FIELDS:
         assert np.array_equal(got[k], ref[k]), k
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_stream_requests_match_reference_alone_and_in_the_mix(prec):
+    from tritok_amd import workloads as W
+    from tritok_amd.serve import Server
+    fx = {k: load_golden(k) for k in ("sv6c", "sv6n")}
+    others = [W.serving_case(sp) for sp in W.serving_stream(count=60) if sp["n"] in (6, 8)][:4]
+    alone = {}
+    with Server(slots=1) as s:
+        for k, g in fx.items():
+            print("alone", k, prec, flush=True)
+            rid = s.submit(host_for(g), prec)
+            info = s.wait(rid)
+            assert info["status"] == 0, info
+            alone[k] = s.fields(rid)
+    hosts = {k: host_for(g) for k, g in fx.items()}
+    order = [("sv6c", hosts["sv6c"]), (0, others[0]), (1, others[1]), ("sv6n", hosts["sv6n"]), (2, others[2]),
+             (3, others[3])]
+    with Server(slots=6) as s:
+        print("mix", prec, flush=True)
+        ids = {k: s.submit(h, prec) for k, h in order}
+        infos = {k: s.wait(i) for k, i in ids.items()}
+        mixed = {k: s.fields(ids[k]) for k in fx}
+    for k in fx:  # each fixture request ran while others were on the device
+        a = infos[k]
+        over = [j for j, b in infos.items() if j != k and b["t_start"] < a["t_end"] and a["t_start"] < b["t_end"]]
+        assert len(over) >= 2, (k, over)
+    for k, g in fx.items():
+        assert infos[k]["status"] == 0, infos[k]
+        for f in FIELDS:
+            assert np.array_equal(mixed[k][f], alone[k][f]), (k, prec, f)
+            ref = g["rst_" + f].reshape(-1)
+            if np.linalg.norm(ref) == 0:
+                assert np.abs(mixed[k][f]).max() < 1e-12, (k, f)
+                continue
+            e = rel_l2(np.asarray(mixed[k][f]).reshape(-1), ref)
+            print(k, prec, f, "vs reference rel L2", e, flush=True)
+            assert e < 1e-8, (k, prec, f, e)
