@@ -220,25 +220,33 @@ def batched_solves_leg(count, seed=20251203):
     t0 = time.perf_counter()
     B = S.LsqrBatch(systems)
     setup = time.perf_counter() - t0
-    B.solve(bs, x0, 5000, 1e-7)  # warm-up
-    xs, istop, nit, ms = B.solve(bs, x0, 5000, 1e-7)
-    B.close()
     m = np.array([len(s_[0]) - 1 for s_ in systems], float)
     nnz = np.array([len(s_[1]) for s_ in systems], float)
     nunk = np.array([s_[3] for s_ in systems], float)
-    alg = float(np.sum(nit * (2 * (12 * nnz + 4 * (m + 1)) + 8 * (3 * m + 6 * nunk))))
-    gbs = alg / (ms * 1e-3) / 1e9
-    log("batched solves: %d systems (%.3g nonzeros) in %.1f ms, %d..%d iterations, %.0f GB/s algorithmic" % (
-        count, nnz.sum(), ms, nit.min(), nit.max(), gbs))
-    return {"workload": "%d lid-driven-cavity vorticity systems, n^3 cells with n uniform in 2..10, Re uniform in "
-                        "[10, 400], synthetic right-hand sides, LSQR eps 1e-7" % count,
-            "systems": count, "nonzeros": float(nnz.sum()), "kernel_ms": ms, "setup_s": setup,
-            "systems_per_s": count / (ms * 1e-3), "iterations": {"min": int(nit.min()), "mean": float(nit.mean()),
-                                                                  "max": int(nit.max())},
-            "istop": {int(k): int(v) for k, v in zip(*np.unique(istop, return_counts=True))},
-            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS,
-                         "bytes": "per system iteration 2*((8+4)*nnz + 4*(m+1)) + 8*(3m + 6n)"}}
+    out = {"workload": "%d lid-driven-cavity vorticity systems, n^3 cells with n uniform in 2..10, Re uniform in "
+                       "[10, 400], synthetic right-hand sides, LSQR eps 1e-7, FP64 and FP32 values" % count,
+           "systems": count, "nonzeros": float(nnz.sum()), "setup_s": setup}
+    x64 = None
+    for prec, vb in (("fp64", 8), ("fp32", 4)):
+        B.set_precision(prec)
+        B.solve(bs, x0, 5000, 1e-7)  # warm-up (FP32: makes the value copies)
+        xs, istop, nit, ms = B.solve(bs, x0, 5000, 1e-7)
+        alg = float(np.sum(nit * (2 * ((vb + 4) * nnz + 4 * (m + 1)) + 8 * (3 * m + 6 * nunk))))
+        gbs = alg / (ms * 1e-3) / 1e9
+        if prec == "fp64":
+            x64 = xs
+        err = max(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)) for a, b in zip(xs, x64))
+        log("batched solves %s: %d systems (%.3g nonzeros) in %.1f ms, %d..%d iterations, %.0f GB/s algorithmic, "
+            "max rel diff vs FP64 %.2e" % (prec, count, nnz.sum(), ms, nit.min(), nit.max(), gbs, err))
+        out[prec] = {"kernel_ms": ms, "systems_per_s": count / (ms * 1e-3),
+                     "iterations": {"min": int(nit.min()), "mean": float(nit.mean()), "max": int(nit.max())},
+                     "istop": {int(k): int(v) for k, v in zip(*np.unique(istop, return_counts=True))},
+                     "max_rel_l2_vs_fp64": err,
+                     "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": gbs / HBM_PEAK_GBS,
+                                  "bytes": "per system iteration 2*((%d+4)*nnz + 4*(m+1)) + 8*(3m + 6n)" % vb}}
+    B.close()
+    return out
 
 
 def mixed_kinematics_leg(n, tol=1e-10):

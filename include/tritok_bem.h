@@ -149,6 +149,14 @@ int tt_lsqr_batch_solve(tt_lsqr_batch *h, const double *b, double *x, int32_t ma
  * be NULL) = the widest gang of the new schedule.  Results depend on the
  * width only through the order of the parallel sums. */
 int tt_lsqr_batch_set_width(tt_lsqr_batch *h, int32_t width, int32_t *max_width);
+/* Values the batched LSQR products read (no reference counterpart; the
+ * reference computes in FP64): TT_PREC_FP64 (default) or TT_PREC_FP32, FP32
+ * copies of the scaled values (made on first use) multiplied into FP64
+ * vectors with FP64 sums.  The hot-start residual b - A x0, the LSQR scalars,
+ * norms and stopping tests stay FP64, so a solve differs from the FP64 one by
+ * the values' rounding (about 6e-8 relative per entry) times the system's
+ * conditioning. */
+int tt_lsqr_batch_set_precision(tt_lsqr_batch *h, int32_t prec);
 void tt_lsqr_batch_free(tt_lsqr_batch *h);
 
 /* Replaces CRSxV (lib_util.f:1351-1372): y = A x on the device for a CSR

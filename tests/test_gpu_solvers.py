@@ -249,3 +249,43 @@ def test_lsqr_batch_matches_oracle_per_system():
         assert all(np.array_equal(u, w) for u, w in zip(xg, xg2)) and np.array_equal(nit_g, nit_g2)
     assert B.set_width(0) >= 1
     B.close()
+
+
+def test_lsqr_batch_fp32_values_match_oracle_on_rounded_values():
+    """FP32 values in the batched products: from x0 = 0 the hot-start
+    residual is b in both, so each solve is the oracle's slvlsqr2 on the
+    scaled values rounded to FP32 (the same iteration to the order of the
+    sums: counts within 2, solutions within 1e-8 at eps 1e-10); against the
+    FP64 solve the difference is the rounding of the values (reported,
+    within 1e-4)."""
+    from tritok_amd import solvers as S
+    systems = [_vte_system(load_golden("dc2")), _vte_system(load_golden("jit2"), e=1)]
+    for m, n, k, seed in [(3000, 1000, 11, 3), (2500, 1200, 9, 6)]:
+        ia, ja, a = _random_csr(m, n, k, seed)
+        systems.append((ia, ja, a, n))
+    rng = np.random.default_rng(19)
+    bs = [rng.standard_normal(len(s[0]) - 1) for s in systems]
+    x0s = [np.zeros(s[3]) for s in systems]
+    B = S.LsqrBatch(systems)
+    x64, _, _, _ = B.solve(bs, x0s, 5000, 1e-10)
+    B.set_precision("fp32")
+    for eps in (1e-7, 1e-10):
+        xs, istop, nit, ms = B.solve(bs, x0s, 5000, eps)
+        print("fp32 batch eps", eps, "device ms", ms, flush=True)
+        for k, (ia, ja, a, n) in enumerate(systems):
+            m = len(ia) - 1
+            ao = np.array(a, np.float64)
+            q = O.prelsqr2(m, n, ia, ja, ao)
+            a32 = ao.astype(np.float32).astype(np.float64)
+            x_ref, istop_ref, nit_ref = O.slvlsqr2(m, n, ia, ja, a32, q, bs[k], x0s[k], 5000, eps)
+            e = rel_l2(xs[k], x_ref)
+            print(k, "nit", nit[k], nit_ref, "istop", istop[k], istop_ref, "rel vs rounded-value oracle", e,
+                  "vs fp64 batch", rel_l2(xs[k], x64[k]), flush=True)
+            assert istop[k] == istop_ref and abs(nit[k] - nit_ref) <= 2
+            assert e < (1e-8 if eps < 1e-9 else 1e-5)
+            if eps < 1e-9:
+                assert rel_l2(xs[k], x64[k]) < 1e-4
+    B.set_precision("fp64")
+    x64b, _, _, _ = B.solve(bs, x0s, 5000, 1e-10)
+    assert all(np.array_equal(u, w) for u, w in zip(x64, x64b))
+    B.close()

@@ -404,6 +404,87 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One cell of the reduced-precision cell kernel, its block and v already in
+// this wavefront's LDS: u' = A v - alpha u (stored, returned) and A^T u'
+// (stored to P).
+template <typename T, int R, int L>
+__device__ __forceinline__ double aw_compute(int t, int c, const T *blk, const float *vsf, float *uuf, double uo,
+                                             double alpha, double beta, double *__restrict__ u,
+                                             double *__restrict__ P) {
+  double un = 0.0;
+  {
+    double s = 0.0;
+    const int row = t < R ? t : R - 1;
+    if constexpr (sizeof(T) == 2) {
+      constexpr int LP = row_len<T, L>();
+      const uint2 *rp = reinterpret_cast<const uint2 *>(blk + row * LP);
+      float f0 = 0.f, f1 = 0.f;
+#pragma unroll 7  // a full unroll holds every LDS read in flight and spills
+      for (int q = 0; q < LP / 4; ++q) {
+        const uint2 cw = rp[q];
+        const float4 va = *reinterpret_cast<const float4 *>(vsf + 4 * q);
+        const _Float16 *h = reinterpret_cast<const _Float16 *>(&cw);
+        f0 += (float)h[0] * va.x;
+        f1 += (float)h[1] * va.y;
+        f0 += (float)h[2] * va.z;
+        f1 += (float)h[3] * va.w;
+      }
+      s = (double)f0 + (double)f1;
+    } else {
+      float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+      for (int l = 0; l + 1 < L; l += 2) {
+        f0 += (float)blk[row * L + l] * vsf[l];
+        f1 += (float)blk[row * L + l + 1] * vsf[l + 1];
+      }
+      if (L & 1) f0 += (float)blk[row * L + L - 1] * vsf[L - 1];
+      s = (double)f0 + (double)f1;
+    }
+    const double uold = beta > 0.0 ? uo / beta : uo;
+    if (t < R) {
+      un = -alpha * uold + s;
+      u[(size_t)c * R + t] = un;
+    }
+  }
+  uuf[t] = (float)un;
+  wave_sync();
+  if constexpr (sizeof(T) == 2) {
+    constexpr int LP = row_len<T, L>(), NPAIR = (L + 1) / 2, RH = (R + 1) / 2;
+    const int p = t & 31, hf = t >> 5, i0 = hf ? RH : 0;
+    const int pc = p < NPAIR ? p : 0;
+    float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+    for (int q = 0; q < RH; ++q) {
+      const int i = i0 + q;
+      if (i < R) {
+        const unsigned int pw = *reinterpret_cast<const unsigned int *>(blk + i * LP + 2 * pc);
+        const _Float16 *h = reinterpret_cast<const _Float16 *>(&pw);
+        const float ui = uuf[i];
+        s0 += (float)h[0] * ui;
+        s1 += (float)h[1] * ui;
+      }
+    }
+    s0 += __shfl_xor(s0, 32, 64);
+    s1 += __shfl_xor(s1, 32, 64);
+    if (hf == 0 && p < NPAIR) {
+      float *Pf = reinterpret_cast<float *>(P);
+      Pf[(size_t)c * L + 2 * p] = s0;
+      if (2 * p + 1 < L) Pf[(size_t)c * L + 2 * p + 1] = s1;
+    }
+  } else {
+    const int col = t < L ? t : L - 1;
+    float f0 = 0.f, f1 = 0.f;
+#pragma unroll
+    for (int i = 0; i + 1 < R; i += 2) {
+      f0 += (float)blk[i * L + col] * uuf[i];
+      f1 += (float)blk[(i + 1) * L + col] * uuf[i + 1];
+    }
+    if (R & 1) f0 += (float)blk[(R - 1) * L + col] * uuf[R - 1];
+    if (t < L) P[(size_t)c * L + t] = (double)f0 + (double)f1;
+  }
+  return un;
+}
+
 // The same cell work on reduced-precision blocks with NW wavefronts per
 // workgroup.  A workgroup takes a contiguous range of whole groups of cgroup
 // cells (a multiple of NW); wavefront w takes cells cb + w, cb + w + NW, ...
@@ -461,109 +542,8 @@ __global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(int ncell, int cgroup, con
   int c = g0 * cgroup + w;
   if (c >= gend(g0)) c = -1;
   int cn = c >= 0 ? next_cell(c) : -1;
-  int jn = -1;
-  double uo = 0.0, vj = 0.0;
-  RegBlock<T, S> rb;
-  if (c >= 0) {
-    const int j = t < L ? colmap[(size_t)c * L + t] : -1;
-    jn = (cn >= 0 && t < L) ? colmap[(size_t)cn * L + t] : -1;
-    uo = t < R ? u[(size_t)c * R + t] : 0.0;
-    rb.load(A + (size_t)c * S);
-    vj = j >= 0 ? vp[j] : 0.0;
-  }
-  const double beta = sin->beta, alpha_prev = sin->alpha;
-  const double nvs = reduce_partials<64 * NW>(pb1, nb, sh);
-  const double alpha = (k == 1 || beta > 0.0) ? sqrt(nvs) : alpha_prev;
-  for (int g = g0; g < g1; ++g) {
-    double acc = 0.0;
-    while (c >= 0 && c < gend(g)) {
-      rb.store(blk);
-      vsf[t] = (float)(t < L ? (alpha > 0.0 ? vj / alpha : vj) : 0.0);
-      wave_sync();
-      const int cnn = cn >= 0 ? next_cell(cn) : -1;
-      double vjn = 0.0, uon = 0.0;
-      int jnn = -1;
-      if (cn >= 0) {
-        vjn = jn >= 0 ? vp[jn] : 0.0;
-        uon = t < R ? u[(size_t)cn * R + t] : 0.0;
-        rb.load(A + (size_t)cn * S);
-        if (cnn >= 0 && t < L) jnn = colmap[(size_t)cnn * L + t];
-      }
-      double un = 0.0;
-      {
-        double s = 0.0;
-        const int row = t < R ? t : R - 1;
-        if constexpr (sizeof(T) == 2) {
-          constexpr int LP = row_len<T, L>();
-          const uint2 *rp = reinterpret_cast<const uint2 *>(blk + row * LP);
-          float f0 = 0.f, f1 = 0.f;
-#pragma unroll
-          for (int q = 0; q < LP / 4; ++q) {
-            const uint2 cw = rp[q];
-            const float4 va = *reinterpret_cast<const float4 *>(vsf + 4 * q);
-            const _Float16 *h = reinterpret_cast<const _Float16 *>(&cw);
-            f0 += (float)h[0] * va.x;
-            f1 += (float)h[1] * va.y;
-            f0 += (float)h[2] * va.z;
-            f1 += (float)h[3] * va.w;
-          }
-          s = (double)f0 + (double)f1;
-        } else {
-          float f0 = 0.f, f1 = 0.f;
-#pragma unroll
-          for (int l = 0; l + 1 < L; l += 2) {
-            f0 += (float)blk[row * L + l] * vsf[l];
-            f1 += (float)blk[row * L + l + 1] * vsf[l + 1];
-          }
-          if (L & 1) f0 += (float)blk[row * L + L - 1] * vsf[L - 1];
-          s = (double)f0 + (double)f1;
-        }
-        const double uold = beta > 0.0 ? uo / beta : uo;
-        if (t < R) {
-          un = -alpha * uold + s;
-          u[(size_t)c * R + t] = un;
-        }
-      }
-      uuf[t] = (float)un;
-      wave_sync();
-      if constexpr (sizeof(T) == 2) {
-        constexpr int LP = row_len<T, L>(), NPAIR = (L + 1) / 2, RH = (R + 1) / 2;
-        const int p = t & 31, hf = t >> 5, i0 = hf ? RH : 0;
-        const int pc = p < NPAIR ? p : 0;
-        float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-        for (int q = 0; q < RH; ++q) {
-          const int i = i0 + q;
-          if (i < R) {
-            const unsigned int pw = *reinterpret_cast<const unsigned int *>(blk + i * LP + 2 * pc);
-            const _Float16 *h = reinterpret_cast<const _Float16 *>(&pw);
-            const float ui = uuf[i];
-            s0 += (float)h[0] * ui;
-            s1 += (float)h[1] * ui;
-          }
-        }
-        s0 += __shfl_xor(s0, 32, 64);
-        s1 += __shfl_xor(s1, 32, 64);
-        if (hf == 0 && p < NPAIR) {
-          float *Pf = reinterpret_cast<float *>(P);
-          Pf[(size_t)c * L + 2 * p] = s0;
-          if (2 * p + 1 < L) Pf[(size_t)c * L + 2 * p + 1] = s1;
-        }
-      } else {
-        const int col = t < L ? t : L - 1;
-        float f0 = 0.f, f1 = 0.f;
-#pragma unroll
-        for (int i = 0; i + 1 < R; i += 2) {
-          f0 += (float)blk[i * L + col] * uuf[i];
-          f1 += (float)blk[(i + 1) * L + col] * uuf[i + 1];
-        }
-        if (R & 1) f0 += (float)blk[(R - 1) * L + col] * uuf[R - 1];
-        if (t < L) P[(size_t)c * L + t] = (double)f0 + (double)f1;
-      }
-      acc += un * un;
-      c = cn, cn = cnn, vj = vjn, jn = jnn, uo = uon;
-      wave_sync();  // blk, vsf and uuf are rewritten next
-    }
+  // the wavefronts' ||u'||^2 sums of group g, added in wavefront order
+  auto group_partial = [&](int g, double acc) {
     const double ws = wave_sum(acc);
     if (t == 0) gp[w] = ws;
     __syncthreads();
@@ -574,6 +554,43 @@ __global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(int ncell, int cgroup, con
       pa1[g] = sg;
     }
     __syncthreads();
+  };
+  {
+    int jn = -1;
+    double uo = 0.0, vj = 0.0;
+    RegBlock<T, S> rb;
+    if (c >= 0) {
+      const int j = t < L ? colmap[(size_t)c * L + t] : -1;
+      jn = (cn >= 0 && t < L) ? colmap[(size_t)cn * L + t] : -1;
+      uo = t < R ? u[(size_t)c * R + t] : 0.0;
+      rb.load(A + (size_t)c * S);
+      vj = j >= 0 ? vp[j] : 0.0;
+    }
+    const double beta = sin->beta, alpha_prev = sin->alpha;
+    const double nvs = reduce_partials<64 * NW>(pb1, nb, sh);
+    const double alpha = (k == 1 || beta > 0.0) ? sqrt(nvs) : alpha_prev;
+    for (int g = g0; g < g1; ++g) {
+      double acc = 0.0;
+      while (c >= 0 && c < gend(g)) {
+        rb.store(blk);
+        vsf[t] = (float)(t < L ? (alpha > 0.0 ? vj / alpha : vj) : 0.0);
+        wave_sync();
+        const int cnn = cn >= 0 ? next_cell(cn) : -1;
+        double vjn = 0.0, uon = 0.0;
+        int jnn = -1;
+        if (cn >= 0) {
+          vjn = jn >= 0 ? vp[jn] : 0.0;
+          uon = t < R ? u[(size_t)cn * R + t] : 0.0;
+          rb.load(A + (size_t)cn * S);
+          if (cnn >= 0 && t < L) jnn = colmap[(size_t)cnn * L + t];
+        }
+        const double un = aw_compute<T, R, L>(t, c, blk, vsf, uuf, uo, alpha, beta, u, P);
+        acc += un * un;
+        c = cn, cn = cnn, vj = vjn, jn = jnn, uo = uon;
+        wave_sync();  // blk, vsf and uuf are rewritten next
+      }
+      group_partial(g, acc);
+    }
   }
 }
 

@@ -123,16 +123,19 @@ struct Gang {
   }
 };
 
-// u <- A v + c u (c may be 0) on this member's rows, returns ||u||^2 over the gang
+// u <- A v + c u (c may be 0) on this member's rows, returns ||u||^2 over the
+// gang; VT = the stored values' type (FP32 values are widened, products and
+// sums in FP64)
+template <typename VT>
 __device__ double rows_apply(Gang &gg, const SysDesc &d, const int *__restrict__ rp, const int *__restrict__ ja,
-                             const double *__restrict__ a, const double *__restrict__ v, double c,
+                             const VT *__restrict__ a, const double *__restrict__ v, double c,
                              double *__restrict__ u, double *sh) {
   const int G = kNT / kLanes, g = gg.rank * G + threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
   const int stride = gg.width * G;
   double acc = 0.0;
   for (int r = g; r < d.m; r += stride) {
     double s = 0.0;
-    for (int e = rp[r] + lane; e < rp[r + 1]; e += kLanes) s += a[e] * v[ja[e]];
+    for (int e = rp[r] + lane; e < rp[r + 1]; e += kLanes) s += (double)a[e] * v[ja[e]];
     s = lane_group_sum(s);
     const double un = c != 0.0 ? s + c * u[r] : s;
     if (lane == 0) {
@@ -144,15 +147,16 @@ __device__ double rows_apply(Gang &gg, const SysDesc &d, const int *__restrict__
 }
 
 // v <- A^T u + c v on this member's columns (CSC), returns ||v||^2 over the gang
+template <typename VT>
 __device__ double cols_apply(Gang &gg, const SysDesc &d, const int *__restrict__ cp, const int *__restrict__ ri,
-                             const double *__restrict__ at, const double *__restrict__ u, double c,
+                             const VT *__restrict__ at, const double *__restrict__ u, double c,
                              double *__restrict__ v, double *sh) {
   const int G = kNT / kLanes, g = gg.rank * G + threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
   const int stride = gg.width * G;
   double acc = 0.0;
   for (int j = g; j < d.n; j += stride) {
     double s = 0.0;
-    for (int e = cp[j] + lane; e < cp[j + 1]; e += kLanes) s += at[e] * u[ri[e]];
+    for (int e = cp[j] + lane; e < cp[j + 1]; e += kLanes) s += (double)at[e] * u[ri[e]];
     s = lane_group_sum(s);
     const double vn = c != 0.0 ? s + c * v[j] : s;
     if (lane == 0) {
@@ -174,11 +178,16 @@ __device__ void scale_own(Gang &gg, int len, double f, double *__restrict__ x) {
   TT_OWN(gg, i, len) x[i] *= f;
 }
 
+// VT: type of the values the LSQR products read (double, or float copies of
+// the scaled values); the hot-start residual b - A x0 always uses the FP64
+// values (ap), and every vector, scalar and sum is FP64.
+template <typename VT>
 __global__ void __launch_bounds__(kNT) lsqr_batch_kernel(const int *__restrict__ aptr, const Assign *__restrict__ asg,
                                                          const SysDesc *__restrict__ sys, const int *__restrict__ rpp,
                                                          const int *__restrict__ jap, const double *__restrict__ ap,
-                                                         const int *__restrict__ cpp, const int *__restrict__ rip,
-                                                         const double *__restrict__ atp, const double *__restrict__ qp,
+                                                         const VT *__restrict__ alo, const int *__restrict__ cpp,
+                                                         const int *__restrict__ rip, const VT *__restrict__ atlo,
+                                                         const double *__restrict__ qp,
                                                          const double *__restrict__ bp, double *__restrict__ xp,
                                                          double *__restrict__ up, double *__restrict__ vp,
                                                          double *__restrict__ wp, double *__restrict__ dxp, int maxit,
@@ -194,7 +203,8 @@ __global__ void __launch_bounds__(kNT) lsqr_batch_kernel(const int *__restrict__
     const int s = as.sys;
     const SysDesc d = sys[s];
     const int *rp = rpp + d.rp, *ja = jap + d.e0, *cp = cpp + d.cp, *ri = rip + d.e0;
-    const double *a = ap + d.e0, *at = atp + d.e0, *q = qp + d.no, *b = bp + d.mo;
+    const double *a = ap + d.e0, *q = qp + d.no, *b = bp + d.mo;
+    const VT *al = alo + d.e0, *at = atlo + d.e0;
     double *x = xp + d.no, *u = up + d.mo, *v = vp + d.no, *w = wp + d.no, *dx = dxp + d.no;
     const int itnlim = maxit > 0 ? maxit : 10 * (d.m + d.n + 50);
     // hot start: u = b - A (x0 / q)
@@ -230,7 +240,7 @@ __global__ void __launch_bounds__(kNT) lsqr_batch_kernel(const int *__restrict__
       const double bnorm = beta;
       for (;;) {
         ++itn;
-        beta = sqrt(rows_apply(gg, d, rp, ja, a, v, -alpha, u, sh));
+        beta = sqrt(rows_apply(gg, d, rp, ja, al, v, -alpha, u, sh));
         const double temp = d2n(d2n(alpha, beta), 0.0);
         anorm = d2n(anorm, temp);
         if (beta > 0.0) {
@@ -308,6 +318,8 @@ struct tt_lsqr_batch {
   DBuf<SysDesc> dsys;
   DBuf<int> rp, ja, cp, ri;
   DBuf<double> a, at, q, b, x, u, v, w, dx;
+  DBuf<float> a32, at32;  // FP32 copies of the scaled values (TT_PREC_FP32)
+  int prec = TT_PREC_FP64;
   DBuf<int> nit, istop;
   // schedule: per workgroup its assignments aptr[g]..aptr[g+1] in global order
   int grid = 0, force_width = 0, max_width = 0;
@@ -324,13 +336,20 @@ struct tt_lsqr_batch {
 
 namespace {
 
-// Workgroups of lsqr_batch_kernel resident on the device at once.
+// Workgroups of lsqr_batch_kernel resident on the device at once (the fewer
+// of the FP64 and FP32 instantiations, so a schedule fits either).
 int resident_grid() {
-  int dev = 0, cus = 0, per_cu = 0;
+  int dev = 0, cus = 0, p64 = 0, p32 = 0;
   TT_HIP(hipGetDevice(&dev));
   TT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lsqr_batch_kernel, kNT, 0));
-  return std::max(1, per_cu) * cus;
+  TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p64, lsqr_batch_kernel<double>, kNT, 0));
+  TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p32, lsqr_batch_kernel<float>, kNT, 0));
+  return std::max(1, std::min(p64, p32)) * cus;
+}
+
+__global__ void narrow_kernel(long n, const double *__restrict__ a, float *__restrict__ b) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) b[k] = (float)a[k];
 }
 
 // List scheduling of the systems onto gangs of workgroups: systems in
@@ -473,9 +492,16 @@ extern "C" int tt_lsqr_batch_solve(tt_lsqr_batch *h, const double *b, double *x,
     TT_HIP(hipEventCreate(&e1));
     TT_HIP(hipEventRecord(e0, h->s));
     // grid <= the workgroups resident at once: the members of a gang run together
-    hipLaunchKernelGGL(lsqr_batch_kernel, dim3(h->grid), dim3(kNT), 0, h->s, h->aptr.p, h->asg.p, h->dsys.p, h->rp.p,
-                       h->ja.p, h->a.p, h->cp.p, h->ri.p, h->at.p, h->q.p, h->b.p, h->x.p, h->u.p, h->v.p, h->w.p,
-                       h->dx.p, maxit, (double)eps4, h->gcnt.p, h->gpart.p, h->err.p, h->nit.p, h->istop.p);
+    if (h->prec == TT_PREC_FP32)
+      hipLaunchKernelGGL(lsqr_batch_kernel<float>, dim3(h->grid), dim3(kNT), 0, h->s, h->aptr.p, h->asg.p, h->dsys.p,
+                         h->rp.p, h->ja.p, h->a.p, h->a32.p, h->cp.p, h->ri.p, h->at32.p, h->q.p, h->b.p, h->x.p,
+                         h->u.p, h->v.p, h->w.p, h->dx.p, maxit, (double)eps4, h->gcnt.p, h->gpart.p, h->err.p,
+                         h->nit.p, h->istop.p);
+    else
+      hipLaunchKernelGGL(lsqr_batch_kernel<double>, dim3(h->grid), dim3(kNT), 0, h->s, h->aptr.p, h->asg.p, h->dsys.p,
+                         h->rp.p, h->ja.p, h->a.p, h->a.p, h->cp.p, h->ri.p, h->at.p, h->q.p, h->b.p, h->x.p, h->u.p,
+                         h->v.p, h->w.p, h->dx.p, maxit, (double)eps4, h->gcnt.p, h->gpart.p, h->err.p, h->nit.p,
+                         h->istop.p);
     TT_LAUNCH_CHECK();
     TT_HIP(hipEventRecord(e1, h->s));
     h->x.download(x, h->ntot, h->s);
@@ -500,6 +526,24 @@ extern "C" int tt_lsqr_batch_set_width(tt_lsqr_batch *h, int32_t width, int32_t 
     h->grid = resident_grid();
     make_schedule(*h);
     if (max_width) *max_width = h->max_width;
+    return 0;
+  });
+}
+
+extern "C" int tt_lsqr_batch_set_precision(tt_lsqr_batch *h, int32_t prec) {
+  return guarded("tt_lsqr_batch_set_precision", [&] {
+    if (!h) throw std::runtime_error("NULL handle");
+    if (prec != TT_PREC_FP64 && prec != TT_PREC_FP32)
+      throw std::runtime_error("batched solves take TT_PREC_FP64 or TT_PREC_FP32 values");
+    if (prec == TT_PREC_FP32 && !h->a32.p) {
+      h->a32.alloc(h->etot), h->at32.alloc(h->etot);
+      const unsigned g = ceil_div(h->etot, 256);
+      hipLaunchKernelGGL(narrow_kernel, dim3(g), dim3(256), 0, h->s, h->etot, h->a.p, h->a32.p);
+      hipLaunchKernelGGL(narrow_kernel, dim3(g), dim3(256), 0, h->s, h->etot, h->at.p, h->at32.p);
+      TT_LAUNCH_CHECK();
+      TT_HIP(hipStreamSynchronize(h->s));
+    }
+    h->prec = prec;
     return 0;
   });
 }

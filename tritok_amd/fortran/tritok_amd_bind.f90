@@ -99,6 +99,15 @@ MODULE tritok_amd_bind
       INTEGER(C_INT32_T) :: nit, istop
       INTEGER(C_INT) :: tt_lsqr_solve
     END FUNCTION
+    ! the same solve on device vectors, allocating nothing (b_dev, x_dev: device addresses)
+    FUNCTION tt_lsqr_solve_noalloc(h, b_dev, x_dev, maxit, eps4, nit, istop) BIND(C, NAME="tt_lsqr_solve_noalloc")
+      IMPORT :: C_INT, C_INT32_T, C_FLOAT, C_PTR
+      TYPE(C_PTR), VALUE :: h, b_dev, x_dev
+      INTEGER(C_INT32_T), VALUE :: maxit
+      REAL(C_FLOAT), VALUE :: eps4
+      INTEGER(C_INT32_T) :: nit, istop
+      INTEGER(C_INT) :: tt_lsqr_solve_noalloc
+    END FUNCTION
     SUBROUTINE tt_lsqr_free(h) BIND(C, NAME="tt_lsqr_free")
       IMPORT :: C_PTR
       TYPE(C_PTR), VALUE :: h
@@ -331,6 +340,12 @@ MODULE tritok_amd_bind
       REAL(C_FLOAT), VALUE :: eps4
       INTEGER(C_INT32_T) :: nit(*), istop(*)
       INTEGER(C_INT) :: tt_lsqr_batch_solve
+    END FUNCTION
+    FUNCTION tt_lsqr_batch_set_precision(h, prec) BIND(C, NAME="tt_lsqr_batch_set_precision")
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: prec
+      INTEGER(C_INT) :: tt_lsqr_batch_set_precision
     END FUNCTION
     SUBROUTINE tt_lsqr_batch_free(h) BIND(C, NAME="tt_lsqr_batch_free")
       IMPORT :: C_PTR

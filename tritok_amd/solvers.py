@@ -89,6 +89,7 @@ _bsetup = _lib._sig("tt_lsqr_batch_setup", i32, i32, PI, PI, C.POINTER(C.c_int64
                     C.POINTER(P))
 _bsolve = _lib._sig("tt_lsqr_batch_solve", i32, P, PD, PD, i32, C.c_float, PI, PI, C.POINTER(C.c_double))
 _bwidth = _lib._sig("tt_lsqr_batch_set_width", i32, P, i32, PI)
+_bprec = _lib._sig("tt_lsqr_batch_set_precision", i32, P, i32)
 _bfree = _lib._sig("tt_lsqr_batch_free", None, P)
 
 
@@ -119,6 +120,10 @@ class LsqrBatch:
         mw = C.c_int32()
         check(_bwidth(self._h, int(width), C.byref(mw)), "tt_lsqr_batch_set_width")
         return mw.value
+
+    def set_precision(self, prec):
+        """Values the LSQR products read: "fp64" or "fp32"."""
+        check(_bprec(self._h, SD_PREC[prec]), "tt_lsqr_batch_set_precision")
 
     def solve(self, b, x0, maxit, eps4):
         """b, x0: lists (or concatenated arrays).  Returns (xs, istop, nit, ms)."""
