@@ -394,6 +394,11 @@ int tt_case_fields(tt_case *c, double *vel, double *vort, double *qvort, double 
  * [6] whole step; and setup times (ms): [7] macro integrals, [8] sdkm
  * integrals, [9] LU factorization and inversion, [10] system build. */
 int tt_case_timings(tt_case *c, double t[11]);
+/* WrstF (lib_IO.f:604-685) for the case's current state: velocity,
+ * vorticity, qvorticity, temp, qtemp; the previous Picard iterate as the pnls
+ * fields; the initial fields of the (single) time step as the pts and ppts
+ * fields; zero DC fields (not on this path). */
+int tt_case_write_restart(tt_case *c, const char *path, int32_t tstep, int32_t glnnlit, double rtime);
 /* HBM bytes held by the case on the device. */
 int64_t tt_case_device_bytes(tt_case *c);
 void tt_case_free(tt_case *c);
@@ -427,6 +432,55 @@ int tt_host_fields(tt_host *h, const double **vel, const double **vort, const do
 /* Writes the mesh as a GiD .geo file the reference can read. */
 int tt_host_write_geo(tt_host *h, const char *path);
 void tt_host_free(tt_host *h);
+
+/* ------------------------------------------------------------------------ */
+/* The reference's files (host only, no device call): integral caches,
+ * restart and iteration history, so that matrices and fields computed here
+ * drop into an unmodified Fortran run and back.  Binary files are Fortran
+ * unformatted sequential files (4-byte record lengths around each record),
+ * vectors in records of 8192 doubles as wrvec writes them (lib_int.f:
+ * 3231-3252), matrices column by column (WrSMat :3174-3186).               */
+
+/* WriteMacroIntDisk (lib_int.f:1341-1391): tri.macro.int from the arrays
+ * tt_macro_integrals returns (nicell*51 or nicell*27 rows, column-major),
+ * INTversion 1 and the header nnodes, nbnodes. */
+int tt_write_macro_int(const char *path, int32_t nicell, int32_t nnodes, int32_t nbnodes, const double *H,
+                       const double *G, const double *B, const double *Abdx, const double *Abdy,
+                       const double *Abdz, const double *Htx, const double *Hty, const double *Htz,
+                       const double *Dx, const double *Dy, const double *Dz, const double ont[11]);
+/* ReadMacroIntDisk (lib_int.f:1393-1441); *nnodes, *nbnodes from the header
+ * (may be NULL). */
+int tt_read_macro_int(const char *path, int32_t nicell, int32_t *nnodes, int32_t *nbnodes, double *H, double *G,
+                      double *B, double *Abdx, double *Abdy, double *Abdz, double *Htx, double *Hty, double *Htz,
+                      double *Dx, double *Dy, double *Dz, double ont[11]);
+/* WriteSDIntDisk (lib_sdkm.f:2641-2676): tri.sdkm.int.N for `rows` boundary
+ * rows (the reference's env%nmn) from the arrays tt_sd_integrals returns;
+ * header nnodes, nbnodes, nproc (the number of rank files; 1 for one GPU). */
+int tt_write_sdkm_int(const char *path, int32_t nnodes, int32_t nbnodes, int32_t nproc, int32_t rows,
+                      const double *H, const double *Htx, const double *Hty, const double *Htz, const double *Dx,
+                      const double *Dy, const double *Dz, const double ont[10]);
+/* ReadSDIntDisk (lib_sdkm.f:2678-2719); hdr_out[3] (may be NULL) = nnodes,
+ * nbnodes, nproc of the header (CheckSDIntFile, :2603-2620). */
+int tt_read_sdkm_int(const char *path, int32_t rows, int32_t hdr_out[3], double *H, double *Htx, double *Hty,
+                     double *Htz, double *Dx, double *Dy, double *Dz, double ont[10]);
+/* WrstF (lib_IO.f:604-685), restart version 4: cnt%tstep, cnt%glnnlit,
+ * cnt%rtime and the seventeen fields in WrstF's order: velocity,
+ * pnlsvelocity, vorticity, ptsvorticity, pptsvorticity, pnlsvorticity
+ * (nnodes,3); qvorticity (nq,3); temp, ptstemp, pptstemp, pnlstemp (nnodes);
+ * qtemp (nq); dc, ptsdc, pptsdc, pnlsdc (nnodes); qdc (nq). */
+int tt_write_restart(const char *path, int32_t nnodes, int32_t nq, int32_t tstep, int32_t glnnlit, double rtime,
+                     const double *const fields[17]);
+/* RrstF (lib_IO.f:505-602): versions 3 (the first twelve fields) and 4. */
+int tt_read_restart(const char *path, int32_t nnodes, int32_t nq, int32_t *version, int32_t *tstep,
+                    int32_t *glnnlit, double *rtime, double *const fields[17]);
+/* tri.sta and tri.ite (either path may be NULL): the header lines of
+ * lib_IO.f:1468-1485 (id_line in place of the program's name, version and
+ * date; NULL gives a default) and per row the lines of CheckConvergencevw
+ * (lib_util.f:1141-1146): tstep, tsnnlit, err (nrows x 9, row-major: the nine
+ * relative changes of tt_case_picard_step, G10.5) and nit (nrows x 12,
+ * row-major, I5). */
+int tt_write_sta_ite(const char *sta_path, const char *ite_path, const char *id_line, int32_t nrows,
+                     const int32_t *tstep, const int32_t *tsnnlit, const double *err, const int32_t *nit);
 
 /* ------------------------------------------------------------------------ */
 /* Request loop over whole cases (no reference counterpart: the reference runs

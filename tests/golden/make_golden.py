@@ -21,6 +21,10 @@ the reference, and converts its outputs into ``tests/golden/<case>.npz``:
 * ``gauss_ref.npz``  -> the reference's Gauss-Legendre table as its own
   GaussPosWeights (src/lib_int.f:2673-3173) fills it, written by the small
   Fortran harness dump_gauss.f90 linked against the reference's objects;
+* ``files_ref.npz`` -> the reference's own files of the dc2 run as bytes or
+  digests: tri.rst (raw), tri.sta and tri.ite (text), and the SHA-256 and
+  record count of tri.macro.int and tri.sdkm.int.1 (whose values dc2.npz
+  holds), for the byte-for-byte tests of the file writers;
 * ``compress_ref.npz`` -> the reference's cross approximations (aca_CrossAprox,
   ACA_AB, aca_RankCrossAprox; src/lib_ACA.f) and wavelet compressions
   (Che_WcomMatrix, src/lib_WT.f:148-210) of the matrices in COMPRESS_CASES,
@@ -451,7 +455,25 @@ def dump_compress():
     print("compress_ref ->", os.path.join(HERE, "compress_ref.npz"), len(cases), "cases")
 
 
-def run_case(name, spec, keep=None):
+def dump_files():
+    """Runs dc2 once more and keeps its files for the writer tests."""
+    import hashlib
+    work = tempfile.mkdtemp(prefix="tt_files_")
+    run_case("dc2", CASES["dc2"], keep=work, save=False)
+    out = {}
+    for key, fname in (("macro", "tri.macro.int"), ("sdkm", "tri.sdkm.int.1")):
+        data = open(os.path.join(work, fname), "rb").read()
+        out[key + "_sha256"] = np.frombuffer(hashlib.sha256(data).digest(), np.uint8)
+        out[key + "_records"] = np.int64(len(read_records(os.path.join(work, fname))))
+        out[key + "_bytes"] = np.int64(len(data))
+    for key, fname in (("rst", "tri.rst"), ("sta", "tri.sta"), ("ite", "tri.ite")):
+        out[key + "_raw"] = np.frombuffer(open(os.path.join(work, fname), "rb").read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "files_ref.npz"), **out)
+    shutil.rmtree(work)
+    print("files_ref ->", os.path.join(HERE, "files_ref.npz"))
+
+
+def run_case(name, spec, keep=None, save=True):
     work = keep or tempfile.mkdtemp(prefix="tt_golden_%s_" % name)
     os.makedirs(work, exist_ok=True)
     if "cube" in spec:  # the package's generated cube mesh and .bic text
@@ -509,14 +531,15 @@ def run_case(name, spec, keep=None):
     out["ite"] = parse_table(os.path.join(work, "tri.ite"), 14).astype(np.int32)
     out["sta"] = parse_table(os.path.join(work, "tri.sta"), 11)
     out["sol"] = parse_table(os.path.join(work, "tri.sol"), 11)
-    np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
-    print(name, "->", os.path.join(HERE, name + ".npz"), "picard its:", len(out["sta"]))
+    if save:
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
+        print(name, "->", os.path.join(HERE, name + ".npz"), "picard its:", len(out["sta"]))
     if keep is None:
         shutil.rmtree(work)
 
 
 if __name__ == "__main__":
-    names = sys.argv[1:] or ["gauss", "compress"] + list(CASES)
+    names = sys.argv[1:] or ["gauss", "compress", "files"] + list(CASES)
     keep = os.environ.get("TT_GOLDEN_KEEP")
     if "gauss" in names:
         dump_gauss()
@@ -524,6 +547,9 @@ if __name__ == "__main__":
     if "compress" in names:
         dump_compress()
         names = [n for n in names if n != "compress"]
+    if "files" in names:
+        dump_files()
+        names = [n for n in names if n != "files"]
     for n in names:
         run = run_mh_case if "mh" in CASES[n] else run_case
         run(n, CASES[n], keep=os.path.join(keep, n) if keep else None)

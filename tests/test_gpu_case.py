@@ -186,3 +186,62 @@ def test_case_without_kinematics_matches_oracle(prec, gauss_table):
             e = rel_l2(np.asarray(f[k]).reshape(-1), np.asarray(of[k]).reshape(-1))
             print("iBw=0", prec, k, e, flush=True)
             assert e < 1e-8, (k, e)
+
+
+def test_restart_and_history_files_of_a_converged_case(tmp_path, gauss_table):
+    """tri.rst (version 4) of the device's converged dc2t state: the fields
+    read back are the case's own bits and within 1e-8 of the reference's
+    restart; tri.sta / tri.ite hold one line per Picard iteration."""
+    from tritok_amd import files as F
+    g = load_golden("dc2t")
+    h, c = _case(g, gauss_table, "fp64")
+    with c:
+        sta, ite = c.run()
+        f = c.fields()
+        c.write_restart(tmp_path / "tri.rst", tstep=1, glnnlit=len(sta))
+    F.write_sta_ite(tmp_path / "tri.sta", tmp_path / "tri.ite", sta, ite)
+    m = h.view().mesh
+    back, ver, ts, gl, rt = F.read_restart(tmp_path / "tri.rst", m.nnodes, m.nq)
+    assert (ver, ts, gl) == (4, 1, len(sta))
+    assert np.array_equal(back["velocity"], f["velocity"]) and np.array_equal(back["qtemp"][:, 0], f["qtemp"])
+    assert np.array_equal(back["pnlsvorticity"], f["vorticity"])  # previous iterate == converged one
+    for k in ("velocity", "vorticity", "qvorticity"):
+        e = rel_l2(back[k].reshape(-1), g["rst_" + k].reshape(-1))
+        print("restart", k, e, flush=True)
+        assert e < 1e-8
+    assert np.array_equal(back["ptsvorticity"], np.zeros_like(back["ptsvorticity"]))  # the initial (rest) field
+    assert len((tmp_path / "tri.sta").read_text().splitlines()) == len(sta) + 2
+    assert len((tmp_path / "tri.ite").read_text().splitlines()) == len(sta) + 2
+
+
+def test_device_integrals_written_as_reference_caches(tmp_path, gauss_table):
+    """Macro and single-domain integrals computed on the device, written as
+    tri.macro.int / tri.sdkm.int.1 and parsed with the fixture parsers: the
+    reference's matrices within 1e-12 (so an unmodified Fortran run reads
+    them as its own cache)."""
+    import os
+    import sys
+    from conftest import GOLDEN
+    from tritok_amd import files as F
+    from tritok_amd import integrals as I
+    sys.path.insert(0, GOLDEN)
+    import make_golden as MG
+    g = load_golden("dc2")
+    h = host_for(g)
+    q = I.quad(4, 4, 4, 4, 2, table=gauss_table)
+    m = h.view().mesh
+    mac = I.macro_integrals(h.view(), q)
+    F.write_macro_int(tmp_path / "tri.macro.int", {**{k: mac[k] for k in F.MACRO}, "ont": mac["ont"]}, m.nnodes,
+                      m.nbnodes)
+    got = MG.parse_macro(str(tmp_path / "tri.macro.int"), m.nicell)
+    for k in F.MACRO:
+        e = np.linalg.norm(got["macro_" + k] - g["macro_" + k]) / np.linalg.norm(g["macro_" + k])
+        assert e < 1e-12, (k, e)
+    sd = I.sd_integrals(h.view(), q)
+    F.write_sdkm_int(tmp_path / "tri.sdkm.int.1", {**{k: sd[k] for k in F.SDKM}, "ont": sd["ont"]}, m.nnodes,
+                     m.nbnodes)
+    got = MG.parse_sdkm(str(tmp_path / "tri.sdkm.int.1"), m.nbnodes, m.nnodes)
+    for k in F.SDKM:
+        e = np.linalg.norm(got["sd_" + k] - g["sd_" + k]) / np.linalg.norm(g["sd_" + k])
+        assert e < 1e-12, (k, e)
+    assert os.path.getsize(tmp_path / "tri.macro.int") == int(load_golden("files_ref")["macro_bytes"])

@@ -29,6 +29,7 @@ _probe_stats = _lib._sig("tt_case_probe_stats", i32, P, i32, C.POINTER(C.c_doubl
 _sysinfo = _lib._sig("tt_case_system_info", i32, P, i32, C.POINTER(i64))
 _setgrid = _lib._sig("tt_case_set_grid", i32, P, i32)
 _setgraphs = _lib._sig("tt_case_set_graphs", i32, P, i32)
+_wrst = _lib._sig("tt_case_write_restart", i32, P, C.c_char_p, i32, i32, C.c_double)
 
 PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
 TIMING_KEYS = ["boundary_vorticity", "km", "temperature", "vte", "convergence", "reserved", "step_total",
@@ -123,6 +124,10 @@ class Case:
         check(_sysinfo(self._h, int(sys), info), "tt_case_system_info")
         return dict(zip(["m", "n", "nnz", "cells", "rows_per_cell", "cols_per_cell", "grid_fp64", "grid_fp32",
                          "grid_fp16"], list(info)))
+
+    def write_restart(self, path, tstep=1, glnnlit=0, rtime=1e20):
+        """tri.rst (WrstF, version 4) of the current state."""
+        check(_wrst(self._h, str(path).encode(), int(tstep), int(glnnlit), float(rtime)), "tt_case_write_restart")
 
     def device_bytes(self):
         return int(_bytes(self._h))

@@ -308,6 +308,7 @@ struct tt_case {
   int it = 0;
   bool finished = false;
   int64_t km_solver_warnings = 0;  // kinematics solves that ended with istop > 3 (the case continues)
+  std::vector<double> vort0, temp0;  // fields at the start of the time step (restart pts/ppts fields)
   double t_phase[11] = {0};
   hipEvent_t ev[8];
 };
@@ -405,6 +406,8 @@ void reset_fields(tt_case &cr, const double *vel, const double *vort, const doub
   hipStream_t s = c->s;
   {
     if (!vel || !vort || !qvort || !temp || !qtemp) throw std::runtime_error("NULL field");
+    c->vort0.assign(vort, vort + (size_t)nn * 3);
+    c->temp0.assign(temp, temp + nn);
     c->vel.upload(vel, (size_t)nn * 3, s);
     c->vort.upload(vort, (size_t)nn * 3, s);
     c->qvort.upload(qvort, (size_t)nq * 3, s);
@@ -692,6 +695,28 @@ extern "C" int tt_case_fields(tt_case *c, double *vel, double *vort, double *qvo
     if (temp) c->temp.download(temp, nn, c->s);
     if (qtemp) c->qtemp.download(qtemp, nq, c->s);
     TT_HIP(hipStreamSynchronize(c->s));
+    return 0;
+  });
+}
+
+extern "C" int tt_case_write_restart(tt_case *c, const char *path, int32_t tstep, int32_t glnnlit, double rtime) {
+  return guarded("tt_case_write_restart", [&] {
+    if (!c || !path) throw std::runtime_error("NULL argument");
+    const size_t nn = c->cv.mesh.nnodes, nq = c->cv.mesh.nq;
+    std::vector<double> vel(nn * 3), pvel(nn * 3), vort(nn * 3), pvort(nn * 3), qvort(nq * 3), temp(nn), ptemp(nn),
+        qtemp(nq), zn(nn, 0.0), zq(nq, 0.0);
+    c->vel.download(vel.data(), nn * 3, c->s), c->pv.download(pvel.data(), nn * 3, c->s);
+    c->vort.download(vort.data(), nn * 3, c->s), c->pw.download(pvort.data(), nn * 3, c->s);
+    c->qvort.download(qvort.data(), nq * 3, c->s);
+    c->temp.download(temp.data(), nn, c->s), c->pt.download(ptemp.data(), nn, c->s);
+    c->qtemp.download(qtemp.data(), nq, c->s);
+    TT_HIP(hipStreamSynchronize(c->s));
+    const double *f[17] = {vel.data(),     pvel.data(),     vort.data(),     c->vort0.data(), c->vort0.data(),
+                           pvort.data(),   qvort.data(),    temp.data(),     c->temp0.data(), c->temp0.data(),
+                           ptemp.data(),   qtemp.data(),    zn.data(),       zn.data(),       zn.data(),
+                           zn.data(),      zq.data()};
+    if (tt_write_restart(path, (int32_t)nn, (int32_t)nq, tstep, glnnlit, rtime, f) < 0)
+      throw std::runtime_error(tt_last_error());
     return 0;
   });
 }
