@@ -355,6 +355,12 @@ int tt_case_set_anchor_period(tt_case *c, int32_t period);
  * precision, 0 returns to the autotuner's choice.  Results do not depend on
  * it (the partial sums are per fixed group of cells). */
 int tt_case_set_grid(tt_case *c, int32_t grid);
+/* Threads per workgroup of the least-squares unknown kernel (no reference
+ * counterpart; a tuning and test hook): 256, 512 or 1024 for every system
+ * and block precision, 0 = leave it to the autotuner (which only picks it
+ * when it also picks the grid).  Results do not depend on it (the partial
+ * sums are per fixed group of 256 unknowns). */
+int tt_case_set_unknown_threads(tt_case *c, int32_t threads);
 /* How the least-squares iterations are issued (no reference counterpart; a
  * tuning and test hook): 0 (default) launches every kernel on the case's
  * stream, 1 replays chunks of 2..64 iterations from HIP graphs captured once
@@ -374,6 +380,11 @@ int tt_case_probe_stats(tt_case *c, int32_t rows, double *avg_ms, int64_t *sampl
  * columns per cell, then the cell-kernel grid the autotuner chose for FP64,
  * FP32 and FP16 blocks (0: not tuned yet)}. */
 int tt_case_system_info(tt_case *c, int32_t sys, int64_t info[9]);
+/* What the autotuner picked for a system (no reference counterpart), per
+ * block precision FP64, FP32, FP16 (0: not tuned yet): the cell kernel's
+ * grid and the unknown kernel's threads per workgroup (256, 512 or 1024).
+ * Neither choice changes a result. */
+int tt_case_system_tuning(tt_case *c, int32_t sys, int32_t grid[3], int32_t threads[3]);
 /* One Picard iteration (main.f90:1200-1589): err[9] = the nine relative
  * changes (tri.sta), nit[12] = solver iterations (tri.ite).  Returns 1 while
  * the loop should continue, 0 when converged or at nnlit, < 0 on error --

@@ -112,6 +112,29 @@ def test_results_do_not_depend_on_cell_kernel_grid(prec, gauss_table):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_results_do_not_depend_on_unknown_kernel_threads(prec):
+    """The autotuner's second choice, threads per workgroup of the unknown
+    kernel, cannot change a result either: partial sums per fixed group of
+    256 unknowns, the cell partials reduced in one fixed order."""
+    from tritok_amd import workloads
+    from tritok_amd.case import Case
+    h = workloads.natural_convection(6, ra=1e4)
+    outs = []
+    for threads in (0, 256, 512, 1024):
+        with Case(h, prec=prec) as c:
+            c.set_unknown_threads(threads)
+            sta, ite = c.run(max_iter=6)
+            outs.append((threads, sta, ite, c.fields(), c.system_tuning(0)))
+    print(prec, [(o[0], o[4]) for o in outs], flush=True)
+    assert outs[0][4]["grid"][{"fp64": 0, "fp16": 2}[prec]] > 0
+    for o in outs[1:]:
+        assert o[4]["threads"][{"fp64": 0, "fp16": 2}[prec]] == o[0]
+        assert np.array_equal(o[1], outs[0][1]) and np.array_equal(o[2], outs[0][2])
+        for k in FIELDS:
+            assert np.array_equal(o[3][k], outs[0][3][k]), (o[0], k)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
 def test_graph_replay_matches_stream_launches(prec):
     """The least-squares iterations give the same bits replayed from captured
     graph chunks, launched kernel by kernel, and replayed with sampled

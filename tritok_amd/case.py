@@ -28,6 +28,8 @@ _probe = _lib._sig("tt_case_probe", i32, P, i32)
 _probe_stats = _lib._sig("tt_case_probe_stats", i32, P, i32, C.POINTER(C.c_double), C.POINTER(i64), C.POINTER(i64))
 _sysinfo = _lib._sig("tt_case_system_info", i32, P, i32, C.POINTER(i64))
 _setgrid = _lib._sig("tt_case_set_grid", i32, P, i32)
+_tuning = _lib._sig("tt_case_system_tuning", i32, P, i32, _lib.PI, _lib.PI)
+_setthreads = _lib._sig("tt_case_set_unknown_threads", i32, P, i32)
 _setgraphs = _lib._sig("tt_case_set_graphs", i32, P, i32)
 _wrst = _lib._sig("tt_case_write_restart", i32, P, C.c_char_p, i32, i32, C.c_double)
 
@@ -128,6 +130,17 @@ class Case:
     def write_restart(self, path, tstep=1, glnnlit=0, rtime=1e20):
         """tri.rst (WrstF, version 4) of the current state."""
         check(_wrst(self._h, str(path).encode(), int(tstep), int(glnnlit), float(rtime)), "tt_case_write_restart")
+
+    def set_unknown_threads(self, threads):
+        """Threads per workgroup of the unknown kernel (256/512/1024; 0: tuned)."""
+        check(_setthreads(self._h, int(threads)), "tt_case_set_unknown_threads")
+
+    def system_tuning(self, sys):
+        """Autotuned launch shapes of a system: {'grid': [fp64, fp32, fp16], 'threads': [...]}."""
+        g, t = np.zeros(3, np.int32), np.zeros(3, np.int32)
+        check(_tuning(self._h, int(sys), g.ctypes.data_as(_lib.PI), t.ctypes.data_as(_lib.PI)),
+              "tt_case_system_tuning")
+        return {"grid": g.tolist(), "threads": t.tolist()}
 
     def device_bytes(self):
         return int(_bytes(self._h))

@@ -826,6 +826,16 @@ extern "C" int tt_case_set_grid(tt_case *c, int32_t grid) {
   });
 }
 
+extern "C" int tt_case_set_unknown_threads(tt_case *c, int32_t threads) {
+  return guarded("tt_case_set_unknown_threads", [&] {
+    if (!c || (threads != 0 && threads != 256 && threads != 512 && threads != 1024))
+      throw std::runtime_error("threads must be 0 (tuned), 256, 512 or 1024");
+    for (SubSys *ss : {&c->vte[0], &c->vte[1], &c->vte[2], &c->tsys, &c->km})
+      for (int p = 0; p < 3; ++p) ss->bs.nb_b[p] = threads;
+    return 0;
+  });
+}
+
 extern "C" int tt_case_set_graphs(tt_case *c, int32_t on) {
   return guarded("tt_case_set_graphs", [&] {
     if (!c || on < 0 || on > 1) throw std::runtime_error("on must be 0 or 1");
@@ -842,6 +852,17 @@ extern "C" int tt_case_system_info(tt_case *c, int32_t sys, int64_t info[9]) {
     const SubSys &S = sys < 3 ? c->vte[sys] : sys == 3 ? c->tsys : c->km;
     info[0] = (int64_t)S.ncell * S.R, info[1] = S.n, info[2] = S.nnz, info[3] = S.ncell, info[4] = S.R, info[5] = S.L;
     for (int p = 0; p < 3; ++p) info[6 + p] = S.bs.grid_a[p];
+    return 0;
+  });
+}
+
+extern "C" int tt_case_system_tuning(tt_case *c, int32_t sys, int32_t grid[3], int32_t threads[3]) {
+  return guarded("tt_case_system_tuning", [&] {
+    if (!c || !grid || !threads) throw std::runtime_error("NULL argument");
+    if (sys < 0 || sys > 4) throw std::runtime_error("sys must be 0..4 (VTE x, y, z, T, KM)");
+    if (sys == 3 && c->cv.iDT != 1) throw std::runtime_error("the case has no energy equation");
+    const SubSys &S = sys < 3 ? c->vte[sys] : sys == 3 ? c->tsys : c->km;
+    for (int p = 0; p < 3; ++p) grid[p] = S.bs.grid_a[p], threads[p] = S.bs.nb_b[p];
     return 0;
   });
 }

@@ -75,6 +75,7 @@ struct BlockSys {
   DBuf<double> q;         // column scaling
   bool has32 = false, has16 = false;
   int grid_a[3] = {0, 0, 0};  // cell-kernel blocks per block precision (BlockPrec), 0 = resident blocks
+  int nb_b[3] = {0, 0, 0};    // unknown-kernel threads per workgroup per block precision, 0 = 256
   // recent solves per right-hand-side slot (a system solved for several
   // field components keeps one history per component): their iteration
   // counts size the first launch batch of the next solve

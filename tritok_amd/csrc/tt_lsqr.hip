@@ -29,6 +29,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include <hip/hip_ext.h>
 
@@ -596,14 +597,56 @@ __global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(int ncell, int cgroup, con
 
 // ---------------------------------------------------------------------------
 // iteration k, unknown part (k = 0: initialisation after r0)
-template <int W, typename PT>
-__global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gtab, const PT *__restrict__ P,
-                                              double *__restrict__ vp, double *__restrict__ w,
-                                              double *__restrict__ x, const double *__restrict__ pa1,
-                                              const double *__restrict__ pa2, int na, double *__restrict__ pb1,
-                                              double *__restrict__ pb2, LsqrState *st) {
-  __shared__ double sh[kNB / 64];
-  const int j = xcd_logical(blockIdx.x, 0, gridDim.x) * kNB + threadIdx.x;
+//
+// The unknown kernel runs NB = 256, 512 or 1024 threads per workgroup (picked
+// by block_tune) with the same bits for any NB: its partial sums are per fixed
+// group of kNB = 256 unknowns, and the reductions of the cell kernel's
+// partials are done by the first kNB threads in the order of a kNB-thread
+// workgroup (the other threads add exact zeros).
+
+// reduce_partials<kNB> computed in a workgroup of NB >= kNB threads
+template <int NB>
+__device__ __forceinline__ double reduce_partials_nb(const double *__restrict__ p, int n, double *sh) {
+  double acc = 0.0;
+  if ((int)threadIdx.x < kNB)
+    for (int base = 0; base < n; base += 16 * kNB) {
+      double v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int i = base + q * kNB + (int)threadIdx.x;
+        v[q] = i < n ? p[i] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc += v[q];
+    }
+  return block_sum<NB>(acc, sh);
+}
+// sum over each group of kNB consecutive threads (wave sums added in wave
+// order, as block_sum<kNB>); every thread gets its group's sum
+template <int NB>
+__device__ __forceinline__ double group_sum_nb(double v, double *sh) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = v;
+  __syncthreads();
+  const int g0 = (threadIdx.x / kNB) * (kNB / 64);
+  double r = 0.0;
+#pragma unroll
+  for (int i = 0; i < kNB / 64; ++i) r += sh[g0 + i];
+  __syncthreads();
+  return r;
+}
+
+template <int W, typename PT, int NB>
+__global__ void __launch_bounds__(NB) lsqr_B(int n, const int *__restrict__ gtab, const PT *__restrict__ P,
+                                             double *__restrict__ vp, double *__restrict__ w,
+                                             double *__restrict__ x, const double *__restrict__ pa1,
+                                             const double *__restrict__ pa2, int na, double *__restrict__ pb1,
+                                             double *__restrict__ pb2, LsqrState *st) {
+  static_assert(NB % kNB == 0, "whole groups of kNB unknowns per workgroup");
+  __shared__ double sh[NB / 64];
+  const int jb = xcd_logical(blockIdx.x, 0, gridDim.x) * NB;  // first unknown of the workgroup
+  const int j = jb + threadIdx.x;
   // the index table and the unknown's own vectors do not depend on the
   // state: they are requested together with it
   // (threads past the last unknown repeat its loads and store nothing, and
@@ -630,7 +673,7 @@ __global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gta
     const PT pe = P[idx[e] >= 0 ? idx[e] : 0];
     pv[e] = idx[e] >= 0 ? (double)pe : 0.0;
   }
-  const double beta = sqrt(reduce_partials<kNB>(pa1, na, sh));  // beta_{k+1}
+  const double beta = sqrt(reduce_partials_nb<NB>(pa1, na, sh));  // beta_{k+1}
   const double alpha = st->alpha, t1 = st->t1, t2 = st->t2, t3 = st->t3;
   double nv = 0.0, dk = 0.0;
   if (j < n) {
@@ -660,11 +703,12 @@ __global__ void __launch_bounds__(kNB) lsqr_B(int n, const int *__restrict__ gta
     vp[j] = nvp;
     nv = nvp * nvp;
   }
-  const double s1 = block_sum<kNB>(nv, sh), s2 = block_sum<kNB>(dk, sh);
-  if (threadIdx.x == 0) pb1[blockIdx.x] = s1, pb2[blockIdx.x] = s2;
+  const double s1 = group_sum_nb<NB>(nv, sh), s2 = group_sum_nb<NB>(dk, sh);
+  const int grp = (jb + (int)threadIdx.x) / kNB, ngrp = (n + kNB - 1) / kNB;
+  if (threadIdx.x % kNB == 0 && grp < ngrp) pb1[grp] = s1, pb2[grp] = s2;
   if (blockIdx.x == 0) {
     double nbv = 0.0;
-    if (k == 0) nbv = sqrt(reduce_partials<kNB>(pa2, na, sh));
+    if (k == 0) nbv = sqrt(reduce_partials_nb<NB>(pa2, na, sh));
     if (threadIdx.x == 0) {
       st->beta = beta;
       if (k == 0) {
@@ -761,16 +805,26 @@ void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st, hipE
 // cell kernel's partials in its block precision's type
 template <int W>
 void launch_B(BlockSys &s, LsqrWork &wk, int k, int prec, hipStream_t st) {
-  if (k >= 1 && prec == kBlocks16) {
-    hipLaunchKernelGGL((lsqr_B<W, float>), dim3(ceil_div(s.n, kNB)), dim3(kNB), 0, st, s.n, s.gtab.p,
-                       (const float *)wk.P.p, wk.v.p, wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, wk.na, wk.pb1.p, wk.pb2.p,
-                       wk.state.p + (k & 1));
-    return;
-  }
-  hipLaunchKernelGGL((lsqr_B<W, double>), dim3(ceil_div(s.n, kNB)), dim3(kNB), 0, st, s.n, s.gtab.p,
-                     (const double *)wk.P.p, wk.v.p,
-                     wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, k == 0 ? s.ncell : wk.na, wk.pb1.p, wk.pb2.p,
-                     wk.state.p + (k & 1));
+  const int nbt = s.nb_b[prec] > 0 ? s.nb_b[prec] : kNB;  // threads per workgroup (tuned)
+  const bool half = k >= 1 && prec == kBlocks16;
+  const int na = k == 0 ? s.ncell : wk.na;
+  auto go = [&](auto nbc) {
+    constexpr int NB = decltype(nbc)::value;
+    if (half)
+      hipLaunchKernelGGL((lsqr_B<W, float, NB>), dim3(ceil_div(s.n, NB)), dim3(NB), 0, st, s.n, s.gtab.p,
+                         (const float *)wk.P.p, wk.v.p, wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, na, wk.pb1.p, wk.pb2.p,
+                         wk.state.p + (k & 1));
+    else
+      hipLaunchKernelGGL((lsqr_B<W, double, NB>), dim3(ceil_div(s.n, NB)), dim3(NB), 0, st, s.n, s.gtab.p,
+                         (const double *)wk.P.p, wk.v.p, wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, na, wk.pb1.p, wk.pb2.p,
+                         wk.state.p + (k & 1));
+  };
+  if (nbt == 1024)
+    go(std::integral_constant<int, 1024>());
+  else if (nbt == 512)
+    go(std::integral_constant<int, 512>());
+  else
+    go(std::integral_constant<int, 256>());
 }
 
 void r0(BlockSys &s, LsqrWork &wk, const double *b, const double *x, hipStream_t st) {
@@ -1001,7 +1055,7 @@ __global__ void fill_kernel(int n, double v, double *__restrict__ a) {
 // cell-kernel grid chosen per (rows per cell, cells, block precision), kept
 // for the process: systems of the same shape tune once
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int>, int> g_tuned;
+std::map<std::tuple<int, int, int>, std::pair<int, int>> g_tuned;  // -> (cell grid, unknown-kernel threads)
 
 }  // namespace
 
@@ -1012,12 +1066,14 @@ std::map<std::tuple<int, int, int>, int> g_tuned;
 // fixed group of cells, so the grid does not change any result.
 void block_tune(BlockSys &s, int prec, hipStream_t st) {
   if (s.grid_a[prec] > 0) return;
+  const int forced_nb = s.nb_b[prec];  // set by tt_case_set_unknown_threads: kept
   const auto key = std::make_tuple(s.R, s.ncell, prec);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
-      s.grid_a[prec] = it->second;
+      s.grid_a[prec] = it->second.first;
+      s.nb_b[prec] = forced_nb > 0 ? forced_nb : it->second.second;
       return;
     }
   }
@@ -1082,8 +1138,40 @@ void block_tune(BlockSys &s, int prec, hipStream_t st) {
   for (auto &e : ev) (void)hipEventDestroy(e);
   // a scratch solve that stopped early timed empty launches: keep the default
   s.grid_a[prec] = tw.host_state->done ? resident : best;
+  // then the unknown kernel's workgroup size on that grid (results do not
+  // depend on it: its partial sums are per fixed group of unknowns)
+  const int nbc[3] = {256, 512, 1024};
+  std::vector<hipEvent_t> ev2(2 * 3 * kReps);
+  for (auto &e : ev2) TT_HIP(hipEventCreate(&e));
+  for (int rep = 0; rep < kReps; ++rep)
+    for (int ci = 0; ci < 3; ++ci) {
+      s.nb_b[prec] = nbc[ci];
+      TT_HIP(hipEventRecord(ev2[2 * (rep * 3 + ci)], st));
+      for (int i = 0; i < kIts; ++i, ++k) {
+        stepA(s, tw, k, prec, st, nullptr, nullptr);
+        stepB(s, tw, k, prec, st);
+      }
+      TT_HIP(hipEventRecord(ev2[2 * (rep * 3 + ci) + 1], st));
+    }
+  TT_LAUNCH_CHECK();
+  TT_HIP(hipMemcpyAsync(tw.host_state, tw.state.p + ((k - 1) & 1), sizeof(LsqrState), hipMemcpyDeviceToHost, st));
+  TT_HIP(hipStreamSynchronize(st));
+  int best_nb = kNB;
+  float best_nb_ms = 1e30f;
+  for (int ci = 0; ci < 3; ++ci) {
+    float t = 1e30f;
+    for (int rep = 0; rep < kReps; ++rep) {
+      float ms = 0.f;
+      TT_HIP(hipEventElapsedTime(&ms, ev2[2 * (rep * 3 + ci)], ev2[2 * (rep * 3 + ci) + 1]));
+      t = std::min(t, ms);
+    }
+    if (t < best_nb_ms) best_nb_ms = t, best_nb = nbc[ci];
+  }
+  for (auto &e : ev2) (void)hipEventDestroy(e);
+  s.nb_b[prec] = tw.host_state->done ? kNB : best_nb;
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tuned[key] = s.grid_a[prec];
+  g_tuned[key] = std::make_pair(s.grid_a[prec], s.nb_b[prec]);
+  if (forced_nb > 0) s.nb_b[prec] = forced_nb;
 }
 
 LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
