@@ -227,6 +227,9 @@ def batched_solves_leg(count, seed=20251203):
                        "[10, 400], synthetic right-hand sides, LSQR eps 1e-7, FP64 and FP32 values" % count,
            "systems": count, "nonzeros": float(nnz.sum()), "setup_s": setup}
     x64 = None
+    scale, tune_ms = B.tune(bs, x0, 5000, 1e-7)  # gang-width autotuner (FP64 values)
+    out["gang_width_tuning"] = {"best_scale": scale, "ms_per_scale": tune_ms}
+    log("batched solves: gang widths scaled by %g (ms per scale %s)" % (scale, tune_ms))
     for prec, vb in (("fp64", 8), ("fp32", 4)):
         B.set_precision(prec)
         B.solve(bs, x0, 5000, 1e-7)  # warm-up (FP32: makes the value copies)

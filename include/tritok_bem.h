@@ -157,6 +157,13 @@ int tt_lsqr_batch_set_width(tt_lsqr_batch *h, int32_t width, int32_t *max_width)
  * the values' rounding (about 6e-8 relative per entry) times the system's
  * conditioning. */
 int tt_lsqr_batch_set_precision(tt_lsqr_batch *h, int32_t prec);
+/* Autotuner of the gang widths (no reference counterpart): times one full
+ * solve of b from the hot start x (unchanged) with the automatic widths
+ * scaled by 0.5, 1, 2 and 4 (ms[4], may be NULL) and keeps the fastest
+ * schedule; *best_scale (may be NULL) = its scale.  Results depend on the
+ * widths only through the order of the parallel sums. */
+int tt_lsqr_batch_tune(tt_lsqr_batch *h, const double *b, const double *x, int32_t maxit, float eps4,
+                       double *best_scale, double ms[4]);
 void tt_lsqr_batch_free(tt_lsqr_batch *h);
 
 /* Replaces CRSxV (lib_util.f:1351-1372): y = A x on the device for a CSR

@@ -289,3 +289,27 @@ def test_lsqr_batch_fp32_values_match_oracle_on_rounded_values():
     x64b, _, _, _ = B.solve(bs, x0s, 5000, 1e-10)
     assert all(np.array_equal(u, w) for u, w in zip(x64, x64b))
     B.close()
+
+
+def test_lsqr_batch_gang_width_tuner_keeps_oracle_parity():
+    from tritok_amd import solvers as S
+    systems = [_vte_system(load_golden("dc2")), _vte_system(load_golden("jit2"), e=1)]
+    for m, n, k, seed in [(3000, 1000, 11, 3), (400, 120, 7, 4)]:
+        ia, ja, a = _random_csr(m, n, k, seed)
+        systems.append((ia, ja, a, n))
+    rng = np.random.default_rng(29)
+    bs = [rng.standard_normal(len(s[0]) - 1) for s in systems]
+    x0s = [0.1 * rng.standard_normal(s[3]) for s in systems]
+    B = S.LsqrBatch(systems)
+    scale, ms = B.tune(bs, x0s, 5000, 1e-10)
+    print("gang width scale", scale, ms, flush=True)
+    assert scale in ms and all(t > 0 for t in ms.values())
+    xs, istop, nit, _ = B.solve(bs, x0s, 5000, 1e-10)
+    for k, (ia, ja, a, n) in enumerate(systems):
+        m = len(ia) - 1
+        ao = np.array(a, np.float64)
+        q = O.prelsqr2(m, n, ia, ja, ao)
+        x_ref, istop_ref, nit_ref = O.slvlsqr2(m, n, ia, ja, ao, q, bs[k], x0s[k], 5000, 1e-10)
+        assert istop[k] == istop_ref and abs(nit[k] - nit_ref) <= 2
+        assert rel_l2(xs[k], x_ref) < 1e-8
+    B.close()

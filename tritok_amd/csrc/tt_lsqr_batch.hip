@@ -323,6 +323,7 @@ struct tt_lsqr_batch {
   DBuf<int> nit, istop;
   // schedule: per workgroup its assignments aptr[g]..aptr[g+1] in global order
   int grid = 0, force_width = 0, max_width = 0;
+  double width_scale = 1.0;  // automatic widths are scaled by this (tt_lsqr_batch_tune)
   DBuf<int> aptr;
   DBuf<Assign> asg;
   DBuf<unsigned> gcnt;
@@ -369,7 +370,7 @@ void make_schedule(tt_lsqr_batch &h) {
   std::vector<int> idx(G);
   h.max_width = 1;
   for (int s : ord) {
-    int W = h.force_width > 0 ? h.force_width : (int)std::floor(h.work[s] / share);
+    int W = h.force_width > 0 ? h.force_width : (int)std::floor(h.width_scale * h.work[s] / share);
     W = std::max(1, std::min({W, 64, G}));
     // the W workgroups that fall free first (ties: lower index)
     std::iota(idx.begin(), idx.end(), 0);
@@ -526,6 +527,39 @@ extern "C" int tt_lsqr_batch_set_width(tt_lsqr_batch *h, int32_t width, int32_t 
     h->grid = resident_grid();
     make_schedule(*h);
     if (max_width) *max_width = h->max_width;
+    return 0;
+  });
+}
+
+extern "C" int tt_lsqr_batch_tune(tt_lsqr_batch *h, const double *b, const double *x, int32_t maxit, float eps4,
+                                  double *best_scale, double ms_out[4]) {
+  return guarded("tt_lsqr_batch_tune", [&] {
+    if (!h || !b || !x) throw std::runtime_error("NULL argument");
+    // candidate scalings of the automatic gang widths, each timed on a full
+    // solve of the given systems (the hot start x is not changed); the
+    // fastest schedule stays
+    const double cand[4] = {0.5, 1.0, 2.0, 4.0};
+    std::vector<double> xw(h->ntot);
+    std::vector<int32_t> nit(h->nsys), ist(h->nsys);
+    double best = 1e300, bs = 1.0;
+    const int fw = h->force_width;
+    h->force_width = 0;
+    for (int c = 0; c < 4; ++c) {
+      h->width_scale = cand[c];
+      h->grid = resident_grid();
+      make_schedule(*h);
+      double t = 0.0;
+      std::memcpy(xw.data(), x, sizeof(double) * h->ntot);
+      if (tt_lsqr_batch_solve(h, b, xw.data(), maxit, eps4, nit.data(), ist.data(), &t) < 0)
+        throw std::runtime_error(tt_last_error());
+      if (ms_out) ms_out[c] = t;
+      if (t < best) best = t, bs = cand[c];
+    }
+    h->width_scale = bs;
+    h->force_width = fw;
+    h->grid = resident_grid();
+    make_schedule(*h);
+    if (best_scale) *best_scale = bs;
     return 0;
   });
 }

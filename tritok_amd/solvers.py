@@ -90,6 +90,7 @@ _bsetup = _lib._sig("tt_lsqr_batch_setup", i32, i32, PI, PI, C.POINTER(C.c_int64
 _bsolve = _lib._sig("tt_lsqr_batch_solve", i32, P, PD, PD, i32, C.c_float, PI, PI, C.POINTER(C.c_double))
 _bwidth = _lib._sig("tt_lsqr_batch_set_width", i32, P, i32, PI)
 _bprec = _lib._sig("tt_lsqr_batch_set_precision", i32, P, i32)
+_btune = _lib._sig("tt_lsqr_batch_tune", i32, P, PD, PD, i32, C.c_float, PD, PD)
 _bfree = _lib._sig("tt_lsqr_batch_free", None, P)
 
 
@@ -120,6 +121,15 @@ class LsqrBatch:
         mw = C.c_int32()
         check(_bwidth(self._h, int(width), C.byref(mw)), "tt_lsqr_batch_set_width")
         return mw.value
+
+    def tune(self, b, x0, maxit, eps4):
+        """Gang-width autotuner: returns (best scale, {scale: ms})."""
+        b = _f64(np.concatenate(b) if isinstance(b, (list, tuple)) else b)
+        x = _f64(np.concatenate(x0) if isinstance(x0, (list, tuple)) else x0)
+        best, ms = C.c_double(), np.zeros(4)
+        check(_btune(self._h, b.ctypes.data_as(PD), x.ctypes.data_as(PD), int(maxit), float(eps4), C.byref(best),
+                     ms.ctypes.data_as(PD)), "tt_lsqr_batch_tune")
+        return best.value, dict(zip((0.5, 1.0, 2.0, 4.0), ms.tolist()))
 
     def set_precision(self, prec):
         """Values the LSQR products read: "fp64" or "fp32"."""
