@@ -77,6 +77,27 @@ def build_library(force=True, jobs=8):
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "tt_fortran_driver")
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran_driver():
+    """The Fortran host example (fortran/tt_fortran_driver.f90 over the
+    ISO_C_BINDING module), linked to the in-tree library; skipped without a
+    Fortran compiler."""
+    if not os.path.exists(FC):
+        print("no Fortran compiler at", FC, "- driver not built", flush=True)
+        return None
+    os.makedirs(OBJDIR, exist_ok=True)
+    mod = os.path.join(OBJDIR, "tritok_amd_bind.o")
+    _run([FC, "-O2", "-c", os.path.join(FORTRAN, "tritok_amd_bind.f90"), "-o", mod, "-module-dir", OBJDIR])
+    _run([FC, "-O2", os.path.join(FORTRAN, "tt_fortran_driver.f90"), mod, "-I", OBJDIR, "-o", DRIVER + ".tmp",
+          "-L" + HERE, "-ltritok_amd", "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build_oracle():
     _run(["make", "-C", os.path.join(REPO, "oracle"), "-B"])
 
@@ -89,6 +110,7 @@ def build_reference():
 
 def main():
     build_library(force="--incremental" not in sys.argv)
+    build_fortran_driver()
     build_oracle()
     if "--no-ref" not in sys.argv:
         build_reference()

@@ -351,6 +351,71 @@ MODULE tritok_amd_bind
       IMPORT :: C_PTR
       TYPE(C_PTR), VALUE :: h
     END SUBROUTINE
+
+    ! host preprocessing: ReadMeshGid3D + GenMakroMesh, ReadBic, ReadInputFile
+    ! (strings NUL-terminated: TRIM(s)//C_NULL_CHAR)
+    FUNCTION tt_host_read_geo(path, h) BIND(C, NAME="tt_host_read_geo")
+      IMPORT :: C_INT, C_CHAR, C_PTR
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      TYPE(C_PTR) :: h
+      INTEGER(C_INT) :: tt_host_read_geo
+    END FUNCTION
+    FUNCTION tt_host_read_bic(h, text) BIND(C, NAME="tt_host_read_bic")
+      IMPORT :: C_INT, C_CHAR, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: text(*)
+      INTEGER(C_INT) :: tt_host_read_bic
+    END FUNCTION
+    FUNCTION tt_host_read_inp(h, text) BIND(C, NAME="tt_host_read_inp")
+      IMPORT :: C_INT, C_CHAR, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: text(*)
+      INTEGER(C_INT) :: tt_host_read_inp
+    END FUNCTION
+    FUNCTION tt_host_case_view(h, v) BIND(C, NAME="tt_host_case_view")
+      IMPORT :: C_INT, C_PTR, tt_case_view
+      TYPE(C_PTR), VALUE :: h
+      TYPE(tt_case_view) :: v
+      INTEGER(C_INT) :: tt_host_case_view
+    END FUNCTION
+    FUNCTION tt_host_fields(h, vel, vort, qvort, temp, qtemp) BIND(C, NAME="tt_host_fields")
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      TYPE(C_PTR) :: vel, vort, qvort, temp, qtemp
+      INTEGER(C_INT) :: tt_host_fields
+    END FUNCTION
+    SUBROUTINE tt_host_free(h) BIND(C, NAME="tt_host_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+    FUNCTION tt_case_run(h, max_iter, sta, ite) BIND(C, NAME="tt_case_run")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: max_iter
+      REAL(C_DOUBLE) :: sta(*)
+      INTEGER(C_INT32_T) :: ite(*)
+      INTEGER(C_INT) :: tt_case_run
+    END FUNCTION
+
+    ! the reference's files: WrstF (lib_IO.f:604), tri.sta / tri.ite (lib_util.f:1141-1146)
+    FUNCTION tt_case_write_restart(h, path, tstep, glnnlit, rtime) BIND(C, NAME="tt_case_write_restart")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_CHAR, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      INTEGER(C_INT32_T), VALUE :: tstep, glnnlit
+      REAL(C_DOUBLE), VALUE :: rtime
+      INTEGER(C_INT) :: tt_case_write_restart
+    END FUNCTION
+    FUNCTION tt_write_sta_ite(sta_path, ite_path, id_line, nrows, tstep, tsnnlit, err, nit) &
+        BIND(C, NAME="tt_write_sta_ite")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_CHAR, C_PTR
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: sta_path(*), ite_path(*)
+      TYPE(C_PTR), VALUE :: id_line
+      INTEGER(C_INT32_T), VALUE :: nrows
+      INTEGER(C_INT32_T), INTENT(IN) :: tstep(*), tsnnlit(*), nit(*)
+      REAL(C_DOUBLE), INTENT(IN) :: err(*)
+      INTEGER(C_INT) :: tt_write_sta_ite
+    END FUNCTION
   END INTERFACE
 
 CONTAINS
