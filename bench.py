@@ -293,6 +293,44 @@ def mixed_kinematics_leg(n, tol=1e-10):
     return out
 
 
+def aca_kinematics_leg(n, leaf=64, eta=1.0, eps=1e-6, reps=5):
+    """The heated cube's single-domain H (nbnodes x nbnodes) compressed where
+    it lies on the device: geometric clusters of at most `leaf` boundary
+    nodes, admissible blocks cross-approximated (ACA type 2, tolerance eps),
+    the rest dense; compression, build time, the four-column product
+    (u_x, u_y, u_z, T) against the dense device product."""
+    from tritok_amd import integrals as I
+    from tritok_amd import solvers as S
+    from tritok_amd import workloads
+    host = workloads.natural_convection(n, ra=1e5, pr=0.71)
+    nb = host.view().mesh.nbnodes
+    sd = S.SdKinematics(host, I.quad(4, 4, 4, 4, 2))
+    t0 = time.perf_counter()
+    hm = sd.hmat("H", leaf=leaf, eta=eta, aca_type=2, aca_eps=eps)
+    build = time.perf_counter() - t0
+    X = np.random.default_rng(20251204).standard_normal((nb, 4))
+    hm.mv(X)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Y = hm.mv(X)
+    mv_ms = 1e3 * (time.perf_counter() - t0) / reps
+    errs = []
+    for c in range(4):
+        ref = sd.matvec("H", X[:, c])
+        errs.append(float(np.linalg.norm(Y[:, c] - ref) / np.linalg.norm(ref)))
+    out = {"workload": "single-domain H of the %d^3 heated cube (%d x %d), leaf %d, eta %g, ACA type 2 tolerance %g"
+                       % (n, nb, nb, leaf, eta, eps),
+           "nbnodes": nb, "parts": hm.nparts, "max_rank": hm.max_rank, "stored_doubles": hm.stored,
+           "compression": hm.stored / float(nb * nb), "build_s": build,
+           "mv4_ms_with_transfers": mv_ms, "mv4_GBps_stored": hm.stored * 8 / (mv_ms * 1e-3) / 1e9,
+           "rel_err_vs_dense": max(errs)}
+    log("aca kinematics %d^3: %d parts, max rank %d, %.3f of dense, build %.2f s, 4-column product %.2f ms, "
+        "error vs dense %.2e" % (n, hm.nparts, hm.max_rank, out["compression"], build, mv_ms, max(errs)))
+    hm.close()
+    sd.close()
+    return out
+
+
 FLOP_SDKM_DOMAIN = 3 * 27 * 2 + 15  # three gradient outputs x 27 shape functions, plus the kernel
 
 
@@ -397,6 +435,8 @@ def main():
                     help="n: batched modified-Helmholtz integrals of n^3 independent cells (0: skip)")
     ap.add_argument("--batch-solves", type=int, default=1024,
                     help="independent cavity systems solved in one launch (0: skip)")
+    ap.add_argument("--aca-kin", type=int, default=20,
+                    help="n: ACA-compressed single-domain H of an n^3 cube (0: skip)")
     ap.add_argument("--mixed-kin", type=int, default=24,
                     help="n: mixed-precision kinematics solve on an n^3 cube (0: skip)")
     ap.add_argument("--assembly-only", type=int, default=0,
@@ -511,6 +551,7 @@ def main():
     case.close()  # free the 20^3 case before the other legs
     acc = accuracy_leg(args.accuracy_n, args.mixed, args.anchor_period) if args.accuracy_n > 0 else None
     mixkin = mixed_kinematics_leg(args.mixed_kin) if args.mixed_kin > 0 else None
+    acakin = aca_kinematics_leg(args.aca_kin) if args.aca_kin > 0 else None
     batched = batched_integrals_leg(args.batch_cells) if args.batch_cells > 0 else None
     bsolves = batched_solves_leg(args.batch_solves) if args.batch_solves > 0 else None
     serving = (serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, ("fp64", args.mixed))
@@ -545,6 +586,7 @@ def main():
         "setup_s": setup_s, "device": dev, "roofline": roofline, "cpu_baseline": cpu,
         "p99_solve_latency_s": serving[args.mixed]["latency_s"]["p99"] if serving else None,
         "serving": serving, "batched_integrals": batched, "batched_solves": bsolves, "mixed_kinematics": mixkin,
+        "aca_kinematics": acakin,
         "assembly": assembly,
     }
     print(json.dumps(out), flush=True)

@@ -290,6 +290,20 @@ int tt_sd_solve(tt_sd *h, const double *vel, double *vort, const double urBw[3])
  * device ms of one correction product}. */
 int tt_sd_refine(tt_sd *h, int32_t eqn, int32_t prec, int32_t nrhs, const double *B, double *X, double tol,
                  int32_t maxit, int32_t *iters, double *relres, double ms[2]);
+/* The AcaCompSmat step of the compressed kinematics (lib_fm.f:2029-2086,
+ * drivers AcaDo :2003-2028, cTree_FillMatrices lib_ctree.f:2-96) on a matrix
+ * that stays on the device: which = 0..6 selects H, Htx, Hty, Htz (nbnodes x
+ * nbnodes) or Dx, Dy, Dz (nbnodes x nnodes); the rows and columns are split
+ * by the library's geometric bisection of the boundary nodes (and, for D, of
+ * all nodes) into clusters of at most `leaf` points, admissible blocks (as
+ * tt_hmat_create) are cross-approximated with aca_type and aca_eps, the rest
+ * kept dense.  The result is a tt_hmat (tt_hmat_mv, tt_hmat_info). */
+int tt_sd_hmat(tt_sd *h, int32_t which, int32_t leaf, double eta, int32_t aca_type, double aca_eps,
+               tt_hmat **out);
+/* y = M x for one of the seven device-resident matrices (which as in
+ * tt_sd_hmat; no reference counterpart: the dense product the compressed
+ * forms are checked against). */
+int tt_sd_matvec(tt_sd *h, int32_t which, const double *x, double *y);
 void tt_sd_free(tt_sd *h);
 
 /* ------------------------------------------------------------------------ */

@@ -170,3 +170,56 @@ def test_hmatrix_given_partition_and_kinematics_matrix(gauss_table):
         parts = (info["rptr"], info["rnodes"], info["cptr"], info["cnodes"], adm)
         with K.HMatrix(H, parts=parts, ctype=3, aca_type=2, aca_eps=1e-6) as h2:
             assert np.array_equal(h2.mv(x), y)
+
+
+def test_kinematics_matrix_cross_approximated_on_the_device():
+    """The 8^3 single-domain H (1,538 x 1,538) compressed where it lies on
+    the device (ACA type 2, tolerance 1e-6, leaf 64): stores less than the
+    dense matrix, and its product with four right-hand sides (the multi-RHS
+    form of the kinematics) matches the dense device product to the
+    compression tolerance."""
+    from tritok_amd import integrals as I, solvers as S, workloads as W
+    host = W.cavity(8, re=100.0)
+    sd = S.SdKinematics(host, I.quad(4, 4, 4, 4, 2))
+    nb = host.view().mesh.nbnodes
+    hm = sd.hmat("H", leaf=64, eta=1.0, aca_type=2, aca_eps=1e-6)
+    print("H-matrix parts", hm.nparts, "stored", hm.stored, "of", nb * nb, "max rank", hm.max_rank, flush=True)
+    assert hm.stored < 0.8 * nb * nb and hm.max_rank > 0
+    X = np.random.default_rng(7).standard_normal((nb, 4))
+    Y = hm.mv(X)
+    for c in range(4):
+        ref = sd.matvec("H", X[:, c])
+        e = np.linalg.norm(Y[:, c] - ref) / np.linalg.norm(ref)
+        print("column", c, "rel err vs dense", e, flush=True)
+        assert e < 1e-4
+    hm.close()
+    sd.close()
+
+
+def test_cross_approximation_of_exactly_zero_blocks():
+    """A residual that is exactly zero (the largest |R| is 0) ends type 2's
+    approximation without a new term; the reference's APPLAY_ACA would divide
+    by the zero pivot.  Such blocks occur in the single-domain H between
+    clusters of one flat wall (r.n = 0), so a block matrix of the coplanar
+    double-layer kernel is the identity with rank-0 admissible parts."""
+    from tritok_amd import compress as K
+    with K.Aca(np.zeros((40, 30)), 2, 1e-6) as h:
+        assert h.rank == 0
+        assert np.array_equal(h.mv(np.ones(30)), np.zeros(40))
+    rng = np.random.default_rng(3)
+    p = np.zeros((600, 3))
+    p[:, :2] = rng.random((600, 2))  # points of the plane z = 0, normal (0, 0, 1)
+    r = p[None, :, :] - p[:, None, :]
+    d = np.linalg.norm(r, axis=2)
+    np.fill_diagonal(d, 1.0)
+    A = r[:, :, 2] / d ** 3  # exactly 0 off the diagonal
+    np.fill_diagonal(A, 1.0)
+    hm = K.HMatrix(A, xr=p, xc=p, leaf=16, eta=1.0, ctype=3, aca_type=2, aca_eps=1e-6)
+    info = hm.parts()
+    adm = info["ctype"] == 3
+    print("coplanar block matrix: parts", hm.nparts, "admissible", int(adm.sum()), "ranks", set(info["rank"][adm]),
+          flush=True)
+    assert adm.any() and np.all(info["rank"][adm] == 0)
+    x = rng.standard_normal(600)
+    assert np.array_equal(hm.mv(x), x)
+    hm.close()

@@ -165,11 +165,22 @@ class HMatrix(_Handle):
                          rn.ctypes.data_as(PI), cptr.ctypes.data_as(PI), cn.ctypes.data_as(PI),
                          adm.ctypes.data_as(PI), int(ctype), int(aca_type), float(aca_eps), float(wt_kappa),
                          C.byref(h)), "tt_hmat_create_parts")
+        self._adopt(h)
+
+    def _adopt(self, h):
         self._h = h
         np_, rl, cl, st, mr = i32(), i64(), i64(), i64(), i32()
         check(_hm_info(h, C.byref(np_), C.byref(rl), C.byref(cl), C.byref(st), C.byref(mr)), "tt_hmat_info")
         self.nparts, self.stored, self.max_rank = np_.value, st.value, mr.value
         self._rl, self._cl = rl.value, cl.value
+
+    @classmethod
+    def from_handle(cls, h, nrow, ncol):
+        """Wraps a tt_hmat made by another entry point (tt_sd_hmat)."""
+        obj = cls.__new__(cls)
+        obj.nrow, obj.ncol = int(nrow), int(ncol)
+        obj._adopt(h)
+        return obj
 
     def parts(self):
         n = self.nparts

@@ -64,3 +64,12 @@ struct HmatDev {
 };
 
 }  // namespace tt
+
+struct tt_hmat;
+namespace tt {
+// tt_hmat_create's block matrix built from a matrix already on the device
+// (A: nrow x ncol, leading dimension lda; xr, xc: host point coordinates,
+// n x 3 column-major); the caller's stream work on A must be complete.
+tt_hmat *hmat_from_device(int nrow, int ncol, const double *A, long lda, const double *xr, const double *xc,
+                          int leaf, double eta, int ctype, int aca_type, double aca_eps, double wt_kappa);
+}  // namespace tt

@@ -107,6 +107,15 @@ __device__ __forceinline__ bool better(double v, long k, double bv, long bk) {
   return v > bv || (v == bv && k < bk);
 }
 
+// (row, column) of the search key of the largest |R|; a search that found no
+// ordered value (an all-NaN residual) leaves the sentinel key, which maps to
+// (0, 0) so that every later access stays inside the matrix
+__device__ __forceinline__ void pivot_of(long key, bool rowmajor, int m, int n, int &i, int &j) {
+  if (key < 0 || key >= (long)m * n) key = 0;
+  i = rowmajor ? (int)(key / n) : (int)(key % m);
+  j = rowmajor ? (int)(key % n) : (int)(key / m);
+}
+
 template <int NT>
 __device__ void block_argmax(double &v, long &k, double *sv, long *sk) {
   const int tid = threadIdx.x;
@@ -301,8 +310,8 @@ __global__ void __launch_bounds__(kNT) aca_pick_kernel(const double *__restrict_
     if (better(pv[p], pk[p], bv, bkey)) bv = pv[p], bkey = pk[p];
   block_argmax<kNT>(bv, bkey, sv, sk);
   if (threadIdx.x == 0) {
-    const int i = RowMajor ? (int)(bkey / n) : (int)(bkey % m);
-    const int j = RowMajor ? (int)(bkey % n) : (int)(bkey / m);
+    int i = 0, j = 0;
+    pivot_of(bkey, RowMajor, m, n, i, j);
     st->pi = i, st->pj = j;
     st->delta = R[i + (size_t)m * j];
   }
@@ -456,8 +465,7 @@ __global__ void __launch_bounds__(kNT) aca_parts_kernel(const AcaJob *__restrict
   }
   block_argmax<kNT>(bv, bkey, sv, sk);
   if (tid == 0) {
-    s_pi = rowmajor ? (int)(bkey / nc) : (int)(bkey % nr);
-    s_pj = rowmajor ? (int)(bkey % nc) : (int)(bkey / nr);
+    pivot_of(bkey, rowmajor, nr, nc, s_pi, s_pj);
     s_snorm = 0.0;
   }
   __syncthreads();
@@ -465,6 +473,9 @@ __global__ void __launch_bounds__(kNT) aca_parts_kernel(const AcaJob *__restrict
   for (int k = 0; k < kfix; ++k) {
     const int pi = s_pi, pj = s_pj;
     const double delta = R[pi + (size_t)nr * pj];
+    // type 2 divides by the pivot: a zero pivot means the residual is zero
+    // (it is the largest |R|), and the approximation ends without a new term
+    if (type == 2 && delta == 0.0) break;
     double *ak = A + (size_t)nr * k, *bk = B + (size_t)nc * k;
     for (int i = tid; i < nr; i += kNT) {
       const double c = R[i + (size_t)nr * pj];
@@ -492,7 +503,7 @@ __global__ void __launch_bounds__(kNT) aca_parts_kernel(const AcaJob *__restrict
       s3 = block_sum<kNT>(s3, sv);
       if (tid == 0) {
         s_snorm = s_snorm + s3 + s1 * s2;
-        s_stop = sqrt(s1) * sqrt(s2) / sqrt(s_snorm) <= eps;
+        s_stop = !(sqrt(s1) * sqrt(s2) / sqrt(s_snorm) > eps);  // continue only while NORM > EPS
       }
       __syncthreads();
       stop = stop || s_stop;
@@ -523,10 +534,7 @@ __global__ void __launch_bounds__(kNT) aca_parts_kernel(const AcaJob *__restrict
       if (better(fabs(r), key, bv, bkey)) bv = fabs(r), bkey = key;
     }
     block_argmax<kNT>(bv, bkey, sv, sk);
-    if (tid == 0) {
-      s_pi = rowmajor ? (int)(bkey / nc) : (int)(bkey % nr);
-      s_pj = rowmajor ? (int)(bkey % nc) : (int)(bkey / nr);
-    }
+    if (tid == 0) pivot_of(bkey, rowmajor, nr, nc, s_pi, s_pj);
     __syncthreads();
   }
   if (tid == 0) ranks[blockIdx.x] = rank;
@@ -741,6 +749,12 @@ int aca_compress_dev(double *R, int m, int n, int type, double eps, const double
   if (kfix > 0) sweep(nullptr, nullptr);
   for (int k = 0; k < kfix; ++k) {
     double *ak = A.p + (size_t)m * k, *bk = Bv.p + (size_t)n * k;
+    if (type == 2) {  // a zero pivot: the residual is zero, nothing more to approximate
+      AcaState h;
+      st.download(&h, 1, s);
+      TT_HIP(hipStreamSynchronize(s));
+      if (h.delta == 0.0) break;
+    }
     hipLaunchKernelGGL(aca_extract_kernel, dim3(ceil_div(std::max(m, n), kNT)), dim3(kNT), 0, s, R, m, n, type, st.p,
                        ak, bk);
     TT_LAUNCH_CHECK();
@@ -759,7 +773,7 @@ int aca_compress_dev(double *R, int m, int n, int type, double eps, const double
       AcaState h;
       st.download(&h, 1, s);
       TT_HIP(hipStreamSynchronize(s));
-      stop = stop || (type == 2 ? h.norm <= eps : h.rms < eps);
+      stop = stop || (type == 2 ? !(h.norm > eps) : h.rms < eps);  // type 2 goes on only while NORM > EPS
     }
     if (stop) break;
     sweep(ak, bk);
@@ -1218,6 +1232,23 @@ static tt_hmat *hmat_from(int32_t nrow, int32_t ncol, const double *full, int64_
   h->x.alloc((size_t)ncol * 4), h->y.alloc((size_t)nrow * 4);
   return h.release();
 }
+
+namespace tt {
+// A block matrix of a device-resident matrix A (nrow x ncol, leading
+// dimension lda, left unchanged) with the library's own partition of the row
+// points xr and column points xc (host arrays, n x 3 column-major).
+tt_hmat *hmat_from_device(int nrow, int ncol, const double *A, long lda, const double *xr, const double *xc,
+                          int leaf, double eta, int ctype, int aca_type, double aca_eps, double wt_kappa) {
+  if (nrow < 1 || ncol < 1 || lda < nrow || leaf < 1 || !(eta > 0.0)) throw std::runtime_error("bad sizes");
+  Partition P = make_partition(nrow, xr, nrow, ncol, xc, ncol, leaf, eta);
+  auto h = std::make_unique<tt_hmat>();
+  TT_HIP(hipStreamCreateWithFlags(&h->s, hipStreamNonBlocking));
+  hmat_build_dev(h->d, nrow, ncol, A, lda, (int)P.admiss.size(), P.rptr.data(), P.rn.data(), P.cptr.data(),
+                 P.cn.data(), P.admiss.data(), ctype, aca_type, aca_eps, wt_kappa, h->s);
+  h->x.alloc((size_t)ncol * 4), h->y.alloc((size_t)nrow * 4);
+  return h.release();
+}
+}  // namespace tt
 
 extern "C" int tt_hmat_create(int32_t nrow, int32_t ncol, const double *full, int64_t lda, const double *xr,
                               const double *xc, int32_t leaf, double eta, int32_t ctype, int32_t aca_type,

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "tritok_bem.h"
+#include "tt_compress.h"
 #include "tt_dense.h"
 #include "tt_integrals.h"
 
@@ -114,8 +115,9 @@ struct tt_sd {
   hipStream_t s = nullptr;
   DevMesh mesh;
   DenseKin dk;
-  int nn = 0;
+  int nn = 0, nb = 0;
   DBuf<double> vel, vort;
+  std::vector<double> xb, xa;  // boundary-node and node coordinates (n x 3, column-major)
   ~tt_sd() {
     if (s) (void)hipStreamDestroy(s);
   }
@@ -134,7 +136,12 @@ extern "C" int tt_sd_create(const tt_mesh_view *m, const tt_quad *q, const int32
     sd_integrals_dev(h->mesh, *m, qh, 0, m->nbnodes, h->dk.mats, ont, h->s);
     double ms = 0;
     dense_setup(h->dk, h->mesh, kode, SdPrec::F64, h->s, &ms);
-    h->nn = m->nnodes;
+    h->nn = m->nnodes, h->nb = m->nbnodes;
+    h->xa.assign(m->x, m->x + (size_t)m->nnodes * 3);
+    h->xb.resize((size_t)m->nbnodes * 3);
+    for (int i = 0; i < m->nbnodes; ++i)
+      for (int d = 0; d < 3; ++d)
+        h->xb[i + (size_t)m->nbnodes * d] = m->x[(m->gbn[i] - 1) + (size_t)m->nnodes * d];
     h->vel.alloc((size_t)m->nnodes * 3);
     h->vort.alloc((size_t)m->nnodes * 3);
     TT_HIP(hipStreamSynchronize(h->s));
@@ -182,3 +189,51 @@ extern "C" int tt_sd_refine(tt_sd *h, int32_t eqn, int32_t prec, int32_t nrhs, c
 }
 
 extern "C" void tt_sd_free(tt_sd *h) { delete h; }
+
+namespace {
+// y = M x for a column-major nr x nc device matrix: one thread per row
+__global__ void dense_mv_kernel(int nr, int nc, const double *__restrict__ M, const double *__restrict__ x,
+                                double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nr) return;
+  double s = 0.0;
+  for (int j = 0; j < nc; ++j) s += M[i + (size_t)nr * j] * x[j];
+  y[i] = s;
+}
+const double *sd_matrix(tt_sd *h, int which, int &ncol) {
+  const SdDev &m = h->dk.mats;
+  const double *p[7] = {m.H.p, m.Htx.p, m.Hty.p, m.Htz.p, m.Dx.p, m.Dy.p, m.Dz.p};
+  if (which < 0 || which > 6) throw std::runtime_error("which must be 0..6 (H, Htx, Hty, Htz, Dx, Dy, Dz)");
+  ncol = which < 4 ? h->nb : h->nn;
+  return p[which];
+}
+}  // namespace
+
+extern "C" int tt_sd_hmat(tt_sd *h, int32_t which, int32_t leaf, double eta, int32_t aca_type, double aca_eps,
+                          tt_hmat **out) {
+  return guarded("tt_sd_hmat", [&] {
+    if (!h || !out) throw std::runtime_error("NULL argument");
+    if (aca_type < 1 || aca_type > 3) throw std::runtime_error("aca_type must be 1, 2 or 3");
+    int ncol = 0;
+    const double *A = sd_matrix(h, which, ncol);
+    TT_HIP(hipStreamSynchronize(h->s));
+    *out = hmat_from_device(h->nb, ncol, A, h->nb, h->xb.data(), which < 4 ? h->xb.data() : h->xa.data(), leaf, eta,
+                            3, aca_type, aca_eps, 0.0);
+    return 0;
+  });
+}
+
+extern "C" int tt_sd_matvec(tt_sd *h, int32_t which, const double *x, double *y) {
+  return guarded("tt_sd_matvec", [&] {
+    if (!h || !x || !y) throw std::runtime_error("NULL argument");
+    int ncol = 0;
+    const double *A = sd_matrix(h, which, ncol);
+    DBuf<double> dx, dy(h->nb);
+    dx.upload(x, ncol, h->s);
+    hipLaunchKernelGGL(dense_mv_kernel, dim3(ceil_div(h->nb, 256)), dim3(256), 0, h->s, h->nb, ncol, A, dx.p, dy.p);
+    TT_LAUNCH_CHECK();
+    dy.download(y, h->nb, h->s);
+    TT_HIP(hipStreamSynchronize(h->s));
+    return 0;
+  });
+}

@@ -22,6 +22,9 @@ _sd_solve = _lib._sig("tt_sd_solve", i32, P, PD, PD, PD)
 _sd_refine = _lib._sig("tt_sd_refine", i32, P, i32, i32, i32, PD, PD, C.c_double, i32, PI, PD, PD)
 SD_PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
 _sd_free = _lib._sig("tt_sd_free", None, P)
+_sd_hmat = _lib._sig("tt_sd_hmat", i32, P, i32, i32, C.c_double, i32, C.c_double, C.POINTER(P))
+_sd_mv = _lib._sig("tt_sd_matvec", i32, P, i32, PD, PD)
+SD_MATS = {"H": 0, "Htx": 1, "Hty": 2, "Htz": 3, "Dx": 4, "Dy": 5, "Dz": 6}
 
 
 def _i32(a):
@@ -194,6 +197,23 @@ class SdKinematics:
         ur = _f64(urbw)
         check(_sd_solve(self._h, vel.ctypes.data_as(PD), w.ctypes.data_as(PD), ur.ctypes.data_as(PD)), "tt_sd_solve")
         return w
+
+    def hmat(self, which="H", leaf=64, eta=1.0, aca_type=2, aca_eps=1e-6):
+        """Block matrix of one device-resident matrix, admissible blocks
+        cross-approximated (tt_sd_hmat); returns a compress.HMatrix."""
+        from .compress import HMatrix
+        h = P()
+        check(_sd_hmat(self._h, SD_MATS[which], int(leaf), float(eta), int(aca_type), float(aca_eps), C.byref(h)),
+              "tt_sd_hmat")
+        m = self.view.mesh
+        return HMatrix.from_handle(h, m.nbnodes, m.nbnodes if which.startswith("H") else m.nnodes)
+
+    def matvec(self, which, x):
+        """Dense product of one device-resident matrix (tt_sd_matvec)."""
+        x = _f64(x)
+        y = np.zeros(self.view.mesh.nbnodes)
+        check(_sd_mv(self._h, SD_MATS[which], x.ctypes.data_as(PD), y.ctypes.data_as(PD)), "tt_sd_matvec")
+        return y
 
     def refine(self, eqn, prec, B, tol=1e-10, maxit=50):
         """Solves A_eqn X = B (nb x nrhs, nrhs <= 4) by FP64 iterative
