@@ -307,9 +307,13 @@ __constant__ static const signed char kNode27[27][3] = {
     {-1, 0, 0},  {0, 0, -1},  {1, 0, 0},   {0, 0, 1},  {0, -1, 0},  {0, 0, 0}};
 
 // LDS doubles one wavefront needs for wave_cell<NC>
+// entries of the singular transform's table: (K, N, I) triples of one chunk
+// of K values (see WaveCell::run)
+constexpr int kTrigTab = 192;
+
 template <int NC>
 constexpr int wave_cell_lds() {
-  return (9 + NC) * 64;
+  return (9 + NC) * 64 + kTrigTab * 4 + 2 * kTrigTab;
 }
 
 template <int NC, bool Helm>
@@ -417,48 +421,73 @@ struct WaveCell {
                 ffi0 = PI + acos(xx1 / aa0);
                 ffi1 = 1.5 * PI + atan((1.0 - xx1) / yy1);
               }
-              for (int K = 0; K < q.nds; ++K) {
-                const double ffi = (ffi1 - ffi0) * q.gds[K] / 2.0 + (ffi1 + ffi0) / 2.0;
-                const double ro0 = aa0 * sin(gama) / sin(ffi - ffi0 + gama);
-                const double sf = sin(ffi), cf = cos(ffi);
-                for (int N = 1; N <= 3; ++N) {
-                  double aa1, eps, th0, th1;
-                  if (N == 1) {
-                    aa1 = 1.0 - zz1;
-                    eps = PI / 2.0;
-                    th0 = 0.0;
-                    th1 = acos(aa1 / sqrt(aa1 * aa1 + ro0 * ro0));
-                  } else if (N == 2) {
-                    aa1 = sqrt(ro0 * ro0 + (1.0 - zz1) * (1.0 - zz1));
-                    eps = acos((1.0 - zz1) / aa1);
-                    th0 = eps;
-                    const double rr1 = fmin(1.0, ro0 / sqrt(zz1 * zz1 + ro0 * ro0));
-                    th1 = 0.5 * PI + acos(rr1);
-                  } else {
-                    aa1 = sqrt(ro0 * ro0 + zz1 * zz1);
-                    const double rr1 = fmin(1.0, ro0 / aa1);
-                    eps = acos(rr1);
-                    th0 = 0.5 * PI + eps;
-                    th1 = PI;
-                  }
-                  const int np = q.nds * q.nds;
-                  for (int base = 0; base < np; base += 64) {
-                    const int t = base + lane;
-                    const bool ok = t < np;
-                    const int I = ok ? t / q.nds : 0, J = ok ? t % q.nds : 0;
+              // The angles of the transform depend on K (ffi, ro0), on (K, N)
+              // (the theta range) and on (K, N, I) (theta, ro1), not on J: a
+              // pre-pass evaluates them once per distinct value, as many
+              // (K, N, I) triples at a time as the table holds, and the
+              // stages read them back (the same expressions, so the same bits
+              // as evaluating them in every lane).
+              const int nds = q.nds, kb = kTrigTab / (3 * nds) > 0 ? kTrigTab / (3 * nds) : 1;
+              double *tab = lds + (9 + NC) * 64, *ktab = tab + 4 * kTrigTab;
+              for (int K0 = 0; K0 < nds; K0 += kb) {
+                const int K1 = K0 + kb < nds ? K0 + kb : nds;
+                const int ne = (K1 - K0) * 3 * nds;
+                for (int e0 = 0; e0 < ne; e0 += 64) {
+                  const int te = e0 + lane;
+                  if (te < ne && te < kTrigTab) {
+                    const int K = K0 + te / (3 * nds), N = (te / nds) % 3 + 1, I = te % nds;
+                    const double ffi = (ffi1 - ffi0) * q.gds[K] / 2.0 + (ffi1 + ffi0) / 2.0;
+                    const double ro0 = aa0 * sin(gama) / sin(ffi - ffi0 + gama);
+                    double aa1, eps, th0, th1;
+                    if (N == 1) {
+                      aa1 = 1.0 - zz1;
+                      eps = PI / 2.0;
+                      th0 = 0.0;
+                      th1 = acos(aa1 / sqrt(aa1 * aa1 + ro0 * ro0));
+                    } else if (N == 2) {
+                      aa1 = sqrt(ro0 * ro0 + (1.0 - zz1) * (1.0 - zz1));
+                      eps = acos((1.0 - zz1) / aa1);
+                      th0 = eps;
+                      const double rr1 = fmin(1.0, ro0 / sqrt(zz1 * zz1 + ro0 * ro0));
+                      th1 = 0.5 * PI + acos(rr1);
+                    } else {
+                      aa1 = sqrt(ro0 * ro0 + zz1 * zz1);
+                      const double rr1 = fmin(1.0, ro0 / aa1);
+                      eps = acos(rr1);
+                      th0 = 0.5 * PI + eps;
+                      th1 = PI;
+                    }
                     const double th = (th1 - th0) * q.gds[I] / 2.0 + (th1 + th0) / 2.0;
-                    const double ro1 = aa1 * sin(eps) / sin(th - th0 + eps);
-                    const double sth = sin(th), cth = cos(th);
-                    const double ro = (ro1 * q.gds[J] + ro1) / 2.0;
-                    double eti = 2.0 * (xx1 + ro * sth * cf) - 1.0;
-                    double etj = 2.0 * (yy1 + ro * sth * sf) - 1.0;
-                    double etk = 2.0 * (zz1 + ro * cth) - 1.0;
-                    eti = a + 0.5 * (eti + 1.0) * (b - a);
-                    etj = cc + 0.5 * (etj + 1.0) * (d - cc);
-                    etk = e + 0.5 * (etk + 1.0) * (f - e);
-                    const double wpre = dex * 1.953125e-3 * q.ods[J] * (q.ods[I] * q.ods[K]) * (ffi1 - ffi0) *
-                                        (th1 - th0) * ro1 * (ro * ro) * sth;
-                    stage(p, c, mu, ok, eti, etj, etk, wpre, true, lds, ax, ay, az);
+                    tab[te] = aa1 * sin(eps) / sin(th - th0 + eps);  // ro1
+                    tab[kTrigTab + te] = sin(th);
+                    tab[2 * kTrigTab + te] = cos(th);
+                    tab[3 * kTrigTab + te] = th1 - th0;
+                    if (N == 1 && I == 0) ktab[K - K0] = sin(ffi), ktab[kTrigTab + K - K0] = cos(ffi);
+                  }
+                }
+                __syncthreads();
+                for (int K = K0; K < K1; ++K) {
+                  const double sf = ktab[K - K0], cf = ktab[kTrigTab + K - K0];
+                  for (int N = 1; N <= 3; ++N) {
+                    const int np = nds * nds;
+                    for (int base = 0; base < np; base += 64) {
+                      const int t = base + lane;
+                      const bool ok = t < np;
+                      const int I = ok ? t / nds : 0, J = ok ? t % nds : 0;
+                      const int te = ((K - K0) * 3 + (N - 1)) * nds + I;
+                      const double ro1 = tab[te], sth = tab[kTrigTab + te], cth = tab[2 * kTrigTab + te];
+                      const double dth = tab[3 * kTrigTab + te];
+                      const double ro = (ro1 * q.gds[J] + ro1) / 2.0;
+                      double eti = 2.0 * (xx1 + ro * sth * cf) - 1.0;
+                      double etj = 2.0 * (yy1 + ro * sth * sf) - 1.0;
+                      double etk = 2.0 * (zz1 + ro * cth) - 1.0;
+                      eti = a + 0.5 * (eti + 1.0) * (b - a);
+                      etj = cc + 0.5 * (etj + 1.0) * (d - cc);
+                      etk = e + 0.5 * (etk + 1.0) * (f - e);
+                      const double wpre = dex * 1.953125e-3 * q.ods[J] * (q.ods[I] * q.ods[K]) * (ffi1 - ffi0) *
+                                          dth * ro1 * (ro * ro) * sth;
+                      stage(p, c, mu, ok, eti, etj, etk, wpre, true, lds, ax, ay, az);
+                    }
                   }
                 }
               }
