@@ -744,6 +744,9 @@ extern "C" int64_t tt_case_device_bytes(tt_case *c) {
 extern "C" void tt_case_free(tt_case *c) {
   if (!c) return;
   hipStream_t s = c->s;
+  // the case's last launches (graph replays included) finish before their
+  // buffers and executable graphs are released
+  if (s) (void)hipStreamSynchronize(s);
   delete c;
   if (s) (void)hipStreamDestroy(s);
 }

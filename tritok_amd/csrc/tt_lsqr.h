@@ -53,6 +53,7 @@ struct LsqrGraphs {
   const LsqrWork *wk = nullptr;      // work area the graphs were captured with
   unsigned gen = 0;                  // its allocation generation at capture
   int grid = 0;                      // cell-kernel grid at capture
+  int nb = 0;                        // unknown-kernel threads per workgroup at capture
   hipGraphExec_t exec[kLevels] = {};
   LsqrGraphs() = default;
   LsqrGraphs(const LsqrGraphs &) = delete;

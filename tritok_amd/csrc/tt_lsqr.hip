@@ -917,14 +917,14 @@ hipGraphExec_t capture_chunk(BlockSys &s, LsqrWork &wk, int prec, int iters, hip
 
 }  // namespace
 
-LsqrGraphs::LsqrGraphs(LsqrGraphs &&o) noexcept : wk(o.wk), gen(o.gen), grid(o.grid) {
+LsqrGraphs::LsqrGraphs(LsqrGraphs &&o) noexcept : wk(o.wk), gen(o.gen), grid(o.grid), nb(o.nb) {
   for (int i = 0; i < kLevels; ++i) exec[i] = o.exec[i], o.exec[i] = nullptr;
   o.wk = nullptr;
 }
 LsqrGraphs &LsqrGraphs::operator=(LsqrGraphs &&o) noexcept {
   if (this != &o) {
     reset();
-    wk = o.wk, gen = o.gen, grid = o.grid;
+    wk = o.wk, gen = o.gen, grid = o.grid, nb = o.nb;
     for (int i = 0; i < kLevels; ++i) exec[i] = o.exec[i], o.exec[i] = nullptr;
     o.wk = nullptr;
   }
@@ -933,7 +933,7 @@ LsqrGraphs &LsqrGraphs::operator=(LsqrGraphs &&o) noexcept {
 void LsqrGraphs::reset() {
   for (auto &e : exec)
     if (e) (void)hipGraphExecDestroy(e), e = nullptr;
-  wk = nullptr, gen = 0, grid = 0;
+  wk = nullptr, gen = 0, grid = 0, nb = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1206,9 +1206,9 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
   const int more = std::max(4, pred / 8);
   int extra = 0;  // batches after the first
   LsqrGraphs &G = s.graphs[prec];
-  if (graphs && (G.wk != &wk || G.gen != wk.gen || G.grid != s.grid_a[prec])) {
+  if (graphs && (G.wk != &wk || G.gen != wk.gen || G.grid != s.grid_a[prec] || G.nb != s.nb_b[prec])) {
     G.reset();
-    G.wk = &wk, G.gen = wk.gen, G.grid = s.grid_a[prec];
+    G.wk = &wk, G.gen = wk.gen, G.grid = s.grid_a[prec], G.nb = s.nb_b[prec];
   }
   // one iteration on the stream (sampled launches record their own events)
   auto one = [&](long pi) {
