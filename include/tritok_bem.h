@@ -307,6 +307,65 @@ int tt_sd_matvec(tt_sd *h, int32_t which, const double *x, double *y);
 void tt_sd_free(tt_sd *h);
 
 /* ------------------------------------------------------------------------ */
+/* Single-domain modified-Helmholtz operator (EQNS imH 3-7, main.f90:556-777
+ * and the solves of :1345-1395): the diffusion-convection equation with
+ * variable diffusivity solved with the modified-Helmholtz fundamental
+ * solution over the whole domain.  Rows: U rows are the nodes (source x(i),
+ * mu = sqrt(beta/diff(i))); Q rows the boundary flux points, element-major
+ * (row 4e + f, source xq(ibcf(4e + f)), mu = sqrt(beta/qdiff(ibcf))).  The
+ * ten matrices of each set (k = 0..9: H, G, B, Bp, Abdx, Abdy, Abdz, Dx, Dy,
+ * Dz) stay on the device, column-major with the row count as leading
+ * dimension; columns: boundary nodes (H, Abd), boundary flux points (G),
+ * nodes (B, Bp, D). */
+typedef struct tt_mhsd tt_mhsd;
+
+/* corMin_SDkin_mHfs (lib_fm.f:1646-1745) without the file: FMATsdmHfs
+ * (lib_fm.f:1183-1645, integrals with free terms and linear-field
+ * corrections) and MakeBprime (lib_fm.f:800-858).  ibcf: 4 nbelem flux-node
+ * ids (tt_host_mh_codes); diff(nnodes), qdiff(nq): diffusivities; beta > 0.
+ * ont[5] = ontH, ontG, ontDx, ontDy, ontDz; *mumax (may be NULL). */
+int tt_mhsd_create(const tt_mesh_view *m, const tt_quad *q, const int32_t *ibcf, const double *diff,
+                   const double *qdiff, double beta, tt_mhsd **out, double ont[5], double *mumax);
+/* The same operator from matrices already computed (the reference's cache
+ * branch of corMin_SDkin_mHfs, ReadmHfullIntDisk lib_fm.f:1746-1820, e.g.
+ * from tt_read_mhsd_int): U[k] nnodes rows, Q[k] 4 nbelem rows, column-major
+ * in the order H, G, B, Bp, Abdx, Abdy, Abdz, Dx, Dy, Dz. */
+int tt_mhsd_create_from(const tt_mesh_view *m, const int32_t *ibcf, const double *diff,
+                        const double *qdiff, double beta, const double *const U[10],
+                        const double *const Q[10], tt_mhsd **out);
+/* Copies matrix k (0..9) of row set `set` (0: U, 1: Q) into dst. */
+int tt_mhsd_matrix(tt_mhsd *h, int32_t set, int32_t k, double *dst);
+/* fmmHSysM (lib_fm.f:1061-1181) and FormPREfm(pret 2) (lib_Solve_v11.f:
+ * 403-451): the system matrix (nunk x nunk) and right-hand-side matrix
+ * (nunk x nknown) of the codes kode(nnodes), qkode(4 nbelem) as
+ * tt_host_mh_codes returns them, factorised on the device (LU and its
+ * inverse).  Errors: codes that do not number 1..nunk / 1..nknown once each,
+ * a singular system. */
+int tt_mhsd_system(tt_mhsd *h, const int32_t *kode, const int32_t *qkode, int32_t nunk, int32_t nknown);
+/* The right-hand-side matrices (B, Bp, Dx, Dy, Dz, Abdx, Abdy, Abdz of both
+ * sets) for the solves (main.f90:609-777): mode 0 dense (imH 3, 4); 1
+ * wavelet, WaveletCompSmat with kappa (imH 5); 2 cross approximation,
+ * AcaCompSmat with aca_type and aca_eps (imH 6); 3 block matrices with the
+ * library's own geometric partition (leaf, eta) and blocks of ct_type 1
+ * dense, 2 wavelet, 3 cross approximation (imH 7; the reference reads its
+ * cluster tree from a file instead).  *ratio (may be NULL) = stored /
+ * full entries. */
+int tt_mhsd_compress(tt_mhsd *h, int32_t mode, int32_t aca_type, double aca_eps, double wt_kappa,
+                     int32_t ct_type, double ct_eta, int32_t leaf, double *ratio);
+/* One solve, fmSolveModHelmDC / DCmm / DCwawt / DCaca / DChmat (lib_fm.f:
+ * 2-1059) by the current mode: r = RHS b plus the equation rows of B (-rhsv)
+ * + Bp u + D_c u (v_c + grad diff_c) - Abd_c u v_c, x = A^-1 r (SolvEQNfm
+ * with the LU), then u(i) = (1 - urmh) u(i) + urmh x on the unknown nodes and
+ * likewise q on the unknown flux points.  u(nnodes), qu(4 nbelem) in place;
+ * velocity, dgrad (nnodes x 3) and rhsv(nnodes) host arrays. */
+int tt_mhsd_solve(tt_mhsd *h, double *u, double *qu, const double *velocity, const double *rhsv,
+                  const double *dgrad, double urmh);
+/* Device times in ms of the last integrals, system, compression and solve,
+ * the compression ratio and the number of unknowns (any may be NULL). */
+int tt_mhsd_stats(const tt_mhsd *h, double ms[4], double *ratio, int32_t *nunk);
+void tt_mhsd_free(tt_mhsd *h);
+
+/* ------------------------------------------------------------------------ */
 /* Whole-case driver: the reference's main.f90 setup (:237-1084) and the
  * steady Picard loop (:1200-1589) with matrices resident in HBM.            */
 
@@ -461,6 +520,17 @@ int tt_host_case_view(tt_host *h, tt_case_view *v);
 /* Initial fields after ReadBic (pointers into h). */
 int tt_host_fields(tt_host *h, const double **vel, const double **vort, const double **qvort,
                    const double **temp, const double **qtemp);
+/* Modified-Helmholtz codes of the single-domain form after ReadBic's MHEL
+ * block (lib_bic.f:366-395) and its numbering (:463-480): kode(nn) as
+ * mesh%mHkode and qkode(4 nbelem) as mesh%mHqfmkode (> 0: known value
+ * number, < 0: minus the unknown number), the prescribed values u(nn) and
+ * q(4 nbelem), the flux-node ids ibcf (4 nbelem, 1-based), the wall of
+ * each boundary node iwn(nb) and of each boundary element bewn(nbelem).
+ * The boundary flux points are element-major: point f of element e is entry
+ * 4 e + f.  Pointers into h. */
+int tt_host_mh_codes(tt_host *h, const int32_t **kode, const int32_t **qkode, const double **u,
+                     const double **q, const int32_t **ibcf, const int32_t **iwn,
+                     const int32_t **bewn, int32_t *nunk, int32_t *nknown);
 /* Writes the mesh as a GiD .geo file the reference can read. */
 int tt_host_write_geo(tt_host *h, const char *path);
 void tt_host_free(tt_host *h);
@@ -485,6 +555,18 @@ int tt_write_macro_int(const char *path, int32_t nicell, int32_t nnodes, int32_t
 int tt_read_macro_int(const char *path, int32_t nicell, int32_t *nnodes, int32_t *nbnodes, double *H, double *G,
                       double *B, double *Abdx, double *Abdy, double *Abdz, double *Htx, double *Hty, double *Htz,
                       double *Dx, double *Dy, double *Dz, double ont[11]);
+/* WritemHfullIntDisk (lib_fm.f:1822-1895): tri.modHelm.full.int from the
+ * matrices tt_mhsd_matrix returns (U[k]: nnodes rows, Q[k]: nbnpof rows, the
+ * order H, G, B, Bp, Abdx, Abdy, Abdz, Dx, Dy, Dz); header version, nnodes,
+ * nbnodes, beta, sumdiff (the sum of diff in node order, the reference's
+ * cache key), mumax; ont[5] last. */
+int tt_write_mhsd_int(const char *path, int32_t version, int32_t nnodes, int32_t nbnodes, int32_t nbnpof,
+                      double beta, double sumdiff, double mumax, const double *const U[10],
+                      const double *const Q[10], const double ont[5]);
+/* ReadmHfullIntDisk (lib_fm.f:1746-1820): hdr_out = version, nnodes, nbnodes;
+ * hd_out = beta, sumdiff, mumax (CheckmHfullIntFile's key, :1897-1916). */
+int tt_read_mhsd_int(const char *path, int32_t nbnpof, int32_t hdr_out[3], double hd_out[3],
+                     double *const U[10], double *const Q[10], double ont[5]);
 /* WriteSDIntDisk (lib_sdkm.f:2641-2676): tri.sdkm.int.N for `rows` boundary
  * rows (the reference's env%nmn) from the arrays tt_sd_integrals returns;
  * header nnodes, nbnodes, nproc (the number of rank files; 1 for one GPU). */

@@ -404,3 +404,93 @@ def last_points():
     d, b = C.c_long(), C.c_long()
     _lastpts(C.byref(d), C.byref(b))
     return d.value, b.value
+
+
+# ---- single-domain modified Helmholtz (lib_fm.f) ----------------------------
+MHSD_NAMES = ["H", "G", "B", "Bp", "Abdx", "Abdy", "Abdz", "Dx", "Dy", "Dz"]
+_mhsd_int = _sig("or_mhsd_integrals", None, C.POINTER(OrQuad), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, PD, PI,
+                 PI, PI, PD, PI, PI, PI, PD, PD, C.c_double, C.POINTER(PD), C.POINTER(PD), PD, PD, C.c_int)
+_mhsd_bprime = _sig("or_mhsd_bprime", None, C.c_int, C.c_int, PI, PD, PD, C.c_double, PD, PD, PD, PD)
+_mhsd_sysm = _sig("or_mhsd_sysm", None, C.c_int, C.c_int, C.c_int, PI, PI, PI, PI, C.c_int, C.c_int, PD, PD, PD, PD,
+                  PD, PD, PD, PD)
+_mhsd_rhs = _sig("or_mhsd_rhs", None, C.c_int, C.c_int, C.c_int, PI, PI, PI, PI, C.c_int, C.c_int, PD, PD,
+                 C.c_double, PD, PD, PD, PD, PD, C.POINTER(PD), C.POINTER(PD), PD, PD)
+_mhsd_vectors = _sig("or_mhsd_vectors", None, C.c_int, C.c_int, PI, PD, PD, PD, PD, C.POINTER(PD))
+_mhsd_rhs_mv = _sig("or_mhsd_rhs_mv", None, C.c_int, C.c_int, PI, PI, C.c_int, C.c_int, PD, PD, PD, C.POINTER(PD),
+                    C.POINTER(PD), PD)
+
+
+def _ibcf_cm(ibcf_em, ne):
+    """element-major flux-node ids (4 ne) -> the reference's (ne, 4) column-major"""
+    return np.ascontiguousarray(np.asarray(ibcf_em, np.int32).reshape(ne, 4).T.reshape(-1))
+
+
+def mhsd_integrals(q, m, ibcf_em, diff, qdiff, beta, nthreads=8):
+    """FMATsdmHfs + MakeBprime.  m: Host.mesh_arrays(); ibcf_em: element-major
+    flux-node ids.  Returns dict U, Q (name -> array), ont, mumax."""
+    nn, nb, ne, nc, nq = m["nn"], m["nb"], m["ne"], m["nc"], m["nq"]
+    nf = 4 * ne
+    x, ibc, idc, ibf, xq = (np.asfortranarray(m[k]) for k in ("x", "ibc", "idc", "ibf", "xq"))
+    gbn, lbn = np.ascontiguousarray(m["gbn"]), np.ascontiguousarray(m["lbn"])
+    ibcf = _ibcf_cm(ibcf_em, ne)
+    cols = [nb, nf, nn, nn, nb, nb, nb, nn, nn, nn]
+    U = [np.zeros((nn, c), order="F") for c in cols]
+    Q = [np.zeros((nf, c), order="F") for c in cols]
+    diff = np.ascontiguousarray(diff, np.float64)
+    qdiff = np.ascontiguousarray(qdiff, np.float64)
+    ont = np.zeros(5)
+    mumax = C.c_double()
+    up = (PD * 10)(*[_d(a) for a in U])
+    qp = (PD * 10)(*[_d(a) for a in Q])
+    _mhsd_int(C.byref(q), nn, nb, ne, nc, nq, _d(x), _i(ibc), _i(idc), _i(ibf), _d(xq), _i(ibcf), _i(gbn), _i(lbn),
+              _d(diff), _d(qdiff), float(beta), up, qp, _d(ont), C.byref(mumax), nthreads)
+    _mhsd_bprime(nn, ne, _i(ibcf), _d(diff), _d(qdiff), float(beta), _d(U[2]), _d(U[3]), _d(Q[2]), _d(Q[3]))
+    return dict(U=dict(zip(MHSD_NAMES, U)), Q=dict(zip(MHSD_NAMES, Q)), ont=ont, mumax=mumax.value)
+
+
+def mhsd_sysm(m, ibcf_em, kode, qkode, nunk, nknown, diff, qdiff, mats):
+    nn, nb, ne = m["nn"], m["nb"], m["ne"]
+    ibcf = _ibcf_cm(ibcf_em, ne)
+    A = np.zeros((nunk, nunk), order="F")
+    R = np.zeros((nunk, max(nknown, 1)), order="F")
+    kode, qkode = np.ascontiguousarray(kode, np.int32), np.ascontiguousarray(qkode, np.int32)
+    _mhsd_sysm(nn, nb, ne, _i(ibcf), _i(np.ascontiguousarray(m["lbn"])), _i(kode), _i(qkode), nunk, nknown,
+               _d(np.ascontiguousarray(diff)), _d(np.ascontiguousarray(qdiff)), _d(mats["U"]["H"]), _d(mats["U"]["G"]),
+               _d(mats["Q"]["H"]), _d(mats["Q"]["G"]), _d(A), _d(R))
+    return A, R
+
+
+def mhsd_rhs(m, ibcf_em, kode, qkode, nunk, nknown, diff, qdiff, beta, u, qu, vel, rhsv, dgrad, mats, RHS):
+    """fmSolveModHelmDC's right-hand side (dense, inline form)."""
+    nn, nb, ne = m["nn"], m["nb"], m["ne"]
+    ibcf = _ibcf_cm(ibcf_em, ne)
+    r = np.zeros(nunk)
+    up = (PD * 10)(*[_d(mats["U"][k]) for k in MHSD_NAMES])
+    qp = (PD * 10)(*[_d(mats["Q"][k]) for k in MHSD_NAMES])
+    vel, dgrad = np.asfortranarray(vel, np.float64), np.asfortranarray(dgrad, np.float64)
+    _mhsd_rhs(nn, nb, ne, _i(ibcf), _i(np.ascontiguousarray(m["gbn"])), _i(np.ascontiguousarray(kode, np.int32)),
+              _i(np.ascontiguousarray(qkode, np.int32)), nunk, nknown, _d(np.ascontiguousarray(diff)),
+              _d(np.ascontiguousarray(qdiff)), float(beta), _d(np.ascontiguousarray(u)), _d(np.ascontiguousarray(qu)),
+              _d(vel), _d(np.ascontiguousarray(rhsv, np.float64)), _d(dgrad), up, qp, _d(RHS), _d(r))
+    return r
+
+
+def mhsd_vectors(m, u, vel, rhsv, dgrad):
+    """the eight product vectors of the matrix-vector variants"""
+    nn, nb = m["nn"], m["nb"]
+    v = [np.zeros(nn) for _ in range(5)] + [np.zeros(nb) for _ in range(3)]
+    vp = (PD * 8)(*[_d(a) for a in v])
+    _mhsd_vectors(nn, nb, _i(np.ascontiguousarray(m["gbn"])), _d(np.ascontiguousarray(u)),
+                  _d(np.asfortranarray(vel, np.float64)), _d(np.ascontiguousarray(rhsv, np.float64)),
+                  _d(np.asfortranarray(dgrad, np.float64)), vp)
+    return v
+
+
+def mhsd_rhs_mv(m, kode, qkode, nunk, nknown, u, qu, RHS, yu, yq):
+    r = np.zeros(nunk)
+    yu = [np.ascontiguousarray(a, np.float64) for a in yu]
+    yq = [np.ascontiguousarray(a, np.float64) for a in yq]
+    _mhsd_rhs_mv(m["nn"], m["ne"], _i(np.ascontiguousarray(kode, np.int32)), _i(np.ascontiguousarray(qkode, np.int32)),
+                 nunk, nknown, _d(np.ascontiguousarray(u)), _d(np.ascontiguousarray(qu)), _d(RHS),
+                 (PD * 8)(*[_d(a) for a in yu]), (PD * 8)(*[_d(a) for a in yq]), _d(r))
+    return r

@@ -126,6 +126,42 @@ void or_sd_integrals(const or_quad *q, int nn, int nb, int ne, int nc, const dou
                      int row1, int ld, double *H, double *Htx, double *Hty, double *Htz,
                      double *Dx, double *Dy, double *Dz, double ont[10], int nthreads);
 
+/* ---- single-domain modified Helmholtz (EQNS imH 3-7, lib_fm.f) ---------- */
+
+/* FMATsdmHfs (lib_fm.f:1183-1645).  U rows: nodes; Q rows: boundary flux
+ * points element-major (row 4e + f, source xq(ibcf(e, f))).  U[10], Q[10]:
+ * H, G, B, Bp, Abdx, Abdy, Abdz, Dx, Dy, Dz, column-major with ld = nn / 4 ne
+ * (Bp is left zero: or_mhsd_bprime).  ibcf: ne x 4 column-major (1-based flux
+ * node ids).  ont = ontH, ontG, ontDx, ontDy, ontDz. */
+void or_mhsd_integrals(const or_quad *q, int nn, int nb, int ne, int nc, int nq, const double *x,
+                       const int *ibc, const int *idc, const int *ibf, const double *xq,
+                       const int *ibcf, const int *gbn, const int *lbn, const double *diff,
+                       const double *qdiff, double beta, double **U, double **Q, double ont[5],
+                       double *mumax, int nthreads);
+/* MakeBprime (lib_fm.f:800-858) */
+void or_mhsd_bprime(int nn, int ne, const int *ibcf, const double *diff, const double *qdiff,
+                    double beta, const double *UB, double *UBp, const double *QB, double *QBp);
+/* fmmHSysM (lib_fm.f:1061-1181): kode(nn), qkode(4 ne) as mesh%mHkode /
+ * mHqfmkode; A (nunk x nunk), RHS (nunk x nknown) column-major. */
+void or_mhsd_sysm(int nn, int nb, int ne, const int *ibcf, const int *lbn, const int *kode,
+                  const int *qkode, int nunk, int nknown, const double *diff, const double *qdiff,
+                  const double *UH, const double *UG, const double *QH, const double *QG, double *A,
+                  double *RHS);
+/* fmSolveModHelmDC's right-hand side (lib_fm.f:905-1016); velocity and dgrad
+ * nn x 3 column-major. */
+void or_mhsd_rhs(int nn, int nb, int ne, const int *ibcf, const int *gbn, const int *kode,
+                 const int *qkode, int nunk, int nknown, const double *diff, const double *qdiff,
+                 double beta, const double *u, const double *qu, const double *velocity,
+                 const double *rhsv, const double *dgrad, double **U, double **Q, const double *RHS,
+                 double *r);
+/* the product vectors and right-hand side of the matrix-vector variants
+ * (fmSolveModHelmDCaca / wawt / hmat / mm) */
+void or_mhsd_vectors(int nn, int nb, const int *gbn, const double *u, const double *velocity,
+                     const double *rhsv, const double *dgrad, double **v);
+void or_mhsd_rhs_mv(int nn, int ne, const int *kode, const int *qkode, int nunk, int nknown,
+                    const double *u, const double *qu, const double *RHS, double **yu, double **yq,
+                    double *r);
+
 /* ---- compressed matrices (lib_ACA.f, lib_WT.f, lib_ctree.f) ------------ */
 
 /* Cross approximation of R (m x n, column-major, overwritten by the residual):

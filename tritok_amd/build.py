@@ -27,7 +27,7 @@ NO_CONTRACT = {"tt_compress.hip"}
 # sources whose matrix-core loops keep their accumulators in VGPRs (without
 # the option the compiler copies loop-carried accumulators between VGPRs and
 # AGPRs around every product chain)
-MFMA_VGPR = {"tt_integrals.hip", "tt_dense.hip"}
+MFMA_VGPR = {"tt_integrals.hip", "tt_dense.hip", "tt_mhsd.hip"}
 
 
 def _run(cmd):

@@ -72,4 +72,8 @@ namespace tt {
 // n x 3 column-major); the caller's stream work on A must be complete.
 tt_hmat *hmat_from_device(int nrow, int ncol, const double *A, long lda, const double *xr, const double *xc,
                           int leaf, double eta, int ctype, int aca_type, double aca_eps, double wt_kappa);
+// y = H x for device vectors x (ncol) and y (nrow) on stream s.
+void hmat_apply(tt_hmat *h, const double *x, double *y, hipStream_t s);
+// doubles the compressed form holds
+long hmat_stored_entries(const tt_hmat *h);
 }  // namespace tt

@@ -1248,6 +1248,12 @@ tt_hmat *hmat_from_device(int nrow, int ncol, const double *A, long lda, const d
   h->x.alloc((size_t)ncol * 4), h->y.alloc((size_t)nrow * 4);
   return h.release();
 }
+
+void hmat_apply(tt_hmat *h, const double *x, double *y, hipStream_t s) {
+  hmat_mv_dev(h->d, 1, x, (long)h->d.ncol, y, (long)h->d.nrow, s);
+}
+
+long hmat_stored_entries(const tt_hmat *h) { return h->d.stored; }
 }  // namespace tt
 
 extern "C" int tt_hmat_create(int32_t nrow, int32_t ncol, const double *full, int64_t lda, const double *xr,

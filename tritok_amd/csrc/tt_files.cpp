@@ -230,6 +230,52 @@ void rst_layout(int32_t nn, int32_t nq, RstLayout out[17]) {
 }
 }  // namespace
 
+// WritemHfullIntDisk (lib_fm.f:1822-1895): version, (nnodes, nbnodes), beta,
+// sum of diff, mumax, the ten U matrices (nnodes rows) and the ten Q
+// matrices (nbnpof rows) in the order H, G, B, Bp, Abdx, Abdy, Abdz, Dx, Dy,
+// Dz, then the five accuracy estimates.
+extern "C" int tt_write_mhsd_int(const char *path, int32_t version, int32_t nnodes, int32_t nbnodes, int32_t nbnpof,
+                                 double beta, double sumdiff, double mumax, const double *const U[10],
+                                 const double *const Q[10], const double ont[5]) {
+  return guard("tt_write_mhsd_int", [&] {
+    if (nnodes < 1 || nbnodes < 1 || nbnpof < 1 || !U || !Q || !ont) throw std::runtime_error("bad argument");
+    const int32_t cols[10] = {nbnodes, nbnpof, nnodes, nnodes, nbnodes, nbnodes, nbnodes, nnodes, nnodes, nnodes};
+    for (int k = 0; k < 10; ++k)
+      if (!U[k] || !Q[k]) throw std::runtime_error("NULL matrix");
+    File f(path, "wb");
+    const int32_t hdr[2] = {nnodes, nbnodes};
+    f.record(&version, 4);
+    f.record(hdr, 8);
+    f.record(&beta, 8), f.record(&sumdiff, 8), f.record(&mumax, 8);
+    for (int k = 0; k < 10; ++k) f.mat(U[k], nnodes, cols[k]);
+    for (int k = 0; k < 10; ++k) f.mat(Q[k], nbnpof, cols[k]);
+    f.record(ont, 5 * sizeof(double));
+    return 0;
+  });
+}
+
+// ReadmHfullIntDisk (lib_fm.f:1746-1820) after CheckmHfullIntFile (:1897-1916):
+// hdr_out = version, nnodes, nbnodes; hd_out = beta, sum of diff, mumax.
+extern "C" int tt_read_mhsd_int(const char *path, int32_t nbnpof, int32_t hdr_out[3], double hd_out[3],
+                                double *const U[10], double *const Q[10], double ont[5]) {
+  return guard("tt_read_mhsd_int", [&] {
+    if (nbnpof < 1 || !hdr_out || !hd_out || !U || !Q || !ont) throw std::runtime_error("bad argument");
+    File f(path, "rb");
+    int32_t ver = 0, hdr[2] = {0, 0};
+    f.read_record(&ver, 4);
+    f.read_record(hdr, 8);
+    hdr_out[0] = ver, hdr_out[1] = hdr[0], hdr_out[2] = hdr[1];
+    f.read_record(&hd_out[0], 8), f.read_record(&hd_out[1], 8), f.read_record(&hd_out[2], 8);
+    const int32_t nnodes = hdr[0], nbnodes = hdr[1];
+    if (nnodes < 1 || nbnodes < 1) throw std::runtime_error("bad header");
+    const int32_t cols[10] = {nbnodes, nbnpof, nnodes, nnodes, nbnodes, nbnodes, nbnodes, nnodes, nnodes, nnodes};
+    for (int k = 0; k < 10; ++k) f.read_mat(U[k], nnodes, cols[k]);
+    for (int k = 0; k < 10; ++k) f.read_mat(Q[k], nbnpof, cols[k]);
+    f.read_record(ont, 5 * sizeof(double));
+    return 0;
+  });
+}
+
 extern "C" int tt_write_restart(const char *path, int32_t nnodes, int32_t nq, int32_t tstep, int32_t glnnlit,
                                 double rtime, const double *const fields[17]) {
   return guard("tt_write_restart", [&] {

@@ -41,6 +41,11 @@ struct tt_host {
   std::vector<int> iwn, kode, wkode, wqkode, Tkode, Tqkode, kmu, kobc;
   int nunk[3] = {0, 0, 0}, nbk[3] = {0, 0, 0}, Tnunk = 0, Tnb = 0, nsmcol = 0, nrhscol = 0;
   std::vector<double> vel, vort, qvort, temp, qtemp;
+  // modified-Helmholtz codes of the single-domain form: nodes (nn) and the
+  // boundary flux points element-major (4 ne); values prescribed by MHEL
+  std::vector<int> mHkode, mHqfmkode, ibcf_em;
+  std::vector<double> mHu, fmqmH;
+  int mHnb = 0, mHfmnunk = 0;
   // tri.inp
   tt_quad quad{8, 8, 4, 4, 1};
   int copy = 0, iBw = 1, iDv = 1, iDw = 1, iDT = 0, iDC = 0, imH = 0;
@@ -458,6 +463,11 @@ extern "C" int tt_host_read_bic(tt_host *hp, const char *bic_text) {
   h.qvort.assign((size_t)nq * 3, 0.0);
   h.temp.assign(nn, 0.0);
   h.qtemp.assign(nq, 0.0);
+  h.mHkode.assign(nn, 0);
+  h.mHqfmkode.assign((size_t)ne * 4, 0);
+  h.mHu.assign(nn, 0.0);
+  h.fmqmH.assign((size_t)ne * 4, 0.0);
+  h.mHnb = h.mHfmnunk = 0;
   // walls of the elements containing each boundary node (bewniwnEdge/Corner, lib_bic.f:506-576)
   std::vector<std::vector<int>> node_walls(nb);
   for (int e = 0; e < ne; ++e)
@@ -585,7 +595,30 @@ extern "C" int tt_host_read_bic(tt_host *hp, const char *bic_text) {
               }
         }
       });
-    } else if (kw == "GEDC" || kw == "MHEL") {
+    } else if (kw == "MHEL") {
+      // modified Helmholtz: 1 wall value (u on the wall's nodes), 2 wall value
+      // (q on the wall's elements' flux points) (ReadBic, lib_bic.f:366-395)
+      block([&] {
+        auto v = toks(line);
+        int ff = std::atoi(v[0].c_str()), wall = std::atoi(v[1].c_str());
+        double val = fnum(v[2]);
+        if (ff == 1) {
+          for (int i = 0; i < nb; ++i)
+            if (h.iwn[i] == wall) {
+              int dn = h.gbn[i] - 1;
+              h.mHu[dn] = val;
+              h.mHkode[dn] = ++h.mHnb;
+            }
+        } else if (ff == 2) {
+          for (int e = 0; e < ne; ++e)
+            if (h.bewn[e] == wall)
+              for (int j = 0; j < 4; ++j) {
+                h.fmqmH[(size_t)e * 4 + j] = val;
+                h.mHqfmkode[(size_t)e * 4 + j] = ++h.mHnb;
+              }
+        }
+      });
+    } else if (kw == "GEDC") {
       block([&] {});
     }
   }
@@ -602,6 +635,14 @@ extern "C" int tt_host_read_bic(tt_host *hp, const char *bic_text) {
   }
   for (int i = 0; i < nn; ++i)
     if (h.Tkode[i] == 0) h.Tkode[i] = -(++h.Tnunk);
+  // modified Helmholtz, single domain: nodes, then the boundary flux points (lib_bic.f:463-480)
+  for (int i = 0; i < nn; ++i)
+    if (h.mHkode[i] == 0) h.mHkode[i] = -(++h.mHfmnunk);
+  for (size_t i = 0; i < h.mHqfmkode.size(); ++i)
+    if (h.mHqfmkode[i] == 0) h.mHqfmkode[i] = -(++h.mHfmnunk);
+  h.ibcf_em.resize((size_t)ne * 4);
+  for (int e = 0; e < ne; ++e)
+    for (int j = 0; j < 4; ++j) h.ibcf_em[(size_t)e * 4 + j] = h.ibcf[e + (size_t)ne * j];
   for (int i = 0; i < nq; ++i)
     if (h.Tqkode[i] == 0) h.Tqkode[i] = -(++h.Tnunk);
   // DetKMunk3
@@ -637,6 +678,19 @@ extern "C" int tt_host_read_bic(tt_host *hp, const char *bic_text) {
     }
   }
   h.have_bc = true;
+  return 0;
+}
+
+extern "C" int tt_host_mh_codes(tt_host *h, const int32_t **kode, const int32_t **qkode, const double **u,
+                                const double **q, const int32_t **ibcf, const int32_t **iwn,
+                                const int32_t **bewn, int32_t *nunk, int32_t *nknown) {
+  if (!h || !h->have_bc) {
+    tt_set_error("tt_host_mh_codes: no boundary conditions read");
+    return -1;
+  }
+  *kode = h->mHkode.data(), *qkode = h->mHqfmkode.data(), *u = h->mHu.data(), *q = h->fmqmH.data();
+  *ibcf = h->ibcf_em.data(), *iwn = h->iwn.data(), *bewn = h->bewn.data();
+  *nunk = h->mHfmnunk, *nknown = h->mHnb;
   return 0;
 }
 

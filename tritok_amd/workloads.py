@@ -48,6 +48,17 @@ def bic_cavity(lid=1.0):
     return "\n".join(lines) + "\n"
 
 
+def bic_mh(dirichlet=(1, 3, 5), neumann=(2, 4, 6)):
+    """Modified-Helmholtz boundary conditions (MHEL block, lib_bic.f:366-395):
+    u prescribed on the `dirichlet` walls, the normal flux on the `neumann`
+    walls; edges and corners belong to walls as in the cavity files.  The
+    values written are placeholders: the manufactured problems set them."""
+    lines = ["# modified Helmholtz, single domain", "EDGE"] + ["%d %d %d" % e for e in _EDGES_CAVITY]
+    lines += ["END", "CORNER"] + ["%d %d %d %d" % c for c in _CORNERS_CAVITY] + ["END", "MHEL"]
+    lines += ["1 %d 0.0" % w for w in dirichlet] + ["2 %d 0.0" % w for w in neumann] + ["END"]
+    return "\n".join(lines) + "\n"
+
+
 def bic_heated(t_x0=0.5, t_x1=-0.5):
     """Differentially heated cube: T fixed on x = 0 and x = 1, adiabatic elsewhere."""
     lines = ["# differentially heated cube", "EDGE"] + ["%d %d %d" % e for e in _EDGES_HEATED] + ["END", "CORNER"]
