@@ -416,9 +416,77 @@ MODULE tritok_amd_bind
       REAL(C_DOUBLE), INTENT(IN) :: err(*)
       INTEGER(C_INT) :: tt_write_sta_ite
     END FUNCTION
+
+    ! single-domain modified Helmholtz (EQNS imH 3-7): corMin_SDkin_mHfs
+    ! (lib_fm.f:1646), fmmHSysM + FormPREfm (:1061), the compressions of
+    ! main.f90:609-777 and fmSolveModHelmDC / DCwawt / DCaca / DChmat
+    FUNCTION tt_host_mh_codes(h, kode, qkode, u, q, ibcf, iwn, bewn, nunk, nknown) BIND(C, NAME="tt_host_mh_codes")
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      TYPE(C_PTR) :: kode, qkode, u, q, ibcf, iwn, bewn
+      INTEGER(C_INT32_T) :: nunk, nknown
+      INTEGER(C_INT) :: tt_host_mh_codes
+    END FUNCTION
+    FUNCTION tt_mhsd_create(m, q, ibcf, diff, qdiff, beta, h, ont, mumax) BIND(C, NAME="tt_mhsd_create")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR, tt_mesh_view, tt_quad
+      TYPE(tt_mesh_view), INTENT(IN) :: m
+      TYPE(tt_quad), INTENT(IN) :: q
+      INTEGER(C_INT32_T), INTENT(IN) :: ibcf(*)
+      REAL(C_DOUBLE), INTENT(IN) :: diff(*), qdiff(*)
+      REAL(C_DOUBLE), VALUE :: beta
+      TYPE(C_PTR) :: h
+      REAL(C_DOUBLE) :: ont(5), mumax
+      INTEGER(C_INT) :: tt_mhsd_create
+    END FUNCTION
+    FUNCTION tt_mhsd_matrix(h, set, k, dst) BIND(C, NAME="tt_mhsd_matrix")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: set, k
+      REAL(C_DOUBLE) :: dst(*)
+      INTEGER(C_INT) :: tt_mhsd_matrix
+    END FUNCTION
+    FUNCTION tt_mhsd_system(h, kode, qkode, nunk, nknown) BIND(C, NAME="tt_mhsd_system")
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), INTENT(IN) :: kode(*), qkode(*)
+      INTEGER(C_INT32_T), VALUE :: nunk, nknown
+      INTEGER(C_INT) :: tt_mhsd_system
+    END FUNCTION
+    FUNCTION tt_mhsd_compress(h, mode, aca_type, aca_eps, wt_kappa, ct_type, ct_eta, leaf, ratio) &
+        BIND(C, NAME="tt_mhsd_compress")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: mode, aca_type, ct_type, leaf
+      REAL(C_DOUBLE), VALUE :: aca_eps, wt_kappa, ct_eta
+      REAL(C_DOUBLE) :: ratio
+      INTEGER(C_INT) :: tt_mhsd_compress
+    END FUNCTION
+    FUNCTION tt_mhsd_solve(h, u, qu, velocity, rhsv, dgrad, urmh) BIND(C, NAME="tt_mhsd_solve")
+      IMPORT :: C_INT, C_DOUBLE, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE) :: u(*), qu(*)
+      REAL(C_DOUBLE), INTENT(IN) :: velocity(*), rhsv(*), dgrad(*)
+      REAL(C_DOUBLE), VALUE :: urmh
+      INTEGER(C_INT) :: tt_mhsd_solve
+    END FUNCTION
+    SUBROUTINE tt_mhsd_free(h) BIND(C, NAME="tt_mhsd_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
   END INTERFACE
 
 CONTAINS
+
+  ! fmSolveModHelmDC (and its wavelet / ACA / block-matrix forms, by the
+  ! operator's compression mode) with the arrays of the reference's call
+  ! (main.f90:1349-1352); the matrices live behind h (tt_mhsd_create,
+  ! tt_mhsd_system, tt_mhsd_compress)
+  SUBROUTINE fmSolveModHelmDC_gpu(h, nnodes, nbnpof, u, qu, velocity, rhsv, diffGrad, urmH)
+    TYPE(C_PTR) :: h
+    INTEGER :: nnodes, nbnpof
+    REAL(8) :: u(nnodes), qu(nbnpof), velocity(nnodes, 3), rhsv(nnodes), diffGrad(nnodes, 3), urmH
+    IF (tt_mhsd_solve(h, u, qu, velocity, rhsv, diffGrad, urmH) < 0) CALL tt_fail('fmSolveModHelmDC_gpu')
+  END SUBROUTINE
 
   ! prelsqr2 with the reference's arguments; h receives the device handle
   ! that slvlsqr2_gpu takes (a is scaled in place, as prelsqr2 does)

@@ -306,53 +306,68 @@ class Operator:
             pass
 
 
-def run(host, q, f, tstep, nstep, nnlit=3, eps=1e-12, urmh=1.0, mode="dense", on_step=None, matrices=None,
-        **comp):
-    """The reference's time loop for imH >= 3 (main.f90:1124-1395, 1618-1671):
-    set-up at t = 0 (modHelm_SetProblem, setGRAD, Int2MacroNodes, integrals,
-    system), then per step the time-derivative source, up to nnlit fixed-point
-    iterations (SetProblem, solve, change check as CheckConvergencevw's mH
-    entry, lib_util.f:1048-1150) and the RMS error.  Returns the list of
-    per-step dicts (u, q, rms, changes) and the operator's timings."""
+def prepare(host, f, tstep):
+    """The host-side set-up at t = 0 (main.f90:532-554, 1124-1130): codes,
+    modHelm_SetProblem, setGRAD of the diffusivity, Int2MacroNodes, and the
+    initial u and q with the prescribed values."""
     m = host.mesh_arrays()
     cd = codes(host)
     beta, beta2, beta3 = time_scheme(tstep)
     u = np.ascontiguousarray(cd["u"], np.float64).copy()
     qu = np.ascontiguousarray(cd["q"], np.float64).copy()
-    zero = np.zeros(m["nn"])
-    p = set_problem(m, cd, f, 0.0, zero)
+    p = set_problem(m, cd, f, 0.0, np.zeros(m["nn"]))
     u[cd["kode"] > 0] = p["u_known"][cd["kode"] > 0]
     qu[cd["qkode"] > 0] = p["q_known"][cd["qkode"] > 0]
     diff = p["diff"]
-    dgrad = set_grad(m, diff)
-    qdiff = int2macro(m, diff)
-    out = []
-    with Operator(host.view(), q, cd, diff, qdiff, beta, matrices=matrices) as op:
+    return dict(m=m, cd=cd, f=f, tstep=tstep, beta=beta, beta2=beta2, beta3=beta3, diff=diff,
+                qdiff=int2macro(m, diff), dgrad=set_grad(m, diff), u0=u, q0=qu)
+
+
+def time_loop(op, pr, nstep, nnlit=3, eps=1e-12, urmh=1.0, on_step=None):
+    """The reference's time loop for imH >= 3 (main.f90:1618-1671): per step
+    the time-derivative source, up to nnlit fixed-point iterations
+    (SetProblem, solve, the change check of CheckConvergencevw's mH entry,
+    lib_util.f:1048-1150) and the RMS error.  Returns per-step dicts (t, u, q,
+    rms, changes, solve_ms: device time of each solve)."""
+    m, cd, f = pr["m"], pr["cd"], pr["f"]
+    u, qu = pr["u0"].copy(), pr["q0"].copy()
+    pts, ppts, pnls = u.copy(), u.copy(), u.copy()
+    t, out = 0.0, []
+    for step in range(nstep):
+        t = t + pr["tstep"]
+        ppts, pts = pts, u.copy()
+        mhtime = -pts * pr["beta2"] + ppts * pr["beta3"]
+        changes, ms = [], []
+        for it in range(nnlit):
+            p = set_problem(m, cd, f, t, mhtime)
+            u[cd["kode"] > 0] = p["u_known"][cd["kode"] > 0]
+            qu[cd["qkode"] > 0] = p["q_known"][cd["qkode"] > 0]
+            op.solve(u, qu, p["vel"], p["rhs"], pr["dgrad"], urmh)
+            ms.append(op.stats()["ms_solve"])
+            dif = np.sum((u - pnls) ** 2)
+            nrm = np.sum(u ** 2)
+            ch = float(np.sqrt(dif / nrm)) if nrm > eps else 0.0
+            pnls = u.copy()
+            changes.append(ch)
+            if ch < eps:
+                break
+        rec = dict(t=t, u=u.copy(), q=qu.copy(), rms=rms_error(m, u, p["anal"]), changes=changes, solve_ms=ms)
+        out.append(rec)
+        if on_step:
+            on_step(rec)
+    return out
+
+
+def run(host, q, f, tstep, nstep, nnlit=3, eps=1e-12, urmh=1.0, mode="dense", on_step=None, matrices=None,
+        **comp):
+    """prepare(), the operator (integrals or given matrices, system,
+    compression by mode) and time_loop().  Returns the per-step records and
+    the operator's timings."""
+    pr = prepare(host, f, tstep)
+    with Operator(host.view(), q, pr["cd"], pr["diff"], pr["qdiff"], pr["beta"], matrices=matrices) as op:
         op.system()
         ratio = op.compress(mode, **comp)
-        pts, ppts, pnls = u.copy(), u.copy(), u.copy()
-        t = 0.0
-        for step in range(nstep):
-            t = t + tstep
-            ppts, pts = pts, u.copy()
-            mhtime = -pts * beta2 + ppts * beta3
-            changes = []
-            for it in range(nnlit):
-                p = set_problem(m, cd, f, t, mhtime)
-                u[cd["kode"] > 0] = p["u_known"][cd["kode"] > 0]
-                qu[cd["qkode"] > 0] = p["q_known"][cd["qkode"] > 0]
-                op.solve(u, qu, p["vel"], p["rhs"], dgrad, urmh)
-                dif = np.sum((u - pnls) ** 2)
-                nrm = np.sum(u ** 2)
-                ch = float(np.sqrt(dif / nrm)) if nrm > eps else 0.0
-                pnls = u.copy()
-                changes.append(ch)
-                if ch < eps:
-                    break
-            rec = dict(t=t, u=u.copy(), q=qu.copy(), rms=rms_error(m, u, p["anal"]), changes=changes)
-            out.append(rec)
-            if on_step:
-                on_step(rec)
+        out = time_loop(op, pr, nstep, nnlit, eps, urmh, on_step)
         st = op.stats()
     st["ratio"] = ratio
     return out, st
