@@ -331,6 +331,55 @@ def aca_kinematics_leg(n, leaf=64, eta=1.0, eps=1e-6, reps=5):
     return out
 
 
+def mhsd_leg(n, f=7, tstep=0.1, nstep=2, nnlit=3):
+    """The single-domain modified-Helmholtz path (EQNS imH 3, 5, 6 of the
+    reference) on the n^3 cube: u prescribed on walls 1, 3, 5, the normal flux
+    on 2, 4, 6, manufactured problem f (diff = 2 + sin(pi/2 xyz)); integrals,
+    system and LU inverse once, then per compression mode (dense, wavelet,
+    cross approximation) two time steps of three fixed-point solves each,
+    with the device time of every solve and the RMS error against the
+    analytic solution.  Dense solve roofline: the eight right-hand-side
+    matrices of the equation rows, RHS and A^-1 streamed once per solve."""
+    from tritok_amd import Host, mhsd, workloads
+    from tritok_amd import integrals as I
+    host = Host.cube(n).read_bic(workloads.bic_mh())
+    t0 = time.perf_counter()
+    pr = mhsd.prepare(host, f, tstep)
+    prep_s = time.perf_counter() - t0
+    cd = pr["cd"]
+    m = pr["m"]
+    nn, nb, nf, nunk, nknown = m["nn"], m["nb"], 4 * m["ne"], cd["nunk"], cd["nknown"]
+    out = {"workload": "%d^3 cube, %d nodes + %d boundary flux points as sources, %d unknowns, manufactured problem "
+                       "%d, %d steps x %d solves" % (n, nn, nf, nunk, f, nstep, nnlit),
+           "host_prepare_s": prep_s}
+    with mhsd.Operator(host.view(), I.quad(4, 4, 4, 4, 2), cd, pr["diff"], pr["qdiff"], pr["beta"]) as op:
+        op.system()
+        st = op.stats()
+        out.update(integrals_ms=st["ms_integrals"], system_ms=st["ms_system"],
+                   matrices_GB=8.0 * (nn + nf) * (5 * nn + 4 * nb + nf) / 1e9)
+        dense_bytes = 8.0 * (nunk * (5 * nn + 3 * nb) + nunk * nknown + nunk * nunk)
+        for mode, kw in (("dense", {}), ("wavelet", dict(wt_kappa=1e-5)), ("aca", dict(aca_type=2, aca_eps=1e-6))):
+            ratio = op.compress(mode, **kw)
+            cms = op.stats()["ms_compress"] if mode != "dense" else 0.0
+            steps = mhsd.time_loop(op, pr, nstep, nnlit)
+            ms = np.array([x for r in steps for x in r["solve_ms"]])
+            rec = {"compress_ms": cms, "stored_over_full": ratio, "solve_ms_mean": float(ms.mean()),
+                   "solve_ms_min": float(ms.min()), "rms_error": [r["rms"] for r in steps]}
+            if mode == "dense":
+                gbs = dense_bytes / (ms.min() * 1e-3) / 1e9
+                rec["roofline"] = {"bound": "hbm", "bytes": dense_bytes, "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                   "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+                u_dense = steps[-1]["u"]
+            else:
+                rec["max_rel_vs_dense"] = float(np.abs(steps[-1]["u"] - u_dense).max() / np.abs(u_dense).max())
+            out[mode] = rec
+            log("mhsd %d^3 %s: ratio %.3f, compress %.1f ms, solve %.3f ms (min %.3f), rms %s" % (
+                n, mode, ratio, cms, rec["solve_ms_mean"], rec["solve_ms_min"], rec["rms_error"]))
+    log("mhsd %d^3: integrals %.1f ms, system %.1f ms, matrices %.2f GB" % (n, out["integrals_ms"], out["system_ms"],
+                                                                           out["matrices_GB"]))
+    return out
+
+
 FLOP_SDKM_DOMAIN = 3 * 27 * 2 + 15  # three gradient outputs x 27 shape functions, plus the kernel
 
 
@@ -437,6 +486,9 @@ def main():
                     help="independent cavity systems solved in one launch (0: skip)")
     ap.add_argument("--aca-kin", type=int, default=20,
                     help="n: ACA-compressed single-domain H of an n^3 cube (0: skip)")
+    ap.add_argument("--mhsd", type=int, default=8,
+                    help="n: single-domain modified-Helmholtz solves on an n^3 cube (0: skip)")
+    ap.add_argument("--mhsd-only", action="store_true", help="only the --mhsd leg, then exit")
     ap.add_argument("--mixed-kin", type=int, default=24,
                     help="n: mixed-precision kinematics solve on an n^3 cube (0: skip)")
     ap.add_argument("--assembly-only", type=int, default=0,
@@ -453,6 +505,9 @@ def main():
 
     if args.assembly_only:
         assembly_only(args.assembly_only)
+        return
+    if args.mhsd_only:
+        log("mhsd only: " + json.dumps(mhsd_leg(args.mhsd)))
         return
     if args.mixed_kin_only:
         mk = mixed_kinematics_leg(args.mixed_kin)
@@ -552,6 +607,7 @@ def main():
     acc = accuracy_leg(args.accuracy_n, args.mixed, args.anchor_period) if args.accuracy_n > 0 else None
     mixkin = mixed_kinematics_leg(args.mixed_kin) if args.mixed_kin > 0 else None
     acakin = aca_kinematics_leg(args.aca_kin) if args.aca_kin > 0 else None
+    mh_sd = mhsd_leg(args.mhsd) if args.mhsd > 0 else None
     batched = batched_integrals_leg(args.batch_cells) if args.batch_cells > 0 else None
     bsolves = batched_solves_leg(args.batch_solves) if args.batch_solves > 0 else None
     serving = (serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, ("fp64", args.mixed))
@@ -586,7 +642,7 @@ def main():
         "setup_s": setup_s, "device": dev, "roofline": roofline, "cpu_baseline": cpu,
         "p99_solve_latency_s": serving[args.mixed]["latency_s"]["p99"] if serving else None,
         "serving": serving, "batched_integrals": batched, "batched_solves": bsolves, "mixed_kinematics": mixkin,
-        "aca_kinematics": acakin,
+        "aca_kinematics": acakin, "single_domain_mh": mh_sd,
         "assembly": assembly,
     }
     print(json.dumps(out), flush=True)

@@ -37,10 +37,13 @@ void wt_mv_dev(WtDev &w, const double *x, double *y, hipStream_t s);
 struct AcaDev {
   int m = 0, n = 0, rank = 0;
   DBuf<double> a, b, t;
+  DBuf<double> part;  // column-slice partial sums of the two products
 };
 int aca_compress_dev(double *R, int m, int n, int type, double eps, const double *vec_host, AcaDev &out,
                      double *norm, hipStream_t s);
 void aca_mv_dev(AcaDev &h, const double *x, double *y, hipStream_t s);
+// the product's work buffers for factors set by hand (m, n, rank, a, b)
+void aca_product_buffers(AcaDev &h);
 
 // Block matrix: parts with row / column node lists (1-based), each dense,
 // wavelet (ctype 2) or cross-approximated (ctype 3).

@@ -247,19 +247,66 @@ __global__ void wt_fill_kernel(const double *__restrict__ Tt, int m, int n, cons
   }
 }
 
-__global__ void __launch_bounds__(kNT) wxv_kernel(const double *__restrict__ x, HaarTab t, double *__restrict__ xw) {
-  const int ii = blockIdx.x * kNT + threadIdx.x + 1;
-  if (ii <= t.nv) xw[ii - 1] = haar_coef(t, ii, true, [&](int j) { return x[j - 1]; });
+
+
+// the same coefficients with one wavefront per coefficient: its range split
+// over the lanes, the lanes' sums added by a butterfly (lane 0's result)
+__global__ void __launch_bounds__(kNT) wxv_wave_kernel(const double *__restrict__ x, HaarTab t,
+                                                       double *__restrict__ xw) {
+  const int ii = blockIdx.x * (kNT / 64) + (int)(threadIdx.x / 64) + 1, lane = threadIdx.x & 63;
+  if (ii > t.nv) return;
+  auto get = [&](int j) { return x[j - 1]; };
+  double acc = 0.0;
+  if (ii == 1) {
+    for (int i = 1 + lane; i <= t.N; i += 64) acc += haar_pad(t, i, true, get) * t.norma;
+  } else {
+    const int k = dvanak(ii), l = ii - (1 << k), d = 1 << (t.kmax - k);
+    const int psp = (2 * l - 2) * d + 1, psr = (2 * l - 1) * d, pzg = 2 * l * d;
+    const double vc = t.valcek[k];
+    for (int i = psp + lane; i <= psr; i += 64) acc += vc * haar_pad(t, i, true, get);
+    for (int i = psr + 1 + lane; i <= pzg; i += 64) acc -= vc * haar_pad(t, i, true, get);
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) xw[ii - 1] = acc;
 }
 
-__global__ void __launch_bounds__(kNT) csr1_kernel(int m, const int *__restrict__ zvr, const int *__restrict__ c,
-                                                   const double *__restrict__ v, const double *__restrict__ x,
-                                                   double *__restrict__ y) {
-  const int r = blockIdx.x * kNT + threadIdx.x;
+// 1-based CSR product with one wavefront per row (lane-strided entries)
+__global__ void __launch_bounds__(kNT) csr_wave_kernel(int m, const int *__restrict__ zvr, const int *__restrict__ c,
+                                                       const double *__restrict__ v, const double *__restrict__ x,
+                                                       double *__restrict__ y) {
+  const int r = blockIdx.x * (kNT / 64) + (int)(threadIdx.x / 64), lane = threadIdx.x & 63;
   if (r >= m) return;
   double acc = 0.0;
-  for (int k = zvr[r]; k <= zvr[r + 1] - 1; ++k) acc = __dadd_rn(acc, __dmul_rn(v[k - 1], x[c[k - 1] - 1]));
-  y[r] = acc;
+  for (int k = zvr[r] - 1 + lane; k < zvr[r + 1] - 1; k += 64) acc += v[k] * x[c[k] - 1];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if (lane == 0) y[r] = acc;
+}
+
+// y = A x for a column-major A (lda) in column slices: part[ks * rows + i]
+__global__ void __launch_bounds__(64) cm_mv_part_kernel(const double *__restrict__ A, long lda, int rows, int cols,
+                                                        const double *__restrict__ x, int ks_cnt,
+                                                        double *__restrict__ part) {
+  const int i = blockIdx.x * 64 + threadIdx.x, ks = blockIdx.y;
+  if (i >= rows) return;
+  const int chunk = (cols + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(cols, j0 + chunk);
+  double acc = 0.0;
+#pragma unroll 8
+  for (int j = j0; j < j1; ++j) acc += A[i + (size_t)lda * j] * x[j];
+  part[(size_t)ks * rows + i] = acc;
+}
+
+__global__ void part_sum_kernel(int rows, int ks_cnt, const double *__restrict__ part, double *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  double s = 0.0;
+  for (int k = 0; k < ks_cnt; ++k) s += part[(size_t)k * rows + i];
+  y[i] = s;
+}
+
+int cm_split(int rows, int cols) {
+  if (rows < 1 || cols < 1) return 1;
+  const int rb = (rows + 63) / 64;
+  return std::max(1, std::min((4096 + rb - 1) / rb, std::max(1, cols / 32)));
 }
 
 __global__ void __launch_bounds__(kNT) wtxv_kernel(const double *__restrict__ yw, HaarTab t, double *__restrict__ y) {
@@ -400,24 +447,7 @@ __global__ void aca_bpack_kernel(const double *__restrict__ Bv, int rank, int n,
   b[t] = Bv[(size_t)k * n + j];
 }
 
-// aca_abxv: t = b x (one thread per rank row), y = a t (one per output row)
-__global__ void __launch_bounds__(kNT) aca_bx_kernel(const double *__restrict__ b, int rank, int n,
-                                                     const double *__restrict__ x, double *__restrict__ t) {
-  const int k = blockIdx.x * kNT + threadIdx.x;
-  if (k >= rank) return;
-  double acc = 0.0;
-  for (int j = 0; j < n; ++j) acc = __dadd_rn(acc, __dmul_rn(b[k + (size_t)rank * j], x[j]));
-  t[k] = acc;
-}
 
-__global__ void __launch_bounds__(kNT) aca_at_kernel(const double *__restrict__ a, int m, int rank,
-                                                     const double *__restrict__ t, double *__restrict__ y) {
-  const int i = blockIdx.x * kNT + threadIdx.x;
-  if (i >= m) return;
-  double acc = 0.0;
-  for (int k = 0; k < rank; ++k) acc = __dadd_rn(acc, __dmul_rn(a[i + (size_t)m * k], t[k]));
-  y[i] = acc;
-}
 
 // ---------------------------------------------------------------------------
 // block matrices: per-part cross approximation (one workgroup per part)
@@ -703,12 +733,18 @@ int wt_compress_dev(const double *A, long lda, int m, int n, double kappa, WtDev
 
 void wt_mv_dev(WtDev &w, const double *x, double *y, hipStream_t s) {
   const HaarTab tc = haar_tab(w.n), tr = haar_tab(w.m);
-  hipLaunchKernelGGL(wxv_kernel, dim3(ceil_div(w.n, kNT)), dim3(kNT), 0, s, x, tc, w.xw.p);
+  hipLaunchKernelGGL(wxv_wave_kernel, dim3(ceil_div(w.n, kNT / 64)), dim3(kNT), 0, s, x, tc, w.xw.p);
   TT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(csr1_kernel, dim3(ceil_div(w.m, kNT)), dim3(kNT), 0, s, w.m, w.zvr.p, w.c.p, w.v.p, w.xw.p, w.yw.p);
+  hipLaunchKernelGGL(csr_wave_kernel, dim3(ceil_div(w.m, kNT / 64)), dim3(kNT), 0, s, w.m, w.zvr.p, w.c.p, w.v.p,
+                     w.xw.p, w.yw.p);
   TT_LAUNCH_CHECK();
   hipLaunchKernelGGL(wtxv_kernel, dim3(ceil_div(w.m, kNT)), dim3(kNT), 0, s, w.yw.p, tr, y);
   TT_LAUNCH_CHECK();
+}
+
+void aca_product_buffers(AcaDev &h) {
+  h.t.alloc(std::max(h.rank, 1));
+  h.part.alloc(std::max({(size_t)cm_split(h.rank, h.n) * h.rank, (size_t)cm_split(h.m, h.rank) * h.m, (size_t)1}));
 }
 
 int aca_compress_dev(double *R, int m, int n, int type, double eps, const double *vec_host, AcaDev &out,
@@ -782,6 +818,7 @@ int aca_compress_dev(double *R, int m, int n, int type, double eps, const double
   out.a.alloc((size_t)m * std::max(rank, 1));
   out.b.alloc((size_t)n * std::max(rank, 1));
   out.t.alloc(std::max(rank, 1));
+  aca_product_buffers(out);
   if (rank > 0) {
     TT_HIP(hipMemcpyAsync(out.a.p, A.p, sizeof(double) * (size_t)m * rank, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(aca_bpack_kernel, dim3(ceil_div((size_t)rank * n, kNT)), dim3(kNT), 0, s, Bv.p, rank, n,
@@ -812,9 +849,19 @@ void aca_mv_dev(AcaDev &h, const double *x, double *y, hipStream_t s) {
     TT_HIP(hipMemsetAsync(y, 0, sizeof(double) * h.m, s));
     return;
   }
-  hipLaunchKernelGGL(aca_bx_kernel, dim3(ceil_div(h.rank, kNT)), dim3(kNT), 0, s, h.b.p, h.rank, h.n, x, h.t.p);
+  // t = b x (rank x n), then y = a t (m x rank): column-slice products
+  const int k1 = cm_split(h.rank, h.n), k2 = cm_split(h.m, h.rank);
+  const size_t need = std::max((size_t)k1 * h.rank, (size_t)k2 * h.m);
+  if (h.part.n < need) throw std::runtime_error("ACA product partials exceed their buffer");
+  hipLaunchKernelGGL(cm_mv_part_kernel, dim3(ceil_div(h.rank, 64), k1), dim3(64), 0, s, h.b.p, (long)h.rank, h.rank,
+                     h.n, x, k1, h.part.p);
   TT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(aca_at_kernel, dim3(ceil_div(h.m, kNT)), dim3(kNT), 0, s, h.a.p, h.m, h.rank, h.t.p, y);
+  hipLaunchKernelGGL(part_sum_kernel, dim3(ceil_div(h.rank, kNT)), dim3(kNT), 0, s, h.rank, k1, h.part.p, h.t.p);
+  TT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(cm_mv_part_kernel, dim3(ceil_div(h.m, 64), k2), dim3(64), 0, s, h.a.p, (long)h.m, h.m, h.rank,
+                     h.t.p, k2, h.part.p);
+  TT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(part_sum_kernel, dim3(ceil_div(h.m, kNT)), dim3(kNT), 0, s, h.m, k2, h.part.p, y);
   TT_LAUNCH_CHECK();
 }
 
@@ -1146,7 +1193,8 @@ extern "C" int tt_aca_abxv(int32_t m, int32_t n, int32_t rank, const double *a, 
     if (m < 1 || n < 1 || rank < 0) throw std::runtime_error("bad sizes");
     AcaDev d;
     d.m = m, d.n = n, d.rank = rank;
-    d.a.upload(a, (size_t)m * rank), d.b.upload(b, (size_t)rank * n), d.t.alloc(std::max(rank, 1));
+    d.a.upload(a, (size_t)m * rank), d.b.upload(b, (size_t)rank * n);
+    aca_product_buffers(d);
     DBuf<double> dx, dy(m);
     dx.upload(x, n);
     aca_mv_dev(d, dx.p, dy.p, 0);

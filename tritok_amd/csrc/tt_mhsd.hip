@@ -440,6 +440,7 @@ __global__ void __launch_bounds__(kRows) mhsd_mv_kernel(int nr, const int *__res
     const int lo = max(j0, base), hi = min(j1, base + mv.ncol[k]);
     const double *__restrict__ M = mv.M[k];
     const double *__restrict__ v = mv.v[k];
+#pragma unroll 8
     for (int j = lo; j < hi; ++j) acc += M[row + ld * (size_t)(j - base)] * v[j - base];
     base += mv.ncol[k];
   }
@@ -479,7 +480,7 @@ __global__ void mhsd_relax_kernel(int nn, int nf, const int *__restrict__ kode, 
 
 int split_cols(int nr, int ncols) {
   const int rows = (nr + kRows - 1) / kRows;
-  int ks = (4096 + rows - 1) / rows;
+  int ks = (8192 + rows - 1) / rows;
   return std::max(1, std::min(ks, std::max(1, ncols / 64)));
 }
 
@@ -924,7 +925,7 @@ extern "C" int tt_mhsd_system(tt_mhsd *h, const int32_t *kode, const int32_t *qk
     h->b.alloc(std::max(nknown, 1)), h->v.alloc((size_t)nn * 5 + (size_t)h->nb * 3);
     h->rhsb.alloc(nunk), h->r.alloc(nunk), h->x.alloc(nunk), h->y.alloc((size_t)8 * std::max(nn, nf));
     // column-slice partials of the products (split_cols keeps slices x rows below this)
-    h->part.alloc((size_t)kRows * (4096 + ceil_div(std::max({nunk, nn, nf}), kRows)));
+    h->part.alloc((size_t)kRows * (8192 + ceil_div(std::max({nunk, nn, nf}), kRows)));
     return 0;
   });
 }
