@@ -140,10 +140,26 @@ def serving_leg(requests, slots, max_iter, precs, seed=20251203):
                 if dt > 0:
                     time.sleep(dt)
                 ids.append(srv.submit(h, prec))
-            infos = [srv.wait(i) for i in ids]
+            infos = []
+            for k, i in enumerate(ids):
+                infos.append(srv.wait(i))
+                if (k + 1) % 20 == 0:
+                    log("serving %s: %d of %d finished after %.1f s" % (prec, k + 1, len(ids),
+                                                                         time.perf_counter() - t0))
             wall = time.perf_counter() - t0
         lat = np.array([i["t_end"] - i["t_submit"] for i in infos])
         run = np.array([i["t_end"] - i["t_start"] for i in infos])
+        setup = np.array([i["t_setup"] - i["t_start"] for i in infos])
+        per_n = {}
+        for sp, i in zip(stream, infos):
+            d = per_n.setdefault(sp["n"], {"count": 0, "setup_s": [], "picard_s": [], "its": []})
+            d["count"] += 1
+            d["setup_s"].append(i["t_setup"] - i["t_start"])
+            d["picard_s"].append(i["t_end"] - i["t_setup"])
+            d["its"].append(i["picard_iterations"])
+        per_n = {str(k): {"count": d["count"], "setup_s_mean": float(np.mean(d["setup_s"])),
+                          "picard_s_mean": float(np.mean(d["picard_s"])), "picard_its_mean": float(np.mean(d["its"]))}
+                 for k, d in sorted(per_n.items())}
         ok = [i["status"] == 0 for i in infos]
         log("serving %s: %d requests in %.1f s, latency p50 %.2f s p99 %.2f s, %d converged, %d capped, %d failed" % (
             prec, requests, wall, np.percentile(lat, 50), np.percentile(lat, 99), sum(ok),
@@ -152,6 +168,8 @@ def serving_leg(requests, slots, max_iter, precs, seed=20251203):
                                    "mean": float(lat.mean()), "max": float(lat.max())},
                      "service_s": {"p50": float(np.percentile(run, 50)), "p99": float(np.percentile(run, 99))},
                      "solves_per_s": requests / wall, "wall_s": wall, "converged": int(sum(ok)),
+                     "keeps_up_with_arrivals": bool(requests / wall >= 20.0),
+                     "setup_share_of_service": float(setup.sum() / run.sum()), "per_size": per_n,
                      "capped": int(sum(i["status"] == 1 for i in infos)),
                      "failed": int(sum(i["status"] < 0 for i in infos)),
                      "mean_picard_iterations": float(np.mean([i["picard_iterations"] for i in infos])),
@@ -476,9 +494,10 @@ def main():
                     help="cube size of the converged mixed-vs-FP64 comparison (0: skip)")
     ap.add_argument("--no-kinematics", action="store_true",
                     help="EQNS iBw = 0: skip the single-domain kinematics (least-squares systems only)")
-    ap.add_argument("--serve-requests", type=int, default=24,
+    ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the serving stream replayed in real time (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-only", action="store_true", help="only the serving leg (mixed precision), then exit")
     ap.add_argument("--serve-max-iter", type=int, default=400)
     ap.add_argument("--batch-cells", type=int, default=16,
                     help="n: batched modified-Helmholtz integrals of n^3 independent cells (0: skip)")
@@ -505,6 +524,13 @@ def main():
 
     if args.assembly_only:
         assembly_only(args.assembly_only)
+        return
+    if args.serve_only:
+        sv = serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, (args.mixed,))
+        for p in sv:
+            if isinstance(sv[p], dict) and "per_request" in sv[p]:
+                sv[p].pop("per_request")
+        log("serving only: " + json.dumps(sv))
         return
     if args.mhsd_only:
         log("mhsd only: " + json.dumps(mhsd_leg(args.mhsd)))
