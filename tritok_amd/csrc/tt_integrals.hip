@@ -215,7 +215,7 @@ template <bool Helm>
 __global__ void __launch_bounds__(64) macro_dom_kernel(MeshPtrs m, const QuadDev *__restrict__ qp,
                                                        const double *__restrict__ mun, const double *__restrict__ muq,
                                                        MacroOut o, unsigned long long *__restrict__ npts) {
-  __shared__ double lds[wave_cell_lds<4>()];
+  __shared__ __align__(16) double lds[wave_cell_lds<4>()];
   const int ic = blockIdx.x, it = blockIdx.y;
   const size_t nc = m.nc, m1 = nc * 51, i = (size_t)ic * 51 + it;
   double p[3], mu;
@@ -615,7 +615,7 @@ __global__ void __launch_bounds__(64) sd_dom_singular_kernel(MeshPtrs m, const Q
                                                              double *__restrict__ staging) {
   // one wavefront per (row, cell) pair: the pair's quadrature points are
   // evaluated 64 at a time (WaveCell, as for the macro domain integrals)
-  __shared__ double lds[wave_cell_lds<3>()];
+  __shared__ __align__(16) double lds[wave_cell_lds<3>()];
   const int k = blockIdx.x;
   if (k >= npairs) return;
   const size_t nc = m.nc;

@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(64) mhsd_dom_singular_kernel(MeshPtrs m, const
                                                                const int *__restrict__ pcell,
                                                                const int *__restrict__ psrc, int npairs,
                                                                double *__restrict__ staging) {
-  __shared__ double lds[wave_cell_lds<4>()];
+  __shared__ __align__(16) double lds[wave_cell_lds<4>()];
   const int k = blockIdx.x;
   if (k >= npairs) return;
   const size_t nc = m.nc;

@@ -292,28 +292,43 @@ __device__ void integrate_cell(const QuadDev &q, const double *p, const double (
 // ---------------------------------------------------------------------------
 // Wavefront-parallel domain integral of one (source, cell) pair.  The same
 // quadrature points as integrate_cell, taken 64 at a time: in the first phase
-// every lane evaluates one point (geometry, Jacobian, kernel) and stages its
-// three 1-D quadratic Lagrange factor triples and NC weighted kernel values in
-// LDS; in the second phase lane k of each half-wave (k < 27) accumulates shape
-// function k's NC sums over that half's 32 points.  Each lane holds NC
-// accumulators instead of 27 NC, so nothing spills.  The 27-node shape
-// functions are the products l_a(xi) l_b(eta) l_c(zeta) of the quadratic
-// Lagrange factors (a, b, c the node's position in {-1, 0, 1}, cshape27's node
-// order), equal to cshape27 to rounding.
+// every lane evaluates one point (geometry, Jacobian, kernel) and stages in
+// LDS the nine products l_a(xi) l_b(eta) of its 1-D quadratic Lagrange factors
+// and the 3 NC products of its NC weighted kernel values with l_c(zeta); in
+// the second phase lane k of each half-wave (k < 27) accumulates shape
+// function k's NC sums over that half's 32 points, one fused multiply-add per
+// sum and point.  Each lane holds NC accumulators instead of 27 NC, so nothing
+// spills.  The 27-node shape functions are the products l_a(xi) l_b(eta)
+// l_c(zeta) of the quadratic Lagrange factors (a, b, c the node's position in
+// {-1, 0, 1}, cshape27's node order), equal to cshape27 to rounding.
+// the three 1-D quadratic Lagrange factors at t (nodes -1, 0, 1)
+__device__ __forceinline__ void lagrange3(double t, double *l) {
+  l[0] = 0.5 * t * (t - 1.0);
+  l[1] = (1.0 - t) * (1.0 + t);
+  l[2] = 0.5 * t * (t + 1.0);
+}
+
 __constant__ static const signed char kNode27[27][3] = {
     {-1, 1, 1},  {-1, 1, -1}, {1, 1, -1},  {1, 1, 1},  {-1, -1, 1}, {-1, -1, -1}, {1, -1, -1},
     {1, -1, 1},  {-1, 1, 0},  {0, 1, -1},  {1, 1, 0},  {0, 1, 1},   {-1, 0, 1},   {-1, 0, -1},
     {1, 0, -1},  {1, 0, 1},   {-1, -1, 0}, {0, -1, -1}, {1, -1, 0}, {0, -1, 1},   {0, 1, 0},
     {-1, 0, 0},  {0, 0, -1},  {1, 0, 0},   {0, 0, 1},  {0, -1, 0},  {0, 0, 0}};
 
-// LDS doubles one wavefront needs for wave_cell<NC>
 // entries of the singular transform's table: (K, N, I) triples of one chunk
 // of K values (see WaveCell::run)
 constexpr int kTrigTab = 192;
+// at most kTrigTab / 3 K values per chunk (one (N, I) grid of at least 3 entries each)
+constexpr int kTrigK = kTrigTab / 3;
+// staging rows: point 32 h + j of a stage sits in column 34 h + j of a row of
+// 68 doubles, so that the rows one read instruction touches (nine products
+// and two half-waves, or three zeta factors and two half-waves) spread over
+// the 64 LDS banks; two consecutive points are one 16-byte read
+constexpr int kStRow = 68, kStHalf = 34;
 
+// LDS doubles one wavefront needs for WaveCell<NC> (16-byte aligned)
 template <int NC>
 constexpr int wave_cell_lds() {
-  return (9 + NC) * 64 + kTrigTab * 4 + 2 * kTrigTab;
+  return (9 + 3 * NC) * kStRow + kTrigTab * 4 + 2 * kTrigK;
 }
 
 template <int NC, bool Helm>
@@ -346,24 +361,33 @@ struct WaveCell {
         if (NC == 4) f[NC - 1] = fung * ajac;
       }
     }
-    double *L = lds, *F = lds + 9 * 64;
-    const double t[3] = {eti, etj, etk};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      L[(3 * d + 0) * 64 + lane] = 0.5 * t[d] * (t[d] - 1.0);
-      L[(3 * d + 1) * 64 + lane] = (1.0 - t[d]) * (1.0 + t[d]);
-      L[(3 * d + 2) * 64 + lane] = 0.5 * t[d] * (t[d] + 1.0);
-    }
-#pragma unroll
-    for (int q = 0; q < NC; ++q) F[q * 64 + lane] = f[q];
-    __syncthreads();
+    double lx[3], ly[3], lz[3];
+    lagrange3(eti, lx);
+    lagrange3(etj, ly);
+    lagrange3(etk, lz);
     const int half = lane >> 5;
-#pragma unroll 4
-    for (int j = 0; j < 32; ++j) {
-      const int pi = half * 32 + j;
-      const double phi = L[ax * 64 + pi] * L[(3 + ay) * 64 + pi] * L[(6 + az) * 64 + pi];
+    double *P = lds + half * kStHalf, *G = P + 9 * kStRow;
+    const int col = lane & 31;
 #pragma unroll
-      for (int q = 0; q < NC; ++q) acc[q] += F[q * 64 + pi] * phi;
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) P[(3 * a + b) * kStRow + col] = lx[a] * ly[b];
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) G[(3 * q + c) * kStRow + col] = f[q] * lz[c];
+    __syncthreads();
+    const double2 *Pr = reinterpret_cast<const double2 *>(P + (3 * ax + ay) * kStRow);
+    const double2 *Gr = reinterpret_cast<const double2 *>(G + az * kStRow);
+#pragma unroll 4
+    for (int j = 0; j < 16; ++j) {
+      const double2 pv = Pr[j];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        const double2 gv = Gr[q * (3 * kStRow / 2) + j];
+        acc[q] += pv.x * gv.x;
+        acc[q] += pv.y * gv.y;
+      }
     }
     __syncthreads();
   }
@@ -428,7 +452,7 @@ struct WaveCell {
               // stages read them back (the same expressions, so the same bits
               // as evaluating them in every lane).
               const int nds = q.nds, kb = kTrigTab / (3 * nds) > 0 ? kTrigTab / (3 * nds) : 1;
-              double *tab = lds + (9 + NC) * 64, *ktab = tab + 4 * kTrigTab;
+              double *tab = lds + (9 + 3 * NC) * kStRow, *ktab = tab + 4 * kTrigTab;
               for (int K0 = 0; K0 < nds; K0 += kb) {
                 const int K1 = K0 + kb < nds ? K0 + kb : nds;
                 const int ne = (K1 - K0) * 3 * nds;
@@ -462,12 +486,12 @@ struct WaveCell {
                     tab[kTrigTab + te] = sin(th);
                     tab[2 * kTrigTab + te] = cos(th);
                     tab[3 * kTrigTab + te] = th1 - th0;
-                    if (N == 1 && I == 0) ktab[K - K0] = sin(ffi), ktab[kTrigTab + K - K0] = cos(ffi);
+                    if (N == 1 && I == 0) ktab[K - K0] = sin(ffi), ktab[kTrigK + K - K0] = cos(ffi);
                   }
                 }
                 __syncthreads();
                 for (int K = K0; K < K1; ++K) {
-                  const double sf = ktab[K - K0], cf = ktab[kTrigTab + K - K0];
+                  const double sf = ktab[K - K0], cf = ktab[kTrigK + K - K0];
                   for (int N = 1; N <= 3; ++N) {
                     const int np = nds * nds;
                     for (int base = 0; base < np; base += 64) {
