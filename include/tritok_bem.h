@@ -474,6 +474,19 @@ int tt_case_system_tuning(tt_case *c, int32_t sys, int32_t grid[3], int32_t thre
  * solve that stops with istop > 3 keeps its updated velocities and the step
  * goes on, as lib_km.f:274 does. */
 int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]);
+/* One Picard iteration of each of ncase cases (1..16) at once: every LSQR
+ * iteration of a phase runs the cell kernels of all the cases' systems in one
+ * launch and their unknown kernels in another (no reference counterpart: the
+ * reference runs one case per process; for a serving loop that batches the
+ * Krylov iterations of its active cases).  Each case's result is bit for bit
+ * that of tt_case_picard_step on it alone.  err (ncase x 9), nit (ncase x 12)
+ * and status (ncase) receive per-case results as tt_case_picard_step's:
+ * status 1 = continue, 0 = converged / at nnlit / already finished,
+ * -1 = failed (tt_last_error names the first failed case); a failed case does
+ * not stop the others.  The cases must iterate with the same block precision;
+ * the batch runs on the first unfinished case's stream, after the others'
+ * streams are drained.  Returns 0, or -1 if the batch could not run. */
+int tt_case_picard_step_batch(tt_case **cases, int32_t ncase, double *err, int32_t *nit, int32_t *status);
 /* Runs Picard iterations until convergence; returns the iteration count. */
 int tt_case_run(tt_case *c, int32_t max_iter, double *sta, int32_t *ite);
 /* Copies the current fields to host arrays (any may be NULL). */

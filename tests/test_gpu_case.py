@@ -159,6 +159,55 @@ def test_graph_replay_matches_stream_launches(prec):
             assert np.array_equal(o[2][k], outs[0][2][k]), k
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_batched_steps_match_cases_alone(prec):
+    """tt_case_picard_step_batch: cases of three sizes, both problem types
+    and different parameters stepped together give, step for step, the bits
+    each gives alone (relative changes, LSQR counts, fields), while one case
+    finishes (its iteration cap, RELA nnlit 4) and leaves the batch and the
+    others go on."""
+    from tritok_amd import workloads
+    from tritok_amd.case import Case, step_batch
+    hosts = [workloads.natural_convection(4, ra=1e4), workloads.cavity(4, re=50.0),
+             workloads.natural_convection(6, ra=3e3), workloads.cavity(3, re=10.0, nnlit=4)]
+    nstep = 7
+    alone = []
+    for i, h in enumerate(hosts):
+        print(prec, "alone", i, flush=True)
+        with Case(h, prec=prec) as c:
+            steps = []
+            for _ in range(nstep):
+                rc, err, nit = c.step()
+                steps.append((rc, err.copy(), nit.copy()))
+                if rc == 0:
+                    break
+            alone.append((steps, c.fields()))
+    assert len(alone[3][0]) == 4, "the case capped at 4 iterations should finish inside the batch"
+    print(prec, "batch", flush=True)
+    cases = [Case(h, prec=prec) for h in hosts]
+    try:
+        got = [[] for _ in hosts]
+        active = list(range(len(hosts)))
+        for _ in range(nstep):
+            st, err, nit = step_batch([cases[i] for i in active])
+            for q, i in enumerate(active):
+                assert st[q] >= 0, (i, st[q])
+                got[i].append((int(st[q]), err[q].copy(), nit[q].copy()))
+            active = [i for q, i in enumerate(active) if st[q] == 1]
+            if not active:
+                break
+        for i, (steps, f) in enumerate(alone):
+            assert len(got[i]) == len(steps), i
+            for (ra, ea, na), (rb, eb, nb) in zip(steps, got[i]):
+                assert ra == rb and np.array_equal(ea, eb) and np.array_equal(na, nb), i
+            fb = cases[i].fields()
+            for k in FIELDS:
+                assert np.array_equal(f[k], fb[k]), (i, k)
+    finally:
+        for c in cases:
+            c.close()
+
+
 def test_case_is_bitwise_repeatable(gauss_table):
     g = load_golden("nc2")
     outs = []

@@ -16,6 +16,7 @@ from ._lib import CaseView, P, check, i32, i64
 _create = _lib._sig("tt_case_create", i32, C.POINTER(CaseView), _lib.PD, _lib.PD, _lib.PD, _lib.PD, _lib.PD, i32,
                     C.c_double, C.POINTER(P))
 _step = _lib._sig("tt_case_picard_step", i32, P, _lib.PD, _lib.PI)
+_step_batch = _lib._sig("tt_case_picard_step_batch", i32, C.POINTER(P), i32, _lib.PD, _lib.PI, _lib.PI)
 _run = _lib._sig("tt_case_run", i32, P, i32, _lib.PD, _lib.PI)
 _fields = _lib._sig("tt_case_fields", i32, P, _lib.PD, _lib.PD, _lib.PD, _lib.PD, _lib.PD)
 _timings = _lib._sig("tt_case_timings", i32, P, _lib.PD)
@@ -161,3 +162,18 @@ class Case:
             self.close()
         except Exception:
             pass
+
+
+def step_batch(cases):
+    """One Picard iteration of each Case in `cases` (1..16, same block
+    precision), their LSQR iterations sharing launches
+    (tt_case_picard_step_batch).  Returns (status, err, nit): status[i] 1 =
+    continue, 0 = finished, -1 = failed; err (k, 9); nit (k, 12)."""
+    k = len(cases)
+    hs = (P * k)(*[c._h for c in cases])
+    err = np.zeros((k, 9))
+    nit = np.zeros((k, 12), np.int32)
+    st = np.zeros(k, np.int32)
+    check(_step_batch(hs, k, err.ctypes.data_as(_lib.PD), nit.ctypes.data_as(_lib.PI), st.ctypes.data_as(_lib.PI)),
+          "tt_case_picard_step_batch")
+    return st, err, nit

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "tritok_bem.h"
@@ -373,26 +374,6 @@ struct EvTimer {
   }
 };
 
-// slot: the field component a system shared by components is solved for.
-// fatal: istop > 3 (condition limit or iteration limit) stops the case, as
-// WarnErr level 4 does after the vorticity and energy solves
-// (lib_solvFDwTE.f:297, lib_solvFDdae.f:265); the kinematics solves only
-// report it and keep the updated x (lib_km.f:274).
-int solve_subsys(tt_case &c, SubSys &S, const double *fnode, const double *fflux, float eps4, int slot,
-                 bool fatal) {
-  const int n = S.n;
-  hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, c.s, n, c.cv.mesh.nnodes, S.unk_field.p,
-                     fnode, fflux, c.x.p);
-  TT_LAUNCH_CHECK();
-  LsqrResult r = block_lsqr(S.bs, c.work, c.b.p, c.x.p, c.cv.lsqs_maxit, eps4, c.lsqr_prec, c.s,
-                            S.R == 51 ? &c.probe51 : &c.probe27, c.graphs, slot);
-  if (r.istop > 3) {
-    if (fatal) throw std::runtime_error("LSQR stopped with istop > 3 (ill-conditioned or iteration limit)");
-    ++c.km_solver_warnings;
-  }
-  return r.nit;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -542,84 +523,202 @@ extern "C" int tt_case_create(const tt_case_view *cv, const double *vel, const d
 }
 
 
-extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
-  return guarded("tt_case_picard_step", [&]() -> int {
-    if (!c) throw std::runtime_error("NULL case");
-    if (c->finished) return 0;
-    const auto &cv = c->cv;
-    const int nn = cv.mesh.nnodes, nb = cv.mesh.nbnodes, nc = cv.mesh.nicell, nq = cv.mesh.nq;
-    hipStream_t s = c->s;
-    const MacroP mp = macro_ptrs(c->macro);
-    int its[12] = {0};
-    double tp[7] = {0};
-    c->it++;
-    EvTimer ttot(s);
-    // ---- boundary vorticities
-    {
-      EvTimer t(s);
-      if (cv.iBw == 1) dense_bw_step(c->dense, c->mesh, c->vel.p, c->vort.p, cv.urBw, s);
-      tp[0] = t.stop();
+namespace {
+// outcome of one case's Picard iteration
+struct StepOut {
+  double err[9] = {0};
+  int32_t nit[12] = {0};
+  int status = 1;  // 1: another iteration follows, 0: finished, -1: failed (error)
+  std::string error;
+};
+
+// One Picard iteration (main.f90:1200-1299) of every case in cs, all on the
+// stream of cs[0].  The phases run case after case; each linear solve of a
+// phase is one block_lsqr per case when cs holds one case (captured chunks
+// and sampled timings as configured), and one block_lsqr_multi over the
+// cases otherwise, so that a batch of cases shares every LSQR launch.  The
+// kernels and their arithmetic are the same either way: a case's iterates do
+// not depend on the batch it runs in.  A case whose vorticity or energy solve
+// stops with istop > 3 (condition or iteration limit) is failed and leaves the
+// remaining phases, as WarnErr level 4 stops the reference
+// (lib_solvFDwTE.f:297, lib_solvFDdae.f:265); the kinematics solves only
+// count it and keep the updated x (lib_km.f:274).  Each solve runs with the
+// solve-history slot of its field component (component e for the kinematics
+// system shared by the three components).
+void picard_multi(const std::vector<tt_case *> &cs, std::vector<StepOut> &out) {
+  const size_t K = cs.size();
+  hipStream_t s = cs[0]->s;
+  out.assign(K, StepOut());
+  std::vector<char> ok(K, 1);
+  for (tt_case *c : cs) c->it++;
+  double tp[7] = {0};
+  EvTimer ttot(s);
+  // the same subsystem of every live case that has it: hot start, solve, result
+  auto solve = [&](auto sys, auto fnode, auto fflux, auto eps, int slot, bool fatal, int its_k, auto has) {
+    std::vector<size_t> who;
+    for (size_t i = 0; i < K; ++i)
+      if (ok[i] && has(*cs[i])) who.push_back(i);
+    if (who.empty()) return;
+    for (size_t i : who) {
+      tt_case &c = *cs[i];
+      SubSys &S = sys(c);
+      hipLaunchKernelGGL(gather_kernel, dim3(ceil_div(S.n, 256)), dim3(256), 0, s, S.n, c.cv.mesh.nnodes,
+                         S.unk_field.p, fnode(c), fflux(c), c.x.p);
+      TT_LAUNCH_CHECK();
     }
-    // ---- domain velocities
-    {
-      EvTimer t(s);
-      for (int e = 0; e < 3; ++e) {
-        hipLaunchKernelGGL(km_rhs_kernel, dim3(ceil_div((size_t)nc, 2)), dim3(64), 0, s, nc, nn, e, mp,
-                           c->mesh.idc.p, c->Af_km.p, c->km.colknown.p, c->km.colfield.p, c->vel.p, c->vort.p, c->b.p);
-        TT_LAUNCH_CHECK();
-        its[3 + e] = solve_subsys(*c, c->km, c->vel.p + (size_t)nn * e, nullptr, c->dlse[e], e, false);
-        hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c->km.n, 256)), dim3(256), 0, s, c->km.n, nn, cv.urDv[e],
-                           c->km.unk_field.p, c->x.p, c->vel.p + (size_t)nn * e, nullptr);
-        if (c->kobc.p)
-          hipLaunchKernelGGL(dvdn_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, e, c->kode.p,
-                             c->mesh.gbn.p, c->kobc.p, c->vel.p);
+    std::vector<LsqrResult> res(who.size());
+    if (K == 1) {
+      tt_case &c = *cs[0];
+      SubSys &S = sys(c);
+      res[0] = block_lsqr(S.bs, c.work, c.b.p, c.x.p, c.cv.lsqs_maxit, eps(c), c.lsqr_prec, s,
+                          S.R == 51 ? &c.probe51 : &c.probe27, c.graphs, slot);
+    } else {
+      std::vector<LsqrJob> jobs(who.size());
+      for (size_t q = 0; q < who.size(); ++q) {
+        tt_case &c = *cs[who[q]];
+        LsqrJob &j = jobs[q];
+        j.s = &sys(c).bs, j.wk = &c.work, j.b = c.b.p, j.x = c.x.p;
+        j.maxit = c.cv.lsqs_maxit, j.eps4 = eps(c), j.slot = slot;
+      }
+      block_lsqr_multi(jobs, cs[0]->lsqr_prec, s);
+      for (size_t q = 0; q < who.size(); ++q) res[q] = jobs[q].res;
+    }
+    for (size_t q = 0; q < who.size(); ++q) {
+      const size_t i = who[q];
+      out[i].nit[its_k] = res[q].nit;
+      if (res[q].istop > 3) {
+        if (fatal) {
+          ok[i] = 0;
+          out[i].status = -1;
+          out[i].error = "LSQR stopped with istop > 3 (ill-conditioned or iteration limit)";
+        } else {
+          ++cs[i]->km_solver_warnings;
+        }
+      }
+    }
+  };
+  auto all = [](tt_case &) { return true; };
+  // ---- boundary vorticities
+  {
+    EvTimer t(s);
+    for (size_t i = 0; i < K; ++i) {
+      tt_case &c = *cs[i];
+      if (c.cv.iBw == 1) dense_bw_step(c.dense, c.mesh, c.vel.p, c.vort.p, c.cv.urBw, s);
+    }
+    tp[0] = t.stop();
+  }
+  // ---- domain velocities
+  {
+    EvTimer t(s);
+    for (int e = 0; e < 3; ++e) {
+      for (size_t i = 0; i < K; ++i) {
+        tt_case &c = *cs[i];
+        const int nc = c.cv.mesh.nicell, nn = c.cv.mesh.nnodes;
+        hipLaunchKernelGGL(km_rhs_kernel, dim3(ceil_div((size_t)nc, 2)), dim3(64), 0, s, nc, nn, e,
+                           macro_ptrs(c.macro), c.mesh.idc.p, c.Af_km.p, c.km.colknown.p, c.km.colfield.p, c.vel.p,
+                           c.vort.p, c.b.p);
         TT_LAUNCH_CHECK();
       }
-      tp[1] = t.stop();
-    }
-    // ---- temperature
-    if (cv.iDT == 1) {
-      EvTimer t(s);
-      double enazalfa = c->ren * cv.PrN / 1.0;
-      hipLaunchKernelGGL(t_rhs_kernel, dim3(nc), dim3(64), 0, s, nc, nn, mp, c->mesh.idc.p,
-                         c->Af_t.p, c->tsys.colknown.p, c->tsys.colfield.p, c->vel.p, c->temp.p, c->qtemp.p,
-                         c->ttime.p, enazalfa, c->b.p);
-      TT_LAUNCH_CHECK();
-      its[9] = solve_subsys(*c, c->tsys, c->temp.p, c->qtemp.p, c->dlse[6], 0, true);
-      hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c->tsys.n, 256)), dim3(256), 0, s, c->tsys.n, nn, cv.urDT,
-                         c->tsys.unk_field.p, c->x.p, c->temp.p, c->qtemp.p);
-      TT_LAUNCH_CHECK();
-      tp[2] = t.stop();
-    }
-    // ---- domain vorticities
-    {
-      EvTimer t(s);
-      double fac = cv.RaN / cv.PrN / c->ren * 1.0 / 1.0;
-      for (int e = 0; e < 3; ++e) {
-        hipLaunchKernelGGL(vte_rhs_kernel, dim3(nc), dim3(64), 0, s, nc, nn, e, mp,
-                           c->mesh.idc.p, c->Af_vte.p, c->vte[e].colknown.p, c->vte[e].colfield.p, c->vel.p,
-                           c->vort.p, c->qvort.p + (size_t)nq * e, c->rhsv.p + (size_t)nn * e, c->temp.p, c->ren,
-                           cv.iDT > 0 ? 1 : 0, fac, cv.gx, cv.gy, cv.gz, c->b.p);
-        TT_LAUNCH_CHECK();
-        its[6 + e] = solve_subsys(*c, c->vte[e], c->vort.p + (size_t)nn * e, c->qvort.p + (size_t)nq * e, c->dlse[3 + e],
-                                  0, true);
-        hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c->vte[e].n, 256)), dim3(256), 0, s, c->vte[e].n, nn,
-                           cv.urDw[e], c->vte[e].unk_field.p, c->x.p, c->vort.p + (size_t)nn * e,
-                           c->qvort.p + (size_t)nq * e);
+      solve([](tt_case &c) -> SubSys & { return c.km; },
+            [e](tt_case &c) { return (const double *)(c.vel.p + (size_t)c.cv.mesh.nnodes * e); },
+            [](tt_case &) { return (const double *)nullptr; }, [e](tt_case &c) { return c.dlse[e]; }, e, false,
+            3 + e, all);
+      for (size_t i = 0; i < K; ++i) {
+        tt_case &c = *cs[i];
+        const int nn = c.cv.mesh.nnodes, nb = c.cv.mesh.nbnodes;
+        hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c.km.n, 256)), dim3(256), 0, s, c.km.n, nn, c.cv.urDv[e],
+                           c.km.unk_field.p, c.x.p, c.vel.p + (size_t)nn * e, nullptr);
+        if (c.kobc.p)
+          hipLaunchKernelGGL(dvdn_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, e, c.kode.p,
+                             c.mesh.gbn.p, c.kobc.p, c.vel.p);
         TT_LAUNCH_CHECK();
       }
-      tp[3] = t.stop();
     }
-    // ---- relative changes (CheckConvergencevw)
-    EvTimer tconv(s);
-    unsigned nbk = ceil_div(nn, 256);
-    hipLaunchKernelGGL(conv_kernel, dim3(nbk), dim3(256), 0, s, nn, c->vel.p, c->vort.p, c->temp.p, c->pv.p, c->pw.p,
-                       c->pt.p, c->cpart.p);
+    tp[1] = t.stop();
+  }
+  // ---- temperature
+  auto has_t = [](tt_case &c) { return c.cv.iDT == 1; };
+  bool any_t = false;
+  for (size_t i = 0; i < K; ++i) any_t |= ok[i] && cs[i]->cv.iDT == 1;
+  if (any_t) {
+    EvTimer t(s);
+    for (size_t i = 0; i < K; ++i) {
+      tt_case &c = *cs[i];
+      if (!ok[i] || c.cv.iDT != 1) continue;
+      const int nc = c.cv.mesh.nicell, nn = c.cv.mesh.nnodes;
+      const double enazalfa = c.ren * c.cv.PrN / 1.0;
+      hipLaunchKernelGGL(t_rhs_kernel, dim3(nc), dim3(64), 0, s, nc, nn, macro_ptrs(c.macro), c.mesh.idc.p,
+                         c.Af_t.p, c.tsys.colknown.p, c.tsys.colfield.p, c.vel.p, c.temp.p, c.qtemp.p, c.ttime.p,
+                         enazalfa, c.b.p);
+      TT_LAUNCH_CHECK();
+    }
+    solve([](tt_case &c) -> SubSys & { return c.tsys; }, [](tt_case &c) { return (const double *)c.temp.p; },
+          [](tt_case &c) { return (const double *)c.qtemp.p; }, [](tt_case &c) { return c.dlse[6]; }, 0, true, 9,
+          has_t);
+    for (size_t i = 0; i < K; ++i) {
+      tt_case &c = *cs[i];
+      if (!ok[i] || c.cv.iDT != 1) continue;
+      hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c.tsys.n, 256)), dim3(256), 0, s, c.tsys.n, c.cv.mesh.nnodes,
+                         c.cv.urDT, c.tsys.unk_field.p, c.x.p, c.temp.p, c.qtemp.p);
+      TT_LAUNCH_CHECK();
+    }
+    tp[2] = t.stop();
+  }
+  // ---- domain vorticities
+  {
+    EvTimer t(s);
+    for (int e = 0; e < 3; ++e) {
+      for (size_t i = 0; i < K; ++i) {
+        tt_case &c = *cs[i];
+        if (!ok[i]) continue;
+        const auto &cv = c.cv;
+        const int nc = cv.mesh.nicell, nn = cv.mesh.nnodes, nq = cv.mesh.nq;
+        const double fac = cv.RaN / cv.PrN / c.ren * 1.0 / 1.0;
+        hipLaunchKernelGGL(vte_rhs_kernel, dim3(nc), dim3(64), 0, s, nc, nn, e, macro_ptrs(c.macro), c.mesh.idc.p,
+                           c.Af_vte.p, c.vte[e].colknown.p, c.vte[e].colfield.p, c.vel.p, c.vort.p,
+                           c.qvort.p + (size_t)nq * e, c.rhsv.p + (size_t)nn * e, c.temp.p, c.ren,
+                           cv.iDT > 0 ? 1 : 0, fac, cv.gx, cv.gy, cv.gz, c.b.p);
+        TT_LAUNCH_CHECK();
+      }
+      solve([e](tt_case &c) -> SubSys & { return c.vte[e]; },
+            [e](tt_case &c) { return (const double *)(c.vort.p + (size_t)c.cv.mesh.nnodes * e); },
+            [e](tt_case &c) { return (const double *)(c.qvort.p + (size_t)c.cv.mesh.nq * e); },
+            [e](tt_case &c) { return c.dlse[3 + e]; }, 0, true, 6 + e, all);
+      for (size_t i = 0; i < K; ++i) {
+        tt_case &c = *cs[i];
+        if (!ok[i]) continue;
+        const int nn = c.cv.mesh.nnodes, nq = c.cv.mesh.nq;
+        hipLaunchKernelGGL(scatter_kernel, dim3(ceil_div(c.vte[e].n, 256)), dim3(256), 0, s, c.vte[e].n, nn,
+                           c.cv.urDw[e], c.vte[e].unk_field.p, c.x.p, c.vort.p + (size_t)nn * e,
+                           c.qvort.p + (size_t)nq * e);
+        TT_LAUNCH_CHECK();
+      }
+    }
+    tp[3] = t.stop();
+  }
+  // ---- relative changes (CheckConvergencevw)
+  EvTimer tconv(s);
+  std::vector<std::vector<double>> parts(K);
+  for (size_t i = 0; i < K; ++i) {
+    tt_case &c = *cs[i];
+    if (!ok[i]) continue;
+    const int nn = c.cv.mesh.nnodes;
+    const unsigned nbk = ceil_div(nn, 256);
+    hipLaunchKernelGGL(conv_kernel, dim3(nbk), dim3(256), 0, s, nn, c.vel.p, c.vort.p, c.temp.p, c.pv.p, c.pw.p,
+                       c.pt.p, c.cpart.p);
     TT_LAUNCH_CHECK();
-    std::vector<double> part((size_t)nbk * 14);
-    c->cpart.download(part.data(), part.size(), s);
-    tp[4] = tconv.stop();
-    tp[6] = ttot.stop();
+    parts[i].resize((size_t)nbk * 14);
+    TT_HIP(hipMemcpyAsync(parts[i].data(), c.cpart.p, sizeof(double) * parts[i].size(), hipMemcpyDeviceToHost, s));
+  }
+  TT_HIP(hipStreamSynchronize(s));
+  tp[4] = tconv.stop();
+  tp[6] = ttot.stop();
+  for (size_t i = 0; i < K; ++i) {
+    tt_case &c = *cs[i];
+    if (!ok[i]) continue;
+    const auto &cv = c.cv;
+    const unsigned nbk = ceil_div(cv.mesh.nnodes, 256);
+    const std::vector<double> &part = parts[i];
     double rnd[9] = {0}, rno[9] = {0};
     for (unsigned k = 0; k < nbk; ++k) {
       for (int j = 0; j < 7; ++j) rnd[j] += part[(size_t)k * 14 + j], rno[j] += part[(size_t)k * 14 + 7 + j];
@@ -639,35 +738,87 @@ extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
       if (e9[j] > 9.999999999) e9[j] = 9.999999999;
     }
     // ---- DLSE
-    if (cv.iDLSE == 1 && c->it > 1) {
-      if (c->it == 2)
-        for (int k = 0; k < 9; ++k) c->dlse[k] = cv.dlse_epsth;
+    if (cv.iDLSE == 1 && c.it > 1) {
+      if (c.it == 2)
+        for (int k = 0; k < 9; ++k) c.dlse[k] = cv.dlse_epsth;
       for (int k = 0; k < 8; ++k) {
-        float old = c->dlse[k];
+        float old = c.dlse[k];
         if (e9[k] / (double)cv.dlse_r < (double)cv.dlse_epsth && e9[k] != 0.0)
-          c->dlse[k] = (float)(e9[k] / (double)cv.dlse_r);
+          c.dlse[k] = (float)(e9[k] / (double)cv.dlse_r);
         else
-          c->dlse[k] = cv.dlse_epsth;
-        if (old < c->dlse[k]) c->dlse[k] = old;
-        if (c->dlse[k] < cv.lsqs_eps) c->dlse[k] = cv.lsqs_eps;
+          c.dlse[k] = cv.dlse_epsth;
+        if (old < c.dlse[k]) c.dlse[k] = old;
+        if (c.dlse[k] < cv.lsqs_eps) c.dlse[k] = cv.lsqs_eps;
       }
     }
-    if (err) std::memcpy(err, e9, sizeof(e9));
-    if (nit) std::memcpy(nit, its, sizeof(its));
-    for (int k = 0; k < 7; ++k) c->t_phase[k] = tp[k];
-    bool more = false;
+    std::memcpy(out[i].err, e9, sizeof(e9));
+    for (int k = 0; k < 7; ++k) c.t_phase[k] = tp[k];
+    bool more = false, finite = true;
     for (int j = 0; j < 9; ++j) {
-      if (!std::isfinite(e9[j])) {
-        c->finished = true;  // the reference would stop here as if converged (NaN > eps is false)
-        throw std::runtime_error("Picard iteration diverged (non-finite relative change)");
-      }
+      finite &= std::isfinite(e9[j]);
       more |= e9[j] > cv.eps;
     }
-    if (!(c->it < cv.nnlit && more)) {
-      c->finished = true;
-      return 0;
+    if (!finite) {
+      c.finished = true;  // the reference would stop here as if converged (NaN > eps is false)
+      out[i].status = -1;
+      out[i].error = "Picard iteration diverged (non-finite relative change)";
+      continue;
     }
-    return 1;
+    if (!(c.it < cv.nnlit && more)) {
+      c.finished = true;
+      out[i].status = 0;
+    }
+  }
+}
+}  // namespace
+
+extern "C" int tt_case_picard_step(tt_case *c, double err[9], int32_t nit[12]) {
+  return guarded("tt_case_picard_step", [&]() -> int {
+    if (!c) throw std::runtime_error("NULL case");
+    if (c->finished) return 0;
+    std::vector<StepOut> out;
+    picard_multi({c}, out);
+    if (err) std::memcpy(err, out[0].err, sizeof(out[0].err));
+    if (nit) std::memcpy(nit, out[0].nit, sizeof(out[0].nit));
+    if (out[0].status < 0) throw std::runtime_error(out[0].error);
+    return out[0].status;
+  });
+}
+
+extern "C" int tt_case_picard_step_batch(tt_case **cases, int32_t ncase, double *err, int32_t *nit,
+                                         int32_t *status) {
+  return guarded("tt_case_picard_step_batch", [&]() -> int {
+    if (!cases || ncase < 1 || ncase > 16) throw std::runtime_error("1..16 cases per batch");
+    std::vector<tt_case *> live;
+    std::vector<int> pos;
+    for (int i = 0; i < ncase; ++i) {
+      tt_case *c = cases[i];
+      if (!c) throw std::runtime_error("NULL case");
+      for (int j = 0; j < i; ++j)
+        if (cases[j] == c) throw std::runtime_error("a case appears twice in the batch");
+      if (status) status[i] = 0;
+      if (err) std::memset(err + (size_t)9 * i, 0, sizeof(double) * 9);
+      if (nit) std::memset(nit + (size_t)12 * i, 0, sizeof(int32_t) * 12);
+      if (c->finished) continue;
+      if (!live.empty() && c->lsqr_prec != live[0]->lsqr_prec)
+        throw std::runtime_error("the cases of a batch must iterate with the same block precision");
+      live.push_back(c), pos.push_back(i);
+    }
+    if (live.empty()) return 0;
+    // the batch runs on the first live case's stream: earlier work of the
+    // others on their own streams completes first
+    for (size_t i = 1; i < live.size(); ++i) TT_HIP(hipStreamSynchronize(live[i]->s));
+    std::vector<StepOut> out;
+    picard_multi(live, out);
+    int failed = 0;
+    for (size_t q = 0; q < live.size(); ++q) {
+      const int i = pos[q];
+      if (err) std::memcpy(err + (size_t)9 * i, out[q].err, sizeof(out[q].err));
+      if (nit) std::memcpy(nit + (size_t)12 * i, out[q].nit, sizeof(out[q].nit));
+      if (status) status[i] = out[q].status;
+      if (out[q].status < 0 && !failed++) tt_set_error("tt_case_picard_step_batch: case %d: %s", i, out[q].error.c_str());
+    }
+    return 0;
   });
 }
 

@@ -138,6 +138,27 @@ struct LsqrResult {
 LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int maxit, float eps4, int prec,
                       hipStream_t st, LsqrProbe *probe = nullptr, bool graphs = true, int slot = 0);
 
+// One system of a batched solve (block_lsqr_multi): as block_lsqr's
+// arguments; res is filled in.
+struct LsqrJob {
+  BlockSys *s = nullptr;
+  LsqrWork *wk = nullptr;  // a work area of its own
+  const double *b = nullptr;
+  double *x = nullptr;
+  int maxit = 0;
+  float eps4 = 0.f;
+  int slot = 0;
+  LsqrResult res;
+};
+
+// block_lsqr on up to 16 systems of one block shape at once, all on stream
+// st: every iteration runs the cell kernels of the systems still iterating in
+// one launch and their unknown kernels in one launch per gather width.  The
+// kernel bodies, grids per system and reductions are those of block_lsqr, so
+// each system's iterates, stop and result are bit for bit those of a solve on
+// its own.
+void block_lsqr_multi(std::vector<LsqrJob> &jobs, int prec, hipStream_t st);
+
 // Picks the cell kernel's grid for block precision prec by timing candidate
 // grids on a scratch solve (once per system shape and precision in a process).
 void block_tune(BlockSys &s, int prec, hipStream_t st);

@@ -228,13 +228,53 @@ __global__ void __launch_bounds__(64) lsqr_r0(const double *__restrict__ A, cons
 // unknown kernel's partials in the same order; when block 0 finds that the
 // iteration stopped, the cell blocks' work of this launch is simply unused
 // (the unknown kernel that follows sees the stop and leaves x unchanged).
+// The arguments of one system's cell kernel and unknown kernel.  A kernel
+// body takes them with its workgroup index and workgroup count, so that one
+// launch can run the iterations of several systems (workgroup ranges per
+// system, see Jobs) with the arithmetic of one launch per system.
+struct CellArgs {
+  int ncell, cgroup, nb;
+  const void *A;  // blocks in the kernel's element type
+  const int *colmap;
+  const double *vp, *pb1, *pb2;
+  double *u, *P, *pa1;
+  const LsqrState *sin;
+  LsqrState *sout;
+};
+struct UnkArgs {
+  int n, na;
+  const int *gtab;
+  const void *P;  // partials in the kernel's type
+  double *vp, *w, *x;
+  const double *pa1, *pa2;
+  double *pb1, *pb2;
+  LsqrState *st;
+};
+// systems of one batched launch: system i runs on workgroups [first[i], first[i+1])
+constexpr int kMaxJobs = 16;
+template <typename Args>
+struct Jobs {
+  Args a[kMaxJobs];
+  int first[kMaxJobs + 1];
+  int n;
+};
+// the system of workgroup b (uniform over the workgroup)
+template <typename Args>
+__device__ __forceinline__ int job_of(const Jobs<Args> &J, int b) {
+  int i = 0;
+  while (i + 1 < J.n && b >= J.first[i + 1]) ++i;
+  return i;
+}
+
+// (the pointers are parameters of their own so that they keep their
+// no-alias property when the body is inlined into a launch wrapper)
 template <typename T, int R, int L>
-__global__ void __launch_bounds__(64) lsqr_A(int ncell, int cgroup, const T *__restrict__ A,
-                                             const int *__restrict__ colmap,
-                                             const double *__restrict__ vp, double *__restrict__ u,
-                                             double *__restrict__ P, double *__restrict__ pa1,
-                                             const double *__restrict__ pb1, const double *__restrict__ pb2, int nb,
-                                             const LsqrState *__restrict__ sin, LsqrState *__restrict__ sout) {
+__device__ __forceinline__ void cell_A(const int ncell, const int cgroup, const T *__restrict__ A,
+                                       const int *__restrict__ colmap, const double *__restrict__ vp,
+                                       double *__restrict__ u, double *__restrict__ P, double *__restrict__ pa1,
+                                       const double *__restrict__ pb1, const double *__restrict__ pb2, const int nb,
+                                       const LsqrState *__restrict__ sin, LsqrState *__restrict__ sout,
+                                       const int bid, const int nblk) {
   constexpr int S = block_len<T, R, L>();
   __shared__ __align__(16) T blk[S];
   __shared__ double vs[64], uu[64], sh[1];
@@ -244,7 +284,7 @@ __global__ void __launch_bounds__(64) lsqr_A(int ncell, int cgroup, const T *__r
   // the iteration index travels with the scalars, so a launch takes no
   // per-iteration argument (a captured chunk replays unchanged)
   const int k = sin->kseq + 1;
-  if (blockIdx.x == 0) {
+  if (bid == 0) {
     const double nvs = reduce_partials<64>(pb1, nb, sh);
     const double dks = k >= 3 ? reduce_partials<64>(pb2, nb, sh) : 0.0;
     if (t == 0) {
@@ -259,7 +299,7 @@ __global__ void __launch_bounds__(64) lsqr_A(int ncell, int cgroup, const T *__r
   // this block's cells: a contiguous range of whole cell groups (consecutive
   // ranges on one XCD); one partial sum of ||u'||^2 per group, so the sums
   // do not depend on the grid size
-  const int G = gridDim.x - 1, lb = xcd_logical(blockIdx.x, 1, G + 1);
+  const int G = nblk - 1, lb = xcd_logical(bid, 1, G + 1);
   const int ngroup = (ncell + cgroup - 1) / cgroup;
   const int g0 = (int)((long long)lb * ngroup / G), g1 = (int)((long long)(lb + 1) * ngroup / G);
   const int cend = min(ncell, g1 * cgroup);
@@ -396,6 +436,19 @@ __global__ void __launch_bounds__(64) lsqr_A(int ncell, int cgroup, const T *__r
   }
 }
 
+#define TT_CELL_ARGS(T, a)                                                                                       \
+  (a).ncell, (a).cgroup, static_cast<const T *>((a).A), (a).colmap, (a).vp, (a).u, (a).P, (a).pa1, (a).pb1, \
+      (a).pb2, (a).nb, (a).sin, (a).sout
+template <typename T, int R, int L>
+__global__ void __launch_bounds__(64) lsqr_A(CellArgs a) {
+  cell_A<T, R, L>(TT_CELL_ARGS(T, a), blockIdx.x, gridDim.x);
+}
+template <typename T, int R, int L>
+__global__ void __launch_bounds__(64) lsqr_A_multi(Jobs<CellArgs> J) {
+  const int i = job_of(J, blockIdx.x);
+  cell_A<T, R, L>(TT_CELL_ARGS(T, J.a[i]), blockIdx.x - J.first[i], J.first[i + 1] - J.first[i]);
+}
+
 // orders a wavefront's LDS stores before its own later LDS loads by other
 // lanes (a wavefront's LDS operations complete in order; this keeps the
 // compiler from moving them across)
@@ -497,13 +550,12 @@ __device__ __forceinline__ double aw_compute(int t, int c, const T *blk, const f
 // wavefront and synchronised within it; the workgroup barrier sits at group
 // ends, which every wavefront reaches.
 template <typename T, int R, int L, int NW>
-__global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(int ncell, int cgroup, const T *__restrict__ A,
-                                                   const int *__restrict__ colmap,
-                                                   const double *__restrict__ vp, double *__restrict__ u,
-                                                   double *__restrict__ P, double *__restrict__ pa1,
-                                                   const double *__restrict__ pb1, const double *__restrict__ pb2,
-                                                   int nb, const LsqrState *__restrict__ sin,
-                                                   LsqrState *__restrict__ sout) {
+__device__ __forceinline__ void cell_Aw(const int ncell, const int cgroup, const T *__restrict__ A,
+                                        const int *__restrict__ colmap, const double *__restrict__ vp,
+                                        double *__restrict__ u, double *__restrict__ P, double *__restrict__ pa1,
+                                        const double *__restrict__ pb1, const double *__restrict__ pb2, const int nb,
+                                        const LsqrState *__restrict__ sin, LsqrState *__restrict__ sout,
+                                        const int bid, const int nblk) {
   static_assert(sizeof(T) <= 4, "multi-wavefront cell kernel: FP32 or FP16 blocks");
   constexpr int S = block_len<T, R, L>();
   __shared__ __align__(16) T blk_w[NW][S];
@@ -515,7 +567,7 @@ __global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(int ncell, int cgroup, con
   T *blk = blk_w[w];
   float *vsf = vsf_w[w], *uuf = uuf_w[w];
   const int k = sin->kseq + 1;
-  if (blockIdx.x == 0) {
+  if (bid == 0) {
     const double nvs = reduce_partials<64 * NW>(pb1, nb, sh);
     const double dks = k >= 3 ? reduce_partials<64 * NW>(pb2, nb, sh) : 0.0;
     if (threadIdx.x == 0) {
@@ -527,7 +579,7 @@ __global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(int ncell, int cgroup, con
     return;
   }
   if (sin->done) return;
-  const int G = gridDim.x - 1, lb = xcd_logical(blockIdx.x, 1, G + 1);
+  const int G = nblk - 1, lb = xcd_logical(bid, 1, G + 1);
   const int ngroup = (ncell + cgroup - 1) / cgroup;
   const int g0 = (int)((long long)lb * ngroup / G), g1 = (int)((long long)(lb + 1) * ngroup / G);
   if (g0 >= g1) return;
@@ -595,6 +647,16 @@ __global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(int ncell, int cgroup, con
   }
 }
 
+template <typename T, int R, int L, int NW>
+__global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(CellArgs a) {
+  cell_Aw<T, R, L, NW>(TT_CELL_ARGS(T, a), blockIdx.x, gridDim.x);
+}
+template <typename T, int R, int L, int NW>
+__global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw_multi(Jobs<CellArgs> J) {
+  const int i = job_of(J, blockIdx.x);
+  cell_Aw<T, R, L, NW>(TT_CELL_ARGS(T, J.a[i]), blockIdx.x - J.first[i], J.first[i + 1] - J.first[i]);
+}
+
 // ---------------------------------------------------------------------------
 // iteration k, unknown part (k = 0: initialisation after r0)
 //
@@ -638,14 +700,14 @@ __device__ __forceinline__ double group_sum_nb(double v, double *sh) {
 }
 
 template <int W, typename PT, int NB>
-__global__ void __launch_bounds__(NB) lsqr_B(int n, const int *__restrict__ gtab, const PT *__restrict__ P,
-                                             double *__restrict__ vp, double *__restrict__ w,
-                                             double *__restrict__ x, const double *__restrict__ pa1,
-                                             const double *__restrict__ pa2, int na, double *__restrict__ pb1,
-                                             double *__restrict__ pb2, LsqrState *st) {
+__device__ __forceinline__ void unk_B(const int n, const int *__restrict__ gtab, const PT *__restrict__ P,
+                                      double *__restrict__ vp, double *__restrict__ w, double *__restrict__ x,
+                                      const double *__restrict__ pa1, const double *__restrict__ pa2, const int na,
+                                      double *__restrict__ pb1, double *__restrict__ pb2, LsqrState *st,
+                                      const int bid, const int nblk) {
   static_assert(NB % kNB == 0, "whole groups of kNB unknowns per workgroup");
   __shared__ double sh[NB / 64];
-  const int jb = xcd_logical(blockIdx.x, 0, gridDim.x) * NB;  // first unknown of the workgroup
+  const int jb = xcd_logical(bid, 0, nblk) * NB;  // first unknown of the workgroup
   const int j = jb + threadIdx.x;
   // the index table and the unknown's own vectors do not depend on the
   // state: they are requested together with it
@@ -706,7 +768,7 @@ __global__ void __launch_bounds__(NB) lsqr_B(int n, const int *__restrict__ gtab
   const double s1 = group_sum_nb<NB>(nv, sh), s2 = group_sum_nb<NB>(dk, sh);
   const int grp = (jb + (int)threadIdx.x) / kNB, ngrp = (n + kNB - 1) / kNB;
   if (threadIdx.x % kNB == 0 && grp < ngrp) pb1[grp] = s1, pb2[grp] = s2;
-  if (blockIdx.x == 0) {
+  if (bid == 0) {
     double nbv = 0.0;
     if (k == 0) nbv = sqrt(reduce_partials_nb<NB>(pa2, na, sh));
     if (threadIdx.x == 0) {
@@ -718,6 +780,18 @@ __global__ void __launch_bounds__(NB) lsqr_B(int n, const int *__restrict__ gtab
       }
     }
   }
+}
+
+#define TT_UNK_ARGS(PT, a) \
+  (a).n, (a).gtab, static_cast<const PT *>((a).P), (a).vp, (a).w, (a).x, (a).pa1, (a).pa2, (a).na, (a).pb1, (a).pb2, (a).st
+template <int W, typename PT, int NB>
+__global__ void __launch_bounds__(NB) lsqr_B(UnkArgs a) {
+  unk_B<W, PT, NB>(TT_UNK_ARGS(PT, a), blockIdx.x, gridDim.x);
+}
+template <int W, typename PT, int NB>
+__global__ void __launch_bounds__(NB) lsqr_B_multi(Jobs<UnkArgs> J) {
+  const int i = job_of(J, blockIdx.x);
+  unk_B<W, PT, NB>(TT_UNK_ARGS(PT, J.a[i]), blockIdx.x - J.first[i], J.first[i + 1] - J.first[i]);
 }
 
 __global__ void finish_kernel(int n, const double *__restrict__ q, const double *__restrict__ dx,
@@ -772,52 +846,72 @@ int persistent_grid(int ngroup) {
   }
   return std::min(ngroup, it->second);
 }
-// e0/e1 non-null: the dispatch itself records its start and end (sampled timing)
-template <typename T, int R, int L>
-void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+// the cell kernel's arguments and grid (workgroups, the scalar one included)
+// for iteration k of system s
+template <typename T>
+CellArgs cell_args(BlockSys &s, LsqrWork &wk, int k, const T *A, int &grid) {
   constexpr int NW = sizeof(T) == 8 ? 1 : kNW;
   const int cg = cell_group(s.ncell, NW);
   const int ngroup = ceil_div(s.ncell, cg);
   const int tuned = s.grid_a[sizeof(T) == 8 ? 0 : sizeof(T) == 4 ? 1 : 2];
-  const int g = std::min(ngroup, tuned > 0 ? tuned : persistent_grid<T, R, L>(ngroup));
+  int g = 0;
+  if (s.R == 51)
+    g = std::min(ngroup, tuned > 0 ? tuned : persistent_grid<T, 51, 51>(ngroup));
+  else
+    g = std::min(ngroup, tuned > 0 ? tuned : persistent_grid<T, 27, 27>(ngroup));
   wk.na = ngroup;
-  const double *vp = wk.v.p, *pb1 = wk.pb1.p, *pb2 = wk.pb2.p;
-  const LsqrState *sin = wk.state.p + ((k - 1) & 1);
-  LsqrState *sout = wk.state.p + (k & 1);
-  const int nb = (int)ceil_div(s.n, kNB);
+  grid = g + 1;
+  CellArgs a;
+  a.ncell = s.ncell, a.cgroup = cg, a.nb = (int)ceil_div(s.n, kNB);
+  a.A = A, a.colmap = s.colmap.p;
+  a.vp = wk.v.p, a.pb1 = wk.pb1.p, a.pb2 = wk.pb2.p;
+  a.u = wk.u.p, a.P = wk.P.p, a.pa1 = wk.pa1.p;
+  a.sin = wk.state.p + ((k - 1) & 1);
+  a.sout = wk.state.p + (k & 1);
+  return a;
+}
+// e0/e1 non-null: the dispatch itself records its start and end (sampled timing)
+template <typename T, int R, int L>
+void launch_A(BlockSys &s, LsqrWork &wk, int k, const T *A, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+  constexpr int NW = sizeof(T) == 8 ? 1 : kNW;
+  int grid = 0;
+  const CellArgs a = cell_args<T>(s, wk, k, A, grid);
   if constexpr (NW == 1) {
     if (e0)
-      hipExtLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, e0, e1, 0, s.ncell, cg, A, s.colmap.p,
-                            vp, wk.u.p, wk.P.p, wk.pa1.p, pb1, pb2, nb, sin, sout);
+      hipExtLaunchKernelGGL((lsqr_A<T, R, L>), dim3(grid), dim3(64), 0, st, e0, e1, 0, a);
     else
-      hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(g + 1), dim3(64), 0, st, s.ncell, cg, A, s.colmap.p, vp, wk.u.p,
-                         wk.P.p, wk.pa1.p, pb1, pb2, nb, sin, sout);
+      hipLaunchKernelGGL((lsqr_A<T, R, L>), dim3(grid), dim3(64), 0, st, a);
   } else {
     if (e0)
-      hipExtLaunchKernelGGL((lsqr_Aw<T, R, L, NW>), dim3(g + 1), dim3(64 * NW), 0, st, e0, e1, 0, s.ncell, cg, A,
-                            s.colmap.p, vp, wk.u.p, wk.P.p, wk.pa1.p, pb1, pb2, nb, sin, sout);
+      hipExtLaunchKernelGGL((lsqr_Aw<T, R, L, NW>), dim3(grid), dim3(64 * NW), 0, st, e0, e1, 0, a);
     else
-      hipLaunchKernelGGL((lsqr_Aw<T, R, L, NW>), dim3(g + 1), dim3(64 * NW), 0, st, s.ncell, cg, A, s.colmap.p, vp,
-                         wk.u.p, wk.P.p, wk.pa1.p, pb1, pb2, nb, sin, sout);
+      hipLaunchKernelGGL((lsqr_Aw<T, R, L, NW>), dim3(grid), dim3(64 * NW), 0, st, a);
   }
 }
-// iteration 0 gathers the initial residual's FP64 partials; later ones the
-// cell kernel's partials in its block precision's type
+// the unknown kernel's arguments for iteration k (k = 0: the initial
+// residual's FP64 partials; later the cell kernel's partials in its block
+// precision's type)
+UnkArgs unk_args(BlockSys &s, LsqrWork &wk, int k) {
+  UnkArgs a;
+  a.n = s.n, a.na = k == 0 ? s.ncell : wk.na;
+  a.gtab = s.gtab.p, a.P = wk.P.p;
+  a.vp = wk.v.p, a.w = wk.w.p, a.x = wk.x.p;
+  a.pa1 = wk.pa1.p, a.pa2 = wk.pa2.p;
+  a.pb1 = wk.pb1.p, a.pb2 = wk.pb2.p;
+  a.st = wk.state.p + (k & 1);
+  return a;
+}
 template <int W>
 void launch_B(BlockSys &s, LsqrWork &wk, int k, int prec, hipStream_t st) {
   const int nbt = s.nb_b[prec] > 0 ? s.nb_b[prec] : kNB;  // threads per workgroup (tuned)
   const bool half = k >= 1 && prec == kBlocks16;
-  const int na = k == 0 ? s.ncell : wk.na;
+  const UnkArgs a = unk_args(s, wk, k);
   auto go = [&](auto nbc) {
     constexpr int NB = decltype(nbc)::value;
     if (half)
-      hipLaunchKernelGGL((lsqr_B<W, float, NB>), dim3(ceil_div(s.n, NB)), dim3(NB), 0, st, s.n, s.gtab.p,
-                         (const float *)wk.P.p, wk.v.p, wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, na, wk.pb1.p, wk.pb2.p,
-                         wk.state.p + (k & 1));
+      hipLaunchKernelGGL((lsqr_B<W, float, NB>), dim3(ceil_div(s.n, NB)), dim3(NB), 0, st, a);
     else
-      hipLaunchKernelGGL((lsqr_B<W, double, NB>), dim3(ceil_div(s.n, NB)), dim3(NB), 0, st, s.n, s.gtab.p,
-                         (const double *)wk.P.p, wk.v.p, wk.w.p, wk.x.p, wk.pa1.p, wk.pa2.p, na, wk.pb1.p, wk.pb2.p,
-                         wk.state.p + (k & 1));
+      hipLaunchKernelGGL((lsqr_B<W, double, NB>), dim3(ceil_div(s.n, NB)), dim3(NB), 0, st, a);
   };
   if (nbt == 1024)
     go(std::integral_constant<int, 1024>());
@@ -857,6 +951,69 @@ void stepB(BlockSys &s, LsqrWork &wk, int k, int prec, hipStream_t st) {
     case 12: launch_B<12>(s, wk, k, prec, st); break;
     default: launch_B<16>(s, wk, k, prec, st); break;
   }
+}
+
+// ---- several systems per launch (block_lsqr_multi): the same kernel bodies
+// on workgroup ranges, one range per system
+template <typename T, int R, int L>
+void launch_A_multi(const std::vector<LsqrJob *> &act, int k, int prec, hipStream_t st) {
+  constexpr int NW = sizeof(T) == 8 ? 1 : kNW;
+  Jobs<CellArgs> J;
+  J.n = (int)act.size();
+  J.first[0] = 0;
+  for (int i = 0; i < J.n; ++i) {
+    BlockSys &s = *act[i]->s;
+    const T *A = prec == kBlocks16   ? (const T *)s.A16.p
+                 : prec == kBlocks32 ? (const T *)s.A32.p
+                                     : (const T *)s.A.p;
+    int grid = 0;
+    J.a[i] = cell_args<T>(s, *act[i]->wk, k, A, grid);
+    J.first[i + 1] = J.first[i] + grid;
+  }
+  if constexpr (NW == 1)
+    hipLaunchKernelGGL((lsqr_A_multi<T, R, L>), dim3(J.first[J.n]), dim3(64), 0, st, J);
+  else
+    hipLaunchKernelGGL((lsqr_Aw_multi<T, R, L, NW>), dim3(J.first[J.n]), dim3(64 * NW), 0, st, J);
+}
+template <int R, int L>
+void stepA_multi_rl(const std::vector<LsqrJob *> &act, int k, int prec, hipStream_t st) {
+  if (prec == kBlocks16)
+    launch_A_multi<_Float16, R, L>(act, k, prec, st);
+  else if (prec == kBlocks32)
+    launch_A_multi<float, R, L>(act, k, prec, st);
+  else
+    launch_A_multi<double, R, L>(act, k, prec, st);
+}
+void stepA_multi(const std::vector<LsqrJob *> &act, int k, int prec, hipStream_t st) {
+  if (act[0]->s->R == 51)
+    stepA_multi_rl<51, 51>(act, k, prec, st);
+  else
+    stepA_multi_rl<27, 27>(act, k, prec, st);
+}
+// the unknown kernels of systems with gather width W, kNB threads per
+// workgroup (the results do not depend on the width of a workgroup)
+template <int W>
+void launch_B_multi(const std::vector<LsqrJob *> &act, int k, int prec, hipStream_t st) {
+  Jobs<UnkArgs> J;
+  J.n = 0;
+  J.first[0] = 0;
+  for (LsqrJob *j : act) {
+    if (j->s->gw != W) continue;
+    J.a[J.n] = unk_args(*j->s, *j->wk, k);
+    J.first[J.n + 1] = J.first[J.n] + (int)ceil_div(j->s->n, kNB);
+    ++J.n;
+  }
+  if (J.n == 0) return;
+  if (k >= 1 && prec == kBlocks16)
+    hipLaunchKernelGGL((lsqr_B_multi<W, float, kNB>), dim3(J.first[J.n]), dim3(kNB), 0, st, J);
+  else
+    hipLaunchKernelGGL((lsqr_B_multi<W, double, kNB>), dim3(J.first[J.n]), dim3(kNB), 0, st, J);
+}
+void stepB_multi(const std::vector<LsqrJob *> &act, int k, int prec, hipStream_t st) {
+  launch_B_multi<4>(act, k, prec, st);
+  launch_B_multi<8>(act, k, prec, st);
+  launch_B_multi<12>(act, k, prec, st);
+  launch_B_multi<16>(act, k, prec, st);
 }
 
 // events of a sampled launch of iteration k (recorded by the dispatch); -1 if not sampled
@@ -1268,6 +1425,80 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
   hipLaunchKernelGGL(finish_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, s.q.p, wk.x.p, x, x);
   TT_LAUNCH_CHECK();
   return res;
+}
+
+void block_lsqr_multi(std::vector<LsqrJob> &jobs, int prec, hipStream_t st) {
+  if (jobs.empty()) return;
+  if (jobs.size() > (size_t)kMaxJobs) throw std::runtime_error("too many systems for one batched LSQR");
+  const int R = jobs[0].s->R;
+  int itnlim_max = 0, pred = 0;
+  for (LsqrJob &j : jobs) {
+    BlockSys &s = *j.s;
+    LsqrWork &wk = *j.wk;
+    if (s.R != R) throw std::runtime_error("batched LSQR systems must share the block shape");
+    if (j.slot < 0 || j.slot > 2) throw std::runtime_error("solve history slot must be 0..2");
+    if (prec == kBlocks32 && !s.has32) throw std::runtime_error("single-precision blocks not built");
+    if (prec == kBlocks16 && !s.has16) throw std::runtime_error("half-precision blocks not built");
+    for (const LsqrJob &o : jobs)
+      if (&o != &j && o.wk == j.wk) throw std::runtime_error("batched LSQR systems need separate work areas");
+    const int m = s.ncell * s.R, n = s.n;
+    block_tune(s, prec, st);
+    wk.ensure(m, n, s.ncell, s.L);
+    LsqrState init;
+    std::memset(&init, 0, sizeof(init));
+    init.cs2 = -1.0;
+    init.atol = (double)j.eps4;
+    init.ctol = 1.0 / 1.0e8;
+    init.itnlim = j.maxit == 0 ? 10 * (m + n + 50) : j.maxit;
+    itnlim_max = std::max(itnlim_max, init.itnlim);
+    LsqrState init2[2] = {init, init};
+    TT_HIP(hipMemcpyAsync(wk.state.p, init2, sizeof(init2), hipMemcpyHostToDevice, st));
+    TT_HIP(hipMemsetAsync(wk.x.p, 0, sizeof(double) * n, st));
+    r0(s, wk, j.b, j.x, st);
+    TT_LAUNCH_CHECK();
+    stepB(s, wk, 0, prec, st);
+    TT_LAUNCH_CHECK();
+    // the batch runs until its slowest system stops: the largest prediction
+    const int p = s.hist[j.slot].predict();
+    pred = (p < 0 || pred < 0) ? -1 : std::max(pred, p);
+  }
+  int k = 1, batch = pred >= 0 ? pred + 4 : 8, extra = 0;
+  const int more = std::max(4, pred / 8);
+  std::vector<LsqrJob *> act;
+  for (LsqrJob &j : jobs) act.push_back(&j);
+  for (;;) {
+    for (int b2 = 0; b2 < batch; ++b2) {
+      stepA_multi(act, k, prec, st);
+      stepB_multi(act, k, prec, st);
+      ++k;
+    }
+    TT_LAUNCH_CHECK();
+    for (LsqrJob *j : act)
+      TT_HIP(hipMemcpyAsync(j->wk->host_state, j->wk->state.p + ((k - 1) & 1), sizeof(LsqrState),
+                            hipMemcpyDeviceToHost, st));
+    TT_HIP(hipStreamSynchronize(st));
+    std::vector<LsqrJob *> still;
+    for (LsqrJob *j : act) {
+      const LsqrState &hs = *j->wk->host_state;
+      if (!hs.done) {
+        still.push_back(j);
+        continue;
+      }
+      // later launches return at once for a stopped system: its x is final
+      j->res.nit = hs.itn;
+      j->res.istop = hs.istop;
+      j->s->hist[j->slot].add(j->res.nit);
+      const int n = j->s->n;
+      hipLaunchKernelGGL(finish_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, j->s->q.p, j->wk->x.p, j->x,
+                         j->x);
+      TT_LAUNCH_CHECK();
+    }
+    act.swap(still);
+    if (act.empty()) break;
+    if (k > itnlim_max + 3) throw std::runtime_error("LSQR did not signal termination");
+    batch = pred >= 0 ? std::min(64, more << std::min(extra, 4)) : std::max(more, std::min(64, k / 4));
+    ++extra;
+  }
 }
 
 }  // namespace tt
