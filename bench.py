@@ -118,11 +118,12 @@ def accuracy_leg(n, mixed, anchor_period=16):
             "seconds": {p: out[p][2] for p in out}}
 
 
-def serving_leg(requests, slots, max_iter, precs, seed=20251203):
+def serving_leg(requests, slots, max_iter, precs, max_batch=8, seed=20251203):
     """The serving workload's request stream (Poisson arrivals at 20/s, cubes
     of 6^3..16^3 subdomains, half convection / half cavity, each to a Picard
     change of 1e-6), first `requests` arrivals submitted in real time to the
-    request loop, once per precision.  Latency = finish - submit."""
+    request loop, once per precision; each slot steps a batch of up to
+    max_batch cases of one kind together.  Latency = finish - submit."""
     from tritok_amd import workloads
     from tritok_amd.serve import Server
     stream = workloads.serving_stream(count=requests, rate=20.0, seed=seed)
@@ -130,9 +131,9 @@ def serving_leg(requests, slots, max_iter, precs, seed=20251203):
     out = {"workload": "first %d requests of the seeded 2000-request stream: Poisson 20/s, n in {6,8,12,16}, "
                        "50%% convection Ra log-U[1e3,1e6] / 50%% cavity Re U[10,400], Picard to 1e-6 "
                        "(cap %d iterations)" % (requests, max_iter),
-           "requests": requests, "slots": slots, "arrival_rate_per_s": 20.0}
+           "requests": requests, "slots": slots, "max_batch": max_batch, "arrival_rate_per_s": 20.0}
     for prec in precs:
-        with Server(slots=slots, max_iter=max_iter) as srv:
+        with Server(slots=slots, max_iter=max_iter, max_batch=max_batch) as srv:
             t0 = time.perf_counter()
             ids = []
             for spec, h in zip(stream, hosts):
@@ -173,6 +174,7 @@ def serving_leg(requests, slots, max_iter, precs, seed=20251203):
                      "capped": int(sum(i["status"] == 1 for i in infos)),
                      "failed": int(sum(i["status"] < 0 for i in infos)),
                      "mean_picard_iterations": float(np.mean([i["picard_iterations"] for i in infos])),
+                     "batch_peak_mean": float(np.mean([i["batch_peak"] for i in infos])),
                      "per_request": [{"n": s["n"], "kind": s["kind"], "its": i["picard_iterations"],
                                       "status": i["status"], "latency_s": round(float(l), 3)}
                                      for s, i, l in zip(stream, infos, lat)]}
@@ -497,6 +499,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200,
                     help="requests of the serving stream replayed in real time (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-batch", type=int, default=8,
+                    help="cases of one kind a slot steps together (1: one case per slot)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving leg (mixed precision), then exit")
     ap.add_argument("--serve-max-iter", type=int, default=400)
     ap.add_argument("--batch-cells", type=int, default=16,
@@ -526,7 +530,7 @@ def main():
         assembly_only(args.assembly_only)
         return
     if args.serve_only:
-        sv = serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, (args.mixed,))
+        sv = serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, (args.mixed,), args.serve_batch)
         for p in sv:
             if isinstance(sv[p], dict) and "per_request" in sv[p]:
                 sv[p].pop("per_request")
@@ -636,7 +640,8 @@ def main():
     mh_sd = mhsd_leg(args.mhsd) if args.mhsd > 0 else None
     batched = batched_integrals_leg(args.batch_cells) if args.batch_cells > 0 else None
     bsolves = batched_solves_leg(args.batch_solves) if args.batch_solves > 0 else None
-    serving = (serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, ("fp64", args.mixed))
+    serving = (serving_leg(args.serve_requests, args.serve_slots, args.serve_max_iter, ("fp64", args.mixed),
+                           args.serve_batch)
                if args.serve_requests > 0 else None)
 
     ph = mixed["phases"]

@@ -612,9 +612,12 @@ int tt_write_sta_ite(const char *sta_path, const char *ite_path, const char *id_
 /* ------------------------------------------------------------------------ */
 /* Request loop over whole cases (no reference counterpart: the reference runs
  * one case per process).  Worker threads, one per slot, admit submitted cases
- * in arrival order as slots and device memory free up (first fit against the
- * budget), build each on its own stream, iterate it to convergence, keep the
- * converged fields on the host and release the device memory.  Cases in
+ * in arrival order as device memory frees up (first fit against the budget).
+ * A worker keeps a batch of up to max_batch active cases of one kind (same
+ * mesh size and precision), steps them together with
+ * tt_case_picard_step_batch, retires the finished ones (converged fields kept
+ * on the host, device memory released) and refills the batch from the queue
+ * between steps.  A case's results do not depend on its batch.  Batches of
  * different slots run concurrently on the device.                           */
 
 typedef struct tt_server tt_server;
@@ -627,6 +630,7 @@ typedef struct tt_server_info {
                                                admitted, built, finished */
   int64_t estimated_bytes, device_bytes;
   char error[256];
+  int32_t batch_peak;        /* most cases that shared one of its Picard steps */
 } tt_server_info;
 
 /* slots concurrent cases; mem_budget device bytes for admission (0: 80 % of
@@ -634,6 +638,10 @@ typedef struct tt_server_info {
  * max_iter caps the Picard steps per case (0: the case's own RELA limit). */
 int tt_server_create(int32_t slots, int64_t mem_budget, int32_t anchor_period, int32_t max_iter,
                      tt_server **out);
+/* As tt_server_create with batches of up to max_batch (1..16) cases per slot
+ * (tt_server_create: one case per slot). */
+int tt_server_create_batched(int32_t slots, int32_t max_batch, int64_t mem_budget, int32_t anchor_period,
+                             int32_t max_iter, tt_server **out);
 /* Queues the case held by h (which must live until the request finishes) in
  * precision prec (TT_PREC_*); *id identifies the request. */
 int tt_server_submit(tt_server *s, tt_host *h, int32_t prec, int64_t *id);

@@ -1,9 +1,11 @@
 """The request loop: each case's converged fields are the same when it runs
-alone and when it runs concurrently with others (slots sharing the device),
-in FP64 and in mixed precision; admission accounting matches the device
-memory the cases hold; two requests of the serving stream (6^3 cavity and
+alone and when it runs concurrently with others (slots sharing the device)
+or in a batch with others (continuous batching: shared LSQR launches), in
+FP64 and in mixed precision; admission accounting matches the device memory
+the cases hold; two requests of the serving stream (6^3 cavity and
 convection, tight tolerances) match the reference's converged fields within
-1e-8 relative L2 whether served alone or among other stream requests."""
+1e-8 relative L2 whether served alone, among other stream requests or in a
+batch with them."""
 import numpy as np
 import pytest
 
@@ -102,3 +104,44 @@ def test_stream_requests_match_reference_alone_and_in_the_mix(prec):
             e = rel_l2(np.asarray(mixed[k][f]).reshape(-1), ref)
             print(k, prec, f, "vs reference rel L2", e, flush=True)
             assert e < 1e-8, (k, prec, f, e)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp16"])
+def test_batched_serving_matches_alone_and_reference(prec):
+    from tritok_amd import workloads as W
+    from tritok_amd.serve import Server
+    fx = {k: load_golden(k) for k in ("sv6c", "sv6n")}
+    hosts = {k: host_for(g) for k, g in fx.items()}
+    others = [W.serving_case(sp) for sp in W.serving_stream(count=80) if sp["n"] == 6][:4]
+    order = [("sv6c", hosts["sv6c"]), (0, others[0]), ("sv6n", hosts["sv6n"]), (1, others[1]), (2, others[2]),
+             (3, others[3])]
+    alone = {}
+    with Server(slots=1) as s:
+        for k, h in order:
+            print("alone", k, prec, flush=True)
+            rid = s.submit(h, prec)
+            info = s.wait(rid)
+            assert info["status"] == 0 and info["batch_peak"] == 1, info
+            alone[k] = (s.fields(rid), info)
+    with Server(slots=1, max_batch=8) as s:
+        print("batched", prec, flush=True)
+        ids = {k: s.submit(h, prec) for k, h in order}
+        infos = {k: s.wait(i) for k, i in ids.items()}
+        got = {k: s.fields(i) for k, i in ids.items()}
+    for k, _ in order:
+        a, b = alone[k][1], infos[k]
+        print(k, prec, "picard", b["picard_iterations"], "batch peak", b["batch_peak"], flush=True)
+        assert b["status"] == 0 and b["batch_peak"] >= 2, b
+        assert b["picard_iterations"] == a["picard_iterations"] and b["lsqr_iterations"] == a["lsqr_iterations"]
+        for f in FIELDS:
+            assert np.array_equal(got[k][f], alone[k][0][f]), (k, prec, f)
+    for k, g in fx.items():
+        for f in FIELDS:
+            ref = g["rst_" + f].reshape(-1)
+            if np.linalg.norm(ref) == 0:
+                assert np.abs(got[k][f]).max() < 1e-12, (k, f)
+                continue
+            e = rel_l2(np.asarray(got[k][f]).reshape(-1), ref)
+            print(k, prec, f, "batched vs reference rel L2", e, flush=True)
+            assert e < 1e-8, (k, prec, f, e)
+

@@ -2,12 +2,15 @@
 // counterpart: the reference runs one case per process).
 //
 // Cases arrive as host-side preprocessed inputs (tt_host); a pool of worker
-// threads, one per slot, admits them in arrival order as slots and device
-// memory free up (first fit against the memory budget), builds each case on
-// its own stream, runs its Picard loop to convergence, copies the converged
-// fields back and releases the device memory.  Cases in different slots
-// overlap on the device: their least-squares and kinematics kernels run
-// concurrently on separate streams.
+// threads, one per slot, admits them in arrival order as device memory frees
+// up (first fit against the memory budget).  A worker holds a batch of up to
+// max_batch active cases of one kind (same mesh size and precision) and
+// steps them together (tt_case_picard_step_batch: every LSQR iteration of a
+// phase is one launch for the whole batch); between steps it retires the
+// cases that finished -- copying their fields back and releasing their device
+// memory -- and admits queued cases of its kind, so the batch is refilled
+// continuously.  A case's results do not depend on the batch it ran in.
+// Batches of different workers run concurrently on separate streams.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,16 +57,25 @@ struct Req {
   int64_t id = 0;
   tt_host *h = nullptr;
   int prec = 0;
+  int64_t kind = 0;  // cases of one kind share batches: mesh size and precision
   tt_server_info info{};
   int64_t est = 0;
   std::vector<double> vel, vort, qvort, temp, qtemp;
   bool done = false;
 };
 
+// a case being iterated by a worker
+struct Active {
+  Req *r = nullptr;
+  tt_case *c = nullptr;
+  int its = 0;
+  double change = 0.0;
+};
+
 }  // namespace
 
 struct tt_server {
-  int slots = 1, device = 0, anchor = 16, max_iter = 0;
+  int slots = 1, max_batch = 1, device = 0, anchor = 16, max_iter = 0;
   int64_t budget = 0, in_use = 0, next_id = 1;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -73,7 +85,8 @@ struct tt_server {
   std::vector<std::thread> workers;
   double t0 = 0;
 
-  void run_one(Req *r) {
+  // builds the case of r; on failure r is finished with the error
+  tt_case *admit(Req *r) {
     tt_case_view v;
     const double *f[5];
     int rc = tt_host_case_view(r->h, &v);
@@ -85,73 +98,146 @@ struct tt_server {
       r->info.device_bytes = tt_case_device_bytes(c);
       if (r->prec != TT_PREC_FP64) rc = tt_case_set_anchor_period(c, anchor);
     }
-    int its = 0, step = 1;
-    double err[9] = {0};
-    while (rc == 0 && step > 0) {
-      int32_t nit[12];
-      step = tt_case_picard_step(c, err, nit);
-      if (step < 0) {
-        rc = step;
-        break;
+    if (rc < 0) {
+      r->info.status = -1;
+      std::snprintf(r->info.error, sizeof(r->info.error), "%s", tt_last_error());
+      if (c) tt_case_free(c);
+      return nullptr;
+    }
+    return c;
+  }
+
+  // copies a finished case's fields back (status 0 converged, 1 at the cap,
+  // -1 failed), frees it and hands the request back
+  void retire(Active &a, int status, const char *err) {
+    Req *r = a.r;
+    r->info.picard_iterations = a.its;
+    r->info.final_change = a.change;
+    if (status >= 0 && a.c) {
+      tt_case_view v;
+      if (tt_host_case_view(r->h, &v) == 0) {
+        const size_t nn = v.mesh.nnodes, nq = v.mesh.nq;
+        r->vel.resize(nn * 3), r->vort.resize(nn * 3), r->qvort.resize(nq * 3), r->temp.resize(nn),
+            r->qtemp.resize(nq);
+        if (tt_case_fields(a.c, r->vel.data(), r->vort.data(), r->qvort.data(), r->temp.data(), r->qtemp.data()) < 0)
+          status = -1, err = tt_last_error();
+      } else {
+        status = -1, err = tt_last_error();
       }
-      ++its;
-      for (int k = 0; k < 12; ++k) r->info.lsqr_iterations += nit[k];
-      if (max_iter > 0 && its >= max_iter) break;
     }
-    if (rc == 0) {
-      const size_t nn = v.mesh.nnodes, nq = v.mesh.nq;
-      r->vel.resize(nn * 3), r->vort.resize(nn * 3), r->qvort.resize(nq * 3), r->temp.resize(nn),
-          r->qtemp.resize(nq);
-      rc = tt_case_fields(c, r->vel.data(), r->vort.data(), r->qvort.data(), r->temp.data(), r->qtemp.data());
+    r->info.status = status;
+    if (status < 0) std::snprintf(r->info.error, sizeof(r->info.error), "%s", err ? err : "failed");
+    if (a.c) tt_case_free(a.c);
+    a.c = nullptr;
+    finish(r);
+  }
+
+  void finish(Req *r) {
+    r->info.t_end = now_s() - t0;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      in_use -= r->est;
+      r->done = true;
     }
-    double change = 0;
-    for (double e : err) change = std::max(change, e);
-    r->info.picard_iterations = its;
-    r->info.final_change = change;
-    r->info.status = rc < 0 ? -1 : (step == 0 ? 0 : 1);  // 0 converged, 1 stopped at the iteration cap
-    if (rc < 0) std::snprintf(r->info.error, sizeof(r->info.error), "%s", tt_last_error());
-    if (c) tt_case_free(c);
+    cv_work.notify_all();
+    cv_done.notify_all();
+  }
+
+  // takes the first queued request (arrival order) that fits the memory
+  // left and, when kind >= 0, is of that kind; nullptr if none (the caller
+  // holds mu)
+  Req *take(int64_t kind) {
+    auto it = std::find_if(queue.begin(), queue.end(), [&](Req *q) {
+      return (kind < 0 || q->kind == kind) && (in_use == 0 || in_use + q->est <= budget);
+    });
+    if (it == queue.end()) return nullptr;
+    Req *r = *it;
+    queue.erase(it);
+    in_use += r->est;
+    return r;
   }
 
   void worker() {
     (void)hipSetDevice(device);
+    std::vector<Active> act;
+    std::vector<tt_case *> cs;
+    std::vector<double> err;
+    std::vector<int32_t> nit, st;
     for (;;) {
-      Req *r = nullptr;
+      // admission: an idle worker waits for any request; a busy one tops its
+      // batch up with queued requests of its kind
+      std::vector<Req *> fresh;
       {
         std::unique_lock<std::mutex> lk(mu);
-        for (;;) {
-          if (stopping && queue.empty()) return;
-          // first request, in arrival order, that fits the memory left
-          auto it = std::find_if(queue.begin(), queue.end(),
-                                 [&](Req *q) { return in_use == 0 || in_use + q->est <= budget; });
-          if (it != queue.end()) {
-            r = *it;
-            queue.erase(it);
-            in_use += r->est;
-            break;
+        if (act.empty()) {
+          for (;;) {
+            if (stopping && queue.empty()) return;
+            Req *r = take(-1);
+            if (r) {
+              fresh.push_back(r);
+              break;
+            }
+            cv_work.wait(lk);
           }
-          cv_work.wait(lk);
+        }
+        const int64_t kind = act.empty() ? fresh[0]->kind : act[0].r->kind;
+        while ((int)(act.size() + fresh.size()) < max_batch) {
+          Req *r = take(kind);
+          if (!r) break;
+          fresh.push_back(r);
         }
       }
-      r->info.t_start = now_s() - t0;
-      run_one(r);
-      r->info.t_end = now_s() - t0;
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        in_use -= r->est;
-        r->done = true;
+      for (Req *r : fresh) {
+        r->info.t_start = now_s() - t0;
+        Active a;
+        a.r = r;
+        a.c = admit(r);
+        if (!a.c) {
+          finish(r);
+          continue;
+        }
+        act.push_back(a);
       }
-      cv_work.notify_all();
-      cv_done.notify_all();
+      if (act.empty()) continue;
+      // one Picard iteration of every active case
+      const int k = (int)act.size();
+      cs.resize(k), err.assign((size_t)9 * k, 0.0), nit.assign((size_t)12 * k, 0), st.assign(k, 0);
+      for (int i = 0; i < k; ++i) cs[i] = act[i].c;
+      const int rc = tt_case_picard_step_batch(cs.data(), k, err.data(), nit.data(), st.data());
+      const std::string batch_err = rc < 0 ? tt_last_error() : "";
+      std::vector<Active> keep;
+      for (int i = 0; i < k; ++i) {
+        Active &a = act[i];
+        if (rc < 0) {
+          retire(a, -1, batch_err.c_str());
+          continue;
+        }
+        ++a.its;
+        a.r->info.batch_peak = std::max(a.r->info.batch_peak, k);
+        for (int q = 0; q < 12; ++q) a.r->info.lsqr_iterations += nit[(size_t)12 * i + q];
+        a.change = 0.0;
+        for (int q = 0; q < 9; ++q) a.change = std::max(a.change, err[(size_t)9 * i + q]);
+        if (st[i] < 0)
+          retire(a, -1, tt_last_error());
+        else if (st[i] == 0)
+          retire(a, 0, nullptr);
+        else if (max_iter > 0 && a.its >= max_iter)
+          retire(a, 1, nullptr);
+        else
+          keep.push_back(a);
+      }
+      act.swap(keep);
     }
   }
 };
 
-extern "C" int tt_server_create(int32_t slots, int64_t mem_budget, int32_t anchor_period, int32_t max_iter,
-                                tt_server **out) {
+extern "C" int tt_server_create_batched(int32_t slots, int32_t max_batch, int64_t mem_budget,
+                                        int32_t anchor_period, int32_t max_iter, tt_server **out) {
   try {
     if (!out || slots < 1 || slots > 64) throw std::runtime_error("slots must be 1..64");
+    if (max_batch < 1 || max_batch > 16) throw std::runtime_error("max_batch must be 1..16");
     auto s = std::make_unique<tt_server>();
+    s->max_batch = max_batch;
     if (hipGetDevice(&s->device) != hipSuccess) throw std::runtime_error("no HIP device");
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) throw std::runtime_error("hipMemGetInfo failed");
@@ -169,6 +255,11 @@ extern "C" int tt_server_create(int32_t slots, int64_t mem_budget, int32_t ancho
   }
 }
 
+extern "C" int tt_server_create(int32_t slots, int64_t mem_budget, int32_t anchor_period, int32_t max_iter,
+                                tt_server **out) {
+  return tt_server_create_batched(slots, 1, mem_budget, anchor_period, max_iter, out);
+}
+
 extern "C" int tt_server_submit(tt_server *s, tt_host *h, int32_t prec, int64_t *id) {
   try {
     if (!s || !h || !id) throw std::runtime_error("NULL argument");
@@ -179,6 +270,7 @@ extern "C" int tt_server_submit(tt_server *s, tt_host *h, int32_t prec, int64_t 
     r->h = h;
     r->prec = prec;
     r->est = estimate_bytes(v, prec);
+    r->kind = (int64_t)v.mesh.nicell * 8 + prec;
     {
       std::lock_guard<std::mutex> lk(s->mu);
       if (s->stopping) throw std::runtime_error("server is shutting down");

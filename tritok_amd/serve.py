@@ -15,10 +15,10 @@ i64 = C.c_int64
 class ServerInfo(C.Structure):
     _fields_ = [("status", i32), ("picard_iterations", i32), ("lsqr_iterations", i64), ("final_change", C.c_double),
                 ("t_submit", C.c_double), ("t_start", C.c_double), ("t_setup", C.c_double), ("t_end", C.c_double),
-                ("estimated_bytes", i64), ("device_bytes", i64), ("error", C.c_char * 256)]
+                ("estimated_bytes", i64), ("device_bytes", i64), ("error", C.c_char * 256), ("batch_peak", i32)]
 
 
-_create = _lib._sig("tt_server_create", i32, i32, i64, i32, i32, C.POINTER(P))
+_create = _lib._sig("tt_server_create_batched", i32, i32, i32, i64, i32, i32, C.POINTER(P))
 _submit = _lib._sig("tt_server_submit", i32, P, P, i32, C.POINTER(i64))
 _wait = _lib._sig("tt_server_wait", i32, P, i64, C.POINTER(ServerInfo))
 _fields = _lib._sig("tt_server_fields", i32, P, i64, PD, PD, PD, PD, PD)
@@ -30,9 +30,10 @@ PREC = {"fp64": 0, "fp32": 1, "bf16": 2, "fp16": 3}
 
 
 class Server:
-    def __init__(self, slots=4, mem_budget=0, anchor_period=16, max_iter=0):
+    def __init__(self, slots=4, mem_budget=0, anchor_period=16, max_iter=0, max_batch=1):
         h = P()
-        check(_create(int(slots), int(mem_budget), int(anchor_period), int(max_iter), C.byref(h)), "tt_server_create")
+        check(_create(int(slots), int(max_batch), int(mem_budget), int(anchor_period), int(max_iter), C.byref(h)),
+              "tt_server_create_batched")
         self._h = h
         self._hosts = {}
 
