@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(64) macro_dom_kernel(MeshPtrs m, const QuadDev
     for (int d = 0; d < 3; ++d) cc[k][d] = m.x[(node - 1) + (size_t)m.nn * d];
   }
   WaveCell<4, Helm> w;
-  w.run(*qp, p, cc, it + 1, mu, lds);
+  w.run(*qp, p, cc, it + 1, mu, lds, false);  // the regular sub-cubes: macro_reg_kernel
   const int lane = threadIdx.x;
   if (lane < 27) {
     o.Dx[i + m1 * lane] = w.acc[0];
@@ -238,6 +238,136 @@ __global__ void __launch_bounds__(64) macro_dom_kernel(MeshPtrs m, const QuadDev
     long n = w.npts;
     for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
     if (lane == 0) atomicAdd(&npts[0], (unsigned long long)n);
+  }
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// The domain integral over the sub-cubes that do not hold the source, for all
+// 51 sources of a cell at once (one workgroup per cell, four wavefronts of 16
+// sources).  These sub-cubes' Gauss points do not depend on the source: their
+// positions and weighted Jacobians are computed once per cell into LDS, and
+// the contraction sum_points shape(point) * kernel(source, point) runs on the
+// FP64 matrix cores (16x16x4: shapes x points x sources), with the kernel
+// values of each point evaluated as WaveCell does.  A source on or inside a
+// sub-cube's closure is integrated over that sub-cube by macro_dom_kernel;
+// this kernel adds the rest to its sums (so it runs after it).
+constexpr int kRegChunk = 512;  // Gauss points staged per pass
+template <bool Helm>
+__global__ void __launch_bounds__(256) macro_reg_kernel(MeshPtrs m, const QuadDev *__restrict__ qp,
+                                                        const double *__restrict__ mun, const double *__restrict__ muq,
+                                                        MacroOut o, unsigned long long *__restrict__ npts) {
+  __shared__ double4 geo[kRegChunk];  // x, y, z, weight * Jacobian
+  __shared__ double4 loc[kRegChunk];  // local coordinates of the point (w: 0 for padding)
+  const QuadDev &q = *qp;
+  const int ic = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, n = lane & 15, kq = lane >> 4;
+  const int it = 16 * w + n;  // this lane's source row (kernel operand)
+  const bool valid = it < 51;
+  const size_t nc = m.nc, m1 = nc * 51, i = (size_t)ic * 51 + (valid ? it : 0);
+  double p[3] = {0.0, 0.0, 0.0}, mu = 0.0;
+  if (valid) macro_source<Helm>(m, ic, it, mun, muq, p, mu);
+  double cc[8][3];
+  for (int k = 0; k < 8; ++k) {
+    const int node = m.idc[ic + nc * (kGeo8[k] - 1)];
+    for (int d = 0; d < 3; ++d) cc[k][d] = m.x[(node - 1) + (size_t)m.nn * d];
+  }
+  // the shape functions of this lane's operand rows 16t + n (node positions)
+  int sa[2], sb[2], sc[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int sf = 16 * t + n;
+    sa[t] = sf < 27 ? kNode27[sf][0] + 1 : 1, sb[t] = sf < 27 ? kNode27[sf][1] + 1 : 1;
+    sc[t] = sf < 27 ? kNode27[sf][2] + 1 : 1;
+  }
+  dbl4 acc[4][2];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) acc[c][0] = acc[c][1] = dbl4{0.0, 0.0, 0.0, 0.0};
+  long cnt = 0;
+  const int nd = q.iiDiv, n1 = q.ndr, np = n1 * n1 * n1;
+  for (int ix = 1; ix <= nd; ++ix)
+    for (int iy = 1; iy <= nd; ++iy)
+      for (int iz = 1; iz <= nd; ++iz) {
+        const double a = -1.0 + (ix - 1) * 2.0 / nd, b = -1.0 + ix * 2.0 / nd;
+        const double c0 = -1.0 + (iy - 1) * 2.0 / nd, d = -1.0 + iy * 2.0 / nd;
+        const double e = -1.0 + (iz - 1) * 2.0 / nd, f = -1.0 + iz * 2.0 / nd;
+        const double dex = 0.125 * (b - a) * (d - c0) * (f - e);
+        // as WaveCell::run decides: the source's position in the sub-cube
+        bool use = valid;
+        if (valid) {
+          const double xx1 = ((-1.0 + 2.0 * kKsi51[it]) - a) / (b - a);
+          const double yy1 = ((-1.0 + 2.0 * kEta51[it]) - c0) / (d - c0);
+          const double zz1 = ((-1.0 + 2.0 * kZet51[it]) - e) / (f - e);
+          use = !(xx1 >= 0.0 && xx1 <= 1.0 && yy1 >= 0.0 && yy1 <= 1.0 && zz1 >= 0.0 && zz1 <= 1.0);
+        }
+        if (use && kq == 0) cnt += np;
+        for (int base = 0; base < np; base += kRegChunk) {
+          const int len = min(kRegChunk, np - base), len4 = (len + 3) / 4 * 4;
+          __syncthreads();  // the previous chunk's reads are done
+          for (int j = threadIdx.x; j < len4; j += 256) {
+            double4 g{0.0, 0.0, 0.0, 0.0}, l{0.0, 0.0, 0.0, 0.0};
+            if (j < len) {
+              const int t = base + j;
+              const int I = t % n1, J = (t / n1) % n1, K = t / (n1 * n1);
+              const double gii = a + 0.5 * (q.gdr[I] + 1.0) * (b - a);
+              const double gij = c0 + 0.5 * (q.gdr[J] + 1.0) * (d - c0);
+              const double gik = e + 0.5 * (q.gdr[K] + 1.0) * (f - e);
+              const double wpre = 1.953125e-3 * q.odr[I] * (q.odr[J] * q.odr[K] * dex);
+              double xc[3], dj;
+              cell_map(cc, gii, gij, gik, xc, &dj);
+              g = double4{xc[0], xc[1], xc[2], wpre * dj};
+              l = double4{gii, gij, gik, 1.0};
+            }
+            geo[j] = g, loc[j] = l;
+          }
+          __syncthreads();
+          for (int k0 = 0; k0 < len4; k0 += 4) {
+            const double4 g = geo[k0 + kq], l = loc[k0 + kq];
+            double lx[3], ly[3], lz[3];
+            lagrange3(l.x, lx);
+            lagrange3(l.y, ly);
+            lagrange3(l.z, lz);
+            double ph[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) ph[t] = lx[sa[t]] * ly[sb[t]] * lz[sc[t]];
+            double fq[4] = {0.0, 0.0, 0.0, 0.0};
+            if (use && l.w != 0.0) {
+              const double rx = p[0] - g.x, ry = p[1] - g.y, rz = p[2] - g.z;
+              double ra = sqrt(rx * rx + ry * ry + rz * rz);
+              if (ra < 1.0e-12) ra = 1.0e-12;
+              const double ira2 = 1.0 / (ra * ra);
+              double fung, gf;
+              kernel_factors<Helm>(ra, mu, fung, gf);
+              fq[0] = rx * ira2 * fung * gf * g.w;
+              fq[1] = ry * ira2 * fung * gf * g.w;
+              fq[2] = rz * ira2 * fung * gf * g.w;
+              fq[3] = fung * g.w;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              acc[c][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph[0], fq[c], acc[c][0], 0, 0, 0);
+              acc[c][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph[1], fq[c], acc[c][1], 0, 0, 0);
+            }
+          }
+        }
+      }
+  // register r of tile t in lane l: shape function 16t + 4r + l/16, source l % 16
+  if (valid) {
+    double *out[4] = {o.Dx, o.Dy, o.Dz, o.B};
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int sf = 16 * t + 4 * r + kq;
+        if (sf >= 27) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c][i + m1 * sf] += acc[c][t][r];
+      }
+  }
+  if (npts) {
+    long v = cnt;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) atomicAdd(&npts[0], (unsigned long long)v);
   }
 }
 
@@ -367,6 +497,8 @@ void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev
     TT_LAUNCH_CHECK();
     hipLaunchKernelGGL(macro_dom_kernel<true>, dim3(ncomp, 51), dim3(64), 0, s, ptrs(m), dq.p, mun, muq, out, cnt.p);
     TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(macro_reg_kernel<true>, dim3(ncomp), dim3(256), 0, s, ptrs(m), dq.p, mun, muq, out, cnt.p);
+    TT_LAUNCH_CHECK();
     hipLaunchKernelGGL(macro_corr_kernel<true>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), ncomp, mun,
                        muq, out);
   } else {
@@ -374,6 +506,9 @@ void macro_integrals_dev(const DevMesh &m, const QuadDev &qh, int copy, MacroDev
                        nullptr, nullptr, out, cnt.p);
     TT_LAUNCH_CHECK();
     hipLaunchKernelGGL(macro_dom_kernel<false>, dim3(ncomp, 51), dim3(64), 0, s, ptrs(m), dq.p, nullptr, nullptr, out,
+                       cnt.p);
+    TT_LAUNCH_CHECK();
+    hipLaunchKernelGGL(macro_reg_kernel<false>, dim3(ncomp), dim3(256), 0, s, ptrs(m), dq.p, nullptr, nullptr, out,
                        cnt.p);
     TT_LAUNCH_CHECK();
     hipLaunchKernelGGL(macro_corr_kernel<false>, dim3(ceil_div(ncomp, 64), 51), dim3(64), 0, s, ptrs(m), ncomp,
@@ -467,7 +602,6 @@ __global__ void __launch_bounds__(256) sd_bnd_pairs_kernel(MeshPtrs m, const Qua
 // is the A operand and the kernel values, one (row, point) pair per lane, are
 // the B operand of v_mfma_f64_16x16x4f64.  Cells of one colour share no node,
 // so the waves of one launch add into disjoint entries of D.
-typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // geo[(c*Pp + p)*4 + {0,1,2,3}] = position and weight w_p |J| of point p in
 // cell c (Pp: the point count padded to a multiple of 16; padding points have

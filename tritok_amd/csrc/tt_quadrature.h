@@ -393,8 +393,11 @@ struct WaveCell {
   }
 
   // the integral over the cell; on return lanes 0..26 hold the sums of shape
-  // functions 0..26 (both halves combined)
-  __device__ void run(const QuadDev &q, const double *p, const double (&c)[8][3], int isrc, double mu, double *lds) {
+  // functions 0..26 (both halves combined).  regular = false leaves out the
+  // sub-cubes that do not hold the source (integrated elsewhere, see
+  // macro_reg_kernel).
+  __device__ void run(const QuadDev &q, const double *p, const double (&c)[8][3], int isrc, double mu, double *lds,
+                      bool regular = true) {
     const double PI = 3.141592653589793, PI2 = 2.0 * PI;
     const int lane = threadIdx.x & 63, k = lane & 31;
     const int ax = k < 27 ? kNode27[k][0] + 1 : 1, ay = k < 27 ? kNode27[k][1] + 1 : 1,
@@ -516,7 +519,7 @@ struct WaveCell {
                 }
               }
             }
-          } else {
+          } else if (regular) {
             const int n1 = q.ndr, np = n1 * n1 * n1;
             for (int base = 0; base < np; base += 64) {
               const int t = base + lane;
