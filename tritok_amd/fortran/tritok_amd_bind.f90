@@ -187,6 +187,15 @@ MODULE tritok_amd_bind
       INTEGER(C_INT32_T) :: nit(12)
       INTEGER(C_INT) :: tt_case_picard_step
     END FUNCTION
+    ! several cases per step (err(9,ncase), nit(12,ncase), status(ncase) per case)
+    FUNCTION tt_case_picard_step_batch(hs, ncase, err, nit, status) BIND(C, NAME="tt_case_picard_step_batch")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_PTR
+      TYPE(C_PTR) :: hs(*)
+      INTEGER(C_INT32_T), VALUE :: ncase
+      REAL(C_DOUBLE) :: err(9, *)
+      INTEGER(C_INT32_T) :: nit(12, *), status(*)
+      INTEGER(C_INT) :: tt_case_picard_step_batch
+    END FUNCTION
     FUNCTION tt_case_fields(h, vel, vort, qvort, temp, qtemp) BIND(C, NAME="tt_case_fields")
       IMPORT :: C_INT, C_DOUBLE, C_PTR
       TYPE(C_PTR), VALUE :: h
