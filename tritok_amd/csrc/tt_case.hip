@@ -311,7 +311,7 @@ struct tt_case {
   int64_t km_solver_warnings = 0;  // kinematics solves that ended with istop > 3 (the case continues)
   std::vector<double> vort0, temp0;  // fields at the start of the time step (restart pts/ppts fields)
   double t_phase[11] = {0};
-  hipEvent_t ev[8];
+  hipEvent_t join = nullptr;  // orders this case's stream against a batch's stream
 };
 
 namespace {
@@ -450,6 +450,7 @@ extern "C" int tt_case_create(const tt_case_view *cv, const double *vel, const d
     const auto &m = cv->mesh;
     const int nn = m.nnodes, nb = m.nbnodes, nc = m.nicell, nq = m.nq;
     TT_HIP(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    TT_HIP(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
     hipStream_t s = c->s;
     c->kode_h.assign(cv->kode, cv->kode + (size_t)nb * 3);
     bool need_kobc = false;
@@ -598,12 +599,22 @@ void picard_multi(const std::vector<tt_case *> &cs, std::vector<StepOut> &out) {
     }
   };
   auto all = [](tt_case &) { return true; };
-  // ---- boundary vorticities
+  // ---- boundary vorticities: the cases' steps are independent; in a batch
+  // each runs on its own stream after the batch stream's earlier work, and
+  // the batch stream waits for them all
   {
     EvTimer t(s);
+    if (K > 1) {
+      TT_HIP(hipEventRecord(cs[0]->join, s));
+      for (size_t i = 1; i < K; ++i) TT_HIP(hipStreamWaitEvent(cs[i]->s, cs[0]->join, 0));
+    }
     for (size_t i = 0; i < K; ++i) {
       tt_case &c = *cs[i];
-      if (c.cv.iBw == 1) dense_bw_step(c.dense, c.mesh, c.vel.p, c.vort.p, c.cv.urBw, s);
+      if (c.cv.iBw == 1) dense_bw_step(c.dense, c.mesh, c.vel.p, c.vort.p, c.cv.urBw, i == 0 ? s : c.s);
+    }
+    for (size_t i = 1; i < K; ++i) {
+      TT_HIP(hipEventRecord(cs[i]->join, cs[i]->s));
+      TT_HIP(hipStreamWaitEvent(s, cs[i]->join, 0));
     }
     tp[0] = t.stop();
   }
@@ -898,7 +909,9 @@ extern "C" void tt_case_free(tt_case *c) {
   // the case's last launches (graph replays included) finish before their
   // buffers and executable graphs are released
   if (s) (void)hipStreamSynchronize(s);
+  hipEvent_t j = c->join;
   delete c;
+  if (j) (void)hipEventDestroy(j);
   if (s) (void)hipStreamDestroy(s);
 }
 

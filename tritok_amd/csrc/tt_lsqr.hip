@@ -178,56 +178,6 @@ inline int cell_group(int ncell, int nw) {
 }
 inline int cell_waves(int prec) { return prec == kBlocks64 ? 1 : kNW; }
 
-// ---------------------------------------------------------------------------
-// r0 = b - A (x0/q), P = A^T r0; per-cell partials of ||r0||^2 and ||b||^2
-template <int R, int L>
-__global__ void __launch_bounds__(64) lsqr_r0(const double *__restrict__ A, const int *__restrict__ colmap,
-                                              const double *__restrict__ q, const double *__restrict__ x0,
-                                              const double *__restrict__ b, double *__restrict__ u,
-                                              double *__restrict__ P, double *__restrict__ pa1,
-                                              double *__restrict__ pa2) {
-  constexpr int S = (R * L + 3) / 4 * 4;
-  __shared__ __align__(16) double blk[S];
-  __shared__ double xs[64], rr[64];
-  const int c = blockIdx.x, t = threadIdx.x;
-  stage_block<double, S>(A + (size_t)c * S, blk);
-  if (t < L) {
-    int j = colmap[(size_t)c * L + t];
-    xs[t] = j >= 0 ? x0[j] / q[j] : 0.0;
-  }
-  const double bb = t < R ? b[(size_t)c * R + t] : 0.0;
-  __syncthreads();
-  double r = 0.0;
-  if (t < R) {
-    double s = 0.0;
-#pragma unroll
-    for (int l = 0; l < L; ++l) s += blk[t * L + l] * xs[l];
-    r = bb - s;
-    u[(size_t)c * R + t] = r;
-    rr[t] = r;
-  }
-  __syncthreads();
-  if (t < L) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < R; ++i) s += blk[i * L + t] * rr[i];
-    P[(size_t)c * L + t] = s;
-  }
-  const double s1 = wave_sum(r * r), s2 = wave_sum(bb * bb);
-  if (t == 0) pa1[c] = s1, pa2[c] = s2;
-}
-
-// ---------------------------------------------------------------------------
-// iteration k, cell part.  Block 0 runs the scalar recurrences (stopping
-// tests of iteration k-2, rotation of iteration k-1) and stores the state;
-// blocks 1..G do u' = A v - alpha u and P = A^T u' for a contiguous range of
-// about ncell/G cells (G sized to the blocks resident on the device at once;
-// the ranges of one XCD adjacent, see xcd_logical), prefetching the
-// next cell's block into registers while computing on the current one.  The
-// cell blocks need only alpha_k = ||v_k||, which every block reduces from the
-// unknown kernel's partials in the same order; when block 0 finds that the
-// iteration stopped, the cell blocks' work of this launch is simply unused
-// (the unknown kernel that follows sees the stop and leaves x unchanged).
 // The arguments of one system's cell kernel and unknown kernel.  A kernel
 // body takes them with its workgroup index and workgroup count, so that one
 // launch can run the iterations of several systems (workgroup ranges per
@@ -240,6 +190,23 @@ struct CellArgs {
   double *u, *P, *pa1;
   const LsqrState *sin;
   LsqrState *sout;
+};
+struct R0Args {  // initial residual of one system
+  const double *A, *q, *x0, *b;
+  const int *colmap;
+  double *u, *P, *pa1, *pa2;
+  int ncell;
+};
+struct InitArgs {  // the state copies and dx = 0 of one system's solve
+  LsqrState *st;
+  double *x;
+  double atol;
+  int n, itnlim;
+};
+struct FinArgs {  // x = q * dx + x0 of one system
+  const double *q, *dx, *x0;
+  double *x;
+  int n;
 };
 struct UnkArgs {
   int n, na;
@@ -268,6 +235,88 @@ __device__ __forceinline__ int job_of(const Jobs<Args> &J, int b) {
 
 // (the pointers are parameters of their own so that they keep their
 // no-alias property when the body is inlined into a launch wrapper)
+// ---------------------------------------------------------------------------
+// r0 = b - A (x0/q), P = A^T r0; per-cell partials of ||r0||^2 and ||b||^2
+template <int R, int L>
+__device__ __forceinline__ void cell_r0(const double *__restrict__ A, const int *__restrict__ colmap,
+                                        const double *__restrict__ q, const double *__restrict__ x0,
+                                        const double *__restrict__ b, double *__restrict__ u, double *__restrict__ P,
+                                        double *__restrict__ pa1, double *__restrict__ pa2, const int c) {
+  constexpr int S = (R * L + 3) / 4 * 4;
+  __shared__ __align__(16) double blk[S];
+  __shared__ double xs[64], rr[64];
+  const int t = threadIdx.x;
+  stage_block<double, S>(A + (size_t)c * S, blk);
+  if (t < L) {
+    int j = colmap[(size_t)c * L + t];
+    xs[t] = j >= 0 ? x0[j] / q[j] : 0.0;
+  }
+  const double bb = t < R ? b[(size_t)c * R + t] : 0.0;
+  __syncthreads();
+  double r = 0.0;
+  if (t < R) {
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < L; ++l) s += blk[t * L + l] * xs[l];
+    r = bb - s;
+    u[(size_t)c * R + t] = r;
+    rr[t] = r;
+  }
+  __syncthreads();
+  if (t < L) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < R; ++i) s += blk[i * L + t] * rr[i];
+    P[(size_t)c * L + t] = s;
+  }
+  const double s1 = wave_sum(r * r), s2 = wave_sum(bb * bb);
+  if (t == 0) pa1[c] = s1, pa2[c] = s2;
+}
+template <int R, int L>
+__global__ void __launch_bounds__(64) lsqr_r0(R0Args a) {
+  cell_r0<R, L>(a.A, a.colmap, a.q, a.x0, a.b, a.u, a.P, a.pa1, a.pa2, blockIdx.x);
+}
+template <int R, int L>
+__global__ void __launch_bounds__(64) lsqr_r0_multi(Jobs<R0Args> J) {
+  const int i = job_of(J, blockIdx.x);
+  const R0Args &a = J.a[i];
+  cell_r0<R, L>(a.A, a.colmap, a.q, a.x0, a.b, a.u, a.P, a.pa1, a.pa2, blockIdx.x - J.first[i]);
+}
+// the solves' starting states (zero but for the constants the recurrences
+// start from) and dx = 0
+__global__ void init_multi_kernel(Jobs<InitArgs> J) {
+  const int i = job_of(J, blockIdx.x);
+  const InitArgs &a = J.a[i];
+  const int j = (blockIdx.x - J.first[i]) * blockDim.x + threadIdx.x;
+  if (j < a.n) a.x[j] = 0.0;
+  if (blockIdx.x == J.first[i] && threadIdx.x < 2) {
+    LsqrState s;
+    memset(&s, 0, sizeof(s));
+    s.cs2 = -1.0;
+    s.atol = a.atol;
+    s.ctol = 1.0 / 1.0e8;
+    s.itnlim = a.itnlim;
+    a.st[threadIdx.x] = s;
+  }
+}
+__global__ void finish_multi_kernel(Jobs<FinArgs> J) {
+  const int i = job_of(J, blockIdx.x);
+  const FinArgs &a = J.a[i];
+  const int j = (blockIdx.x - J.first[i]) * blockDim.x + threadIdx.x;
+  if (j < a.n) a.x[j] = a.q[j] * a.dx[j] + a.x0[j];
+}
+
+// ---------------------------------------------------------------------------
+// iteration k, cell part.  Block 0 runs the scalar recurrences (stopping
+// tests of iteration k-2, rotation of iteration k-1) and stores the state;
+// blocks 1..G do u' = A v - alpha u and P = A^T u' for a contiguous range of
+// about ncell/G cells (G sized to the blocks resident on the device at once;
+// the ranges of one XCD adjacent, see xcd_logical), prefetching the
+// next cell's block into registers while computing on the current one.  The
+// cell blocks need only alpha_k = ||v_k||, which every block reduces from the
+// unknown kernel's partials in the same order; when block 0 finds that the
+// iteration stopped, the cell blocks' work of this launch is simply unused
+// (the unknown kernel that follows sees the stop and leaves x unchanged).
 template <typename T, int R, int L>
 __device__ __forceinline__ void cell_A(const int ncell, const int cgroup, const T *__restrict__ A,
                                        const int *__restrict__ colmap, const double *__restrict__ vp,
@@ -802,10 +851,16 @@ __global__ void finish_kernel(int n, const double *__restrict__ q, const double 
 }
 
 // launch helpers over the supported block shapes and gather widths
+R0Args r0_args(BlockSys &s, LsqrWork &wk, const double *b, const double *x) {
+  R0Args a;
+  a.A = s.A.p, a.q = s.q.p, a.x0 = x, a.b = b, a.colmap = s.colmap.p;
+  a.u = wk.u.p, a.P = wk.P.p, a.pa1 = wk.pa1.p, a.pa2 = wk.pa2.p;
+  a.ncell = s.ncell;
+  return a;
+}
 template <int R, int L>
 void launch_r0(BlockSys &s, LsqrWork &wk, const double *b, const double *x, hipStream_t st) {
-  hipLaunchKernelGGL((lsqr_r0<R, L>), dim3(s.ncell), dim3(64), 0, st, s.A.p, s.colmap.p, s.q.p, x, b, wk.u.p, wk.P.p,
-                     wk.pa1.p, wk.pa2.p);
+  hipLaunchKernelGGL((lsqr_r0<R, L>), dim3(s.ncell), dim3(64), 0, st, r0_args(s, wk, b, x));
 }
 // the cell kernel of a block type: one wavefront per workgroup on FP64
 // blocks (their LDS staging limits residency), kNW on FP32/FP16 blocks
@@ -1444,28 +1499,40 @@ void block_lsqr_multi(std::vector<LsqrJob> &jobs, int prec, hipStream_t st) {
     const int m = s.ncell * s.R, n = s.n;
     block_tune(s, prec, st);
     wk.ensure(m, n, s.ncell, s.L);
-    LsqrState init;
-    std::memset(&init, 0, sizeof(init));
-    init.cs2 = -1.0;
-    init.atol = (double)j.eps4;
-    init.ctol = 1.0 / 1.0e8;
-    init.itnlim = j.maxit == 0 ? 10 * (m + n + 50) : j.maxit;
-    itnlim_max = std::max(itnlim_max, init.itnlim);
-    LsqrState init2[2] = {init, init};
-    TT_HIP(hipMemcpyAsync(wk.state.p, init2, sizeof(init2), hipMemcpyHostToDevice, st));
-    TT_HIP(hipMemsetAsync(wk.x.p, 0, sizeof(double) * n, st));
-    r0(s, wk, j.b, j.x, st);
-    TT_LAUNCH_CHECK();
-    stepB(s, wk, 0, prec, st);
-    TT_LAUNCH_CHECK();
+    itnlim_max = std::max(itnlim_max, j.maxit == 0 ? 10 * (m + n + 50) : j.maxit);
     // the batch runs until its slowest system stops: the largest prediction
     const int p = s.hist[j.slot].predict();
     pred = (p < 0 || pred < 0) ? -1 : std::max(pred, p);
   }
-  int k = 1, batch = pred >= 0 ? pred + 4 : 8, extra = 0;
-  const int more = std::max(4, pred / 8);
   std::vector<LsqrJob *> act;
   for (LsqrJob &j : jobs) act.push_back(&j);
+  // starting states (as block_lsqr's), initial residuals, iteration 0's
+  // unknown kernels: one launch each for all systems
+  {
+    Jobs<InitArgs> I;
+    Jobs<R0Args> Z;
+    I.n = Z.n = (int)jobs.size();
+    I.first[0] = Z.first[0] = 0;
+    for (int i = 0; i < I.n; ++i) {
+      LsqrJob &j = jobs[i];
+      BlockSys &s = *j.s;
+      const int m = s.ncell * s.R, n = s.n;
+      I.a[i] = InitArgs{j.wk->state.p, j.wk->x.p, (double)j.eps4, n, j.maxit == 0 ? 10 * (m + n + 50) : j.maxit};
+      I.first[i + 1] = I.first[i] + (int)ceil_div(n, 256);
+      Z.a[i] = r0_args(s, *j.wk, j.b, j.x);
+      Z.first[i + 1] = Z.first[i] + s.ncell;
+    }
+    hipLaunchKernelGGL(init_multi_kernel, dim3(I.first[I.n]), dim3(256), 0, st, I);
+    if (R == 51)
+      hipLaunchKernelGGL((lsqr_r0_multi<51, 51>), dim3(Z.first[Z.n]), dim3(64), 0, st, Z);
+    else
+      hipLaunchKernelGGL((lsqr_r0_multi<27, 27>), dim3(Z.first[Z.n]), dim3(64), 0, st, Z);
+    TT_LAUNCH_CHECK();
+    stepB_multi(act, 0, prec, st);
+    TT_LAUNCH_CHECK();
+  }
+  int k = 1, batch = pred >= 0 ? pred + 4 : 8, extra = 0;
+  const int more = std::max(4, pred / 8);
   for (;;) {
     for (int b2 = 0; b2 < batch; ++b2) {
       stepA_multi(act, k, prec, st);
@@ -1478,6 +1545,9 @@ void block_lsqr_multi(std::vector<LsqrJob> &jobs, int prec, hipStream_t st) {
                             hipMemcpyDeviceToHost, st));
     TT_HIP(hipStreamSynchronize(st));
     std::vector<LsqrJob *> still;
+    Jobs<FinArgs> F;
+    F.n = 0;
+    F.first[0] = 0;
     for (LsqrJob *j : act) {
       const LsqrState &hs = *j->wk->host_state;
       if (!hs.done) {
@@ -1488,9 +1558,12 @@ void block_lsqr_multi(std::vector<LsqrJob> &jobs, int prec, hipStream_t st) {
       j->res.nit = hs.itn;
       j->res.istop = hs.istop;
       j->s->hist[j->slot].add(j->res.nit);
-      const int n = j->s->n;
-      hipLaunchKernelGGL(finish_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, j->s->q.p, j->wk->x.p, j->x,
-                         j->x);
+      F.a[F.n] = FinArgs{j->s->q.p, j->wk->x.p, j->x, j->x, j->s->n};
+      F.first[F.n + 1] = F.first[F.n] + (int)ceil_div(j->s->n, 256);
+      ++F.n;
+    }
+    if (F.n > 0) {
+      hipLaunchKernelGGL(finish_multi_kernel, dim3(F.first[F.n]), dim3(256), 0, st, F);
       TT_LAUNCH_CHECK();
     }
     act.swap(still);
