@@ -24,6 +24,11 @@ HIPFLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "
 # sources whose products and sums must round separately, as the reference's
 # (pivot searches and thresholds compare values with the CPU restatement)
 NO_CONTRACT = {"tt_compress.hip"}
+# sources whose products and sums fuse only within one expression, so that
+# the same expressions compile to the same roundings in every kernel that
+# holds them (the least-squares kernels: the persistent and the kernel-by-
+# kernel forms must give the same bits)
+CONTRACT_ON = {"tt_lsqr.hip"}
 # sources whose matrix-core loops keep their accumulators in VGPRs (without
 # the option the compiler copies loop-carried accumulators between VGPRs and
 # AGPRs around every product chain)
@@ -56,6 +61,8 @@ def build_library(force=True, jobs=8):
             flags = HIPFLAGS
             if os.path.basename(s) in NO_CONTRACT:
                 flags = [f for f in HIPFLAGS if not f.startswith("-ffp-contract")] + ["-ffp-contract=off"]
+            if os.path.basename(s) in CONTRACT_ON:
+                flags = [f for f in HIPFLAGS if not f.startswith("-ffp-contract")] + ["-ffp-contract=on"]
             if os.path.basename(s) in MFMA_VGPR:
                 flags = flags + ["-mllvm", "-amdgpu-mfma-vgpr-form"]
             cmd = [HIPCC, "-x", "hip"] + COMMON + flags + ["-c", s, "-o", o]

@@ -510,9 +510,17 @@ __device__ __forceinline__ void wave_sync() {
 // One cell of the reduced-precision cell kernel, its block and v already in
 // this wavefront's LDS: u' = A v - alpha u (stored, returned) and A^T u'
 // (stored to P).
+// v entry of column l, held by lane l, as a wavefront-uniform value
+__device__ __forceinline__ float lane_float(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// On 16-bit blocks the v entries come from the lanes' own vf (lane l: column
+// l) through the scalar unit rather than LDS broadcasts; on FP32 blocks from
+// vsf in LDS.
 template <typename T, int R, int L>
-__device__ __forceinline__ double aw_compute(int t, int c, const T *blk, const float *vsf, float *uuf, double uo,
-                                             double alpha, double beta, double *__restrict__ u,
+__device__ __forceinline__ double aw_compute(int t, int c, const T *blk, const float *vsf, float vf, float *uuf,
+                                             double uo, double alpha, double beta, double *__restrict__ u,
                                              double *__restrict__ P) {
   double un = 0.0;
   {
@@ -522,15 +530,14 @@ __device__ __forceinline__ double aw_compute(int t, int c, const T *blk, const f
       constexpr int LP = row_len<T, L>();
       const uint2 *rp = reinterpret_cast<const uint2 *>(blk + row * LP);
       float f0 = 0.f, f1 = 0.f;
-#pragma unroll 7  // a full unroll holds every LDS read in flight and spills
+#pragma unroll
       for (int q = 0; q < LP / 4; ++q) {
         const uint2 cw = rp[q];
-        const float4 va = *reinterpret_cast<const float4 *>(vsf + 4 * q);
         const _Float16 *h = reinterpret_cast<const _Float16 *>(&cw);
-        f0 += (float)h[0] * va.x;
-        f1 += (float)h[1] * va.y;
-        f0 += (float)h[2] * va.z;
-        f1 += (float)h[3] * va.w;
+        f0 += (float)h[0] * lane_float(vf, 4 * q);
+        f1 += (float)h[1] * lane_float(vf, 4 * q + 1);
+        f0 += (float)h[2] * lane_float(vf, 4 * q + 2);
+        f1 += (float)h[3] * lane_float(vf, 4 * q + 3);
       }
       s = (double)f0 + (double)f1;
     } else {
@@ -675,7 +682,8 @@ __device__ __forceinline__ void cell_Aw(const int ncell, const int cgroup, const
       double acc = 0.0;
       while (c >= 0 && c < gend(g)) {
         rb.store(blk);
-        vsf[t] = (float)(t < L ? (alpha > 0.0 ? vj / alpha : vj) : 0.0);
+        const float vf = (float)(t < L ? (alpha > 0.0 ? vj / alpha : vj) : 0.0);
+        if constexpr (sizeof(T) != 2) vsf[t] = vf;
         wave_sync();
         const int cnn = cn >= 0 ? next_cell(cn) : -1;
         double vjn = 0.0, uon = 0.0;
@@ -686,7 +694,7 @@ __device__ __forceinline__ void cell_Aw(const int ncell, const int cgroup, const
           rb.load(A + (size_t)cn * S);
           if (cnn >= 0 && t < L) jnn = colmap[(size_t)cnn * L + t];
         }
-        const double un = aw_compute<T, R, L>(t, c, blk, vsf, uuf, uo, alpha, beta, u, P);
+        const double un = aw_compute<T, R, L>(t, c, blk, vsf, vf, uuf, uo, alpha, beta, u, P);
         acc += un * un;
         c = cn, cn = cnn, vj = vjn, jn = jnn, uo = uon;
         wave_sync();  // blk, vsf and uuf are rewritten next
