@@ -24,9 +24,10 @@ struct DenseKin {
   DBuf<int> kode;             // nb x 3
   DBuf<int> isbnd;            // nn: boundary index + 1 or 0
   // Reduced-precision form: the ten matrices (order H, Htx, Hty, Htz, Dx, Dy,
-  // Dz, Ainv x, y, z) stored with leading dimension ldl (nb rounded up to 8)
-  // and a power-of-two scale per row (10 x ldl floats); products are taken on
-  // the change of the fields since an FP64 anchor step.
+  // Dz, Ainv x, y, z) stored in row blocks of 256 rows (each block column
+  // after column, so a product workgroup streams contiguous memory; ldl = nb
+  // rounded up to 256) and a power-of-two scale per row (10 x ldl floats);
+  // products are taken on the change of the fields since an FP64 anchor step.
   SdPrec prec = SdPrec::F64;
   int ldl = 0;
   DBuf<uint16_t> lo16[10];

@@ -263,6 +263,15 @@ struct Load4<Lo::BF16> {
   }
 };
 
+constexpr int kLoRows = 4;                    // rows per thread of the product kernels
+constexpr int kLoBlockRows = 64 * kLoRows;    // rows per workgroup (and per stored row block)
+
+// offset of (row i0, column j) in a copy with nc columns (i0 a multiple of
+// kLoRows: the thread's four rows follow)
+__device__ __forceinline__ size_t lo_at(int i0, size_t j, int nc) {
+  return ((size_t)(i0 / kLoBlockRows) * nc + j) * kLoBlockRows + i0 % kLoBlockRows;
+}
+
 // largest |M(i, j)| over a chunk of columns, per row
 __global__ void rowmax_kernel(int nr, int nc, int ks_cnt, const double *__restrict__ M, float *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, ks = blockIdx.y;
@@ -281,13 +290,18 @@ __global__ void rowscale_kernel(int nr, int ks_cnt, const float *__restrict__ pa
   if (m > 0.f) frexpf(m, &e);  // m = f * 2^e, f in [0.5, 1)
   scale[i] = ldexpf(1.f, e);   // stored = value / scale
 }
+// Reduced-precision copies are stored in row blocks of kLoBlockRows rows:
+// block b holds rows b*kLoBlockRows.. of every column, column after column,
+// so that a workgroup of the product kernels (one row block, a range of
+// columns) streams one contiguous piece of memory.
 template <Lo L>
 __global__ void to_low_kernel(int nr, int nc, int ldl, const double *__restrict__ M, const float *__restrict__ scale,
                               typename Load4<L>::T *__restrict__ out) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (size_t)ldl * nc) return;
-  const int i = (int)(k % ldl);
-  const size_t j = k / ldl;
+  const size_t bsz = (size_t)kLoBlockRows * nc;
+  const int i = (int)(k / bsz) * kLoBlockRows + (int)(k % kLoBlockRows);
+  const size_t j = k % bsz / kLoBlockRows;
   const float v = i < nr ? (float)(M[i + (size_t)nr * j] / (double)scale[i]) : 0.f;
   if constexpr (L == Lo::F32)
     out[k] = v;
@@ -309,8 +323,6 @@ __global__ void delta4_kernel(int n, int nn, const int *__restrict__ gather, con
                        isbnd ? (isbnd[g] ? 0.f : 1.f) : 0.f);
 }
 
-constexpr int kLoRows = 4;                    // rows per thread
-constexpr int kLoBlockRows = 64 * kLoRows;    // rows per block
 
 // delta of [H Htx Hty Htz] products: sum_j Hq(i,j) * dv(j, c), dv = the change of
 // the velocity at boundary node gbn(j) (delta4_kernel); four columns per step,
@@ -352,7 +364,7 @@ __global__ void __launch_bounds__(64) lprod_h_kernel(int nb, int ldl, int ks_cnt
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       d[u] = dv[j + u];
-      const size_t o = (size_t)ldl * (j + u) + i0;
+      const size_t o = lo_at(i0, j + u, nb);
 #pragma unroll
       for (int q = 0; q < 4; ++q) m[u][q] = Load4<L>::at(Hq[q] + o);
     }
@@ -361,7 +373,7 @@ __global__ void __launch_bounds__(64) lprod_h_kernel(int nb, int ldl, int ks_cnt
   }
   for (; j < j1; ++j) {
     float4 m[4];
-    const size_t o = (size_t)ldl * j + i0;
+    const size_t o = lo_at(i0, j, nb);
 #pragma unroll
     for (int q = 0; q < 4; ++q) m[q] = Load4<L>::at(Hq[q] + o);
     step(dv[j], m);
@@ -422,7 +434,7 @@ __global__ void __launch_bounds__(64) lprod_d_kernel(int nb, int nn, int ldl, in
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       w[u] = dw[k + u];
-      const size_t o = (size_t)ldl * (k + u) + i0;
+      const size_t o = lo_at(i0, k + u, nn);
 #pragma unroll
       for (int q = 0; q < 3; ++q) m[u][q] = Load4<L>::at(Dq[q] + o);
     }
@@ -431,7 +443,7 @@ __global__ void __launch_bounds__(64) lprod_d_kernel(int nb, int nn, int ldl, in
   }
   for (; k < k1; ++k) {
     float4 m[3];
-    const size_t o = (size_t)ldl * k + i0;
+    const size_t o = lo_at(i0, k, nn);
 #pragma unroll
     for (int q = 0; q < 3; ++q) m[q] = Load4<L>::at(Dq[q] + o);
     const float4 w = dw[k];
@@ -465,7 +477,7 @@ __global__ void __launch_bounds__(64) lgemv_kernel(int nb, int ldl, int ks_cnt, 
 #pragma unroll 8
   for (int j = j0; j < j1; ++j) {
     const float d = (float)(x_a ? x[j] - x_a[j] : x[j]);
-    const float4 m = Load4<L>::at(M + (size_t)ldl * j + i0);
+    const float4 m = Load4<L>::at(M + lo_at(i0, j, nb));
     a[0] += m.x * d, a[1] += m.y * d, a[2] += m.z * d, a[3] += m.w * d;
   }
 #pragma unroll
@@ -475,7 +487,7 @@ __global__ void __launch_bounds__(64) lgemv_kernel(int nb, int ldl, int ks_cnt, 
 
 // partials of sum_j D_lo(i, gbn(j)) * dw(j) for one or two D matrices
 template <Lo L>
-__global__ void __launch_bounds__(64) ldcorr_kernel(int nb, int ldl, int ks_cnt, int nm,
+__global__ void __launch_bounds__(64) ldcorr_kernel(int nb, int nn, int ks_cnt, int nm,
                                                     const typename Load4<L>::T *__restrict__ D0,
                                                     const typename Load4<L>::T *__restrict__ D1,
                                                     const float *__restrict__ s0, const float *__restrict__ s1,
@@ -490,7 +502,7 @@ __global__ void __launch_bounds__(64) ldcorr_kernel(int nb, int ldl, int ks_cnt,
 #pragma unroll 8
   for (int j = j0; j < j1; ++j) {
     const float d = (float)dw[j];
-    const size_t col = (size_t)(gbn[j] - 1) * ldl + i0;
+    const size_t col = lo_at(i0, (size_t)(gbn[j] - 1), nn);
     const float4 m = Load4<L>::at(D0 + col);
     a0[0] += m.x * d, a0[1] += m.y * d, a0[2] += m.z * d, a0[3] += m.w * d;
     if (nm > 1) {
@@ -695,7 +707,7 @@ void step_delta(DenseKin &d, const DevMesh &m, const double *vel, double *vort, 
     TT_LAUNCH_CHECK();
     if (e < 2) {
       const int nm = 2 - e;
-      hipLaunchKernelGGL((ldcorr_kernel<L>), dim3(rb, ksh), dim3(64), 0, s, nb, ldl, ksh, nm, lo(4 + e + 1),
+      hipLaunchKernelGGL((ldcorr_kernel<L>), dim3(rb, ksh), dim3(64), 0, s, nb, nn, ksh, nm, lo(4 + e + 1),
                          nm > 1 ? lo(4 + e + 2) : lo(4 + e + 1), sc + (size_t)(4 + e + 1) * ldl,
                          sc + (size_t)(4 + (nm > 1 ? e + 2 : e + 1)) * ldl, m.gbn.p, (const double *)d.dw.p, d.part.p);
       hipLaunchKernelGGL(add_slots_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nm, ksh, d.part.p,
@@ -732,7 +744,7 @@ void dense_set_precision(DenseKin &d, SdPrec prec, hipStream_t s) {
   d.prec = prec;
   if (prec == SdPrec::F64) return;
   const int nb = d.nb, nn = d.nn;
-  d.ldl = (nb + 7) / 8 * 8;
+  d.ldl = (nb + kLoBlockRows - 1) / kLoBlockRows * kLoBlockRows;  // whole row blocks
   d.rscale.alloc((size_t)10 * d.ldl);
   const SdDev &M = d.mats;
   const double *mats[10] = {M.H.p, M.Htx.p, M.Hty.p, M.Htz.p, M.Dx.p, M.Dy.p, M.Dz.p, d.Ainv[0].p, d.Ainv[1].p,
