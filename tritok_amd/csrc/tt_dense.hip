@@ -164,7 +164,16 @@ __global__ void __launch_bounds__(kRowsPerBlock) gemv_kernel(int nb, int ks_cnt,
   const int chunk = (nb + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(nb, j0 + chunk);
   if (i >= nb) return;
   double acc = 0.0;
-  for (int j = j0; j < j1; ++j) acc += M[i + (size_t)nb * j] * x[j];
+  int j = j0;
+  // eight columns' loads issued together, added in column order
+  for (; j + 8 <= j1; j += 8) {
+    double m[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m[u] = M[i + (size_t)nb * (j + u)], xv[u] = x[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += m[u] * xv[u];
+  }
+  for (; j < j1; ++j) acc += M[i + (size_t)nb * j] * x[j];
   part[(size_t)ks * nb + i] = acc;
 }
 
