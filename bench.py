@@ -351,6 +351,70 @@ def aca_kinematics_leg(n, leaf=64, eta=1.0, eps=1e-6, reps=5):
     return out
 
 
+def reference_leg(n=8, nnlit=10):
+    """The reference itself (oracle/_ref/Tritok, compiled from its own sources)
+    on the CPU-runnable configuration: the n^3 lid-driven cavity, Re = 100,
+    URDW 0.1, from rest, nnlit Picard iterations on one host core (one MPI
+    rank).  Its integral caches are written from this library's device
+    integrals, so the reference's set-up reads them instead of integrating
+    (about a minute on one core); the rate is its own CPU time of the Picard
+    loop (tri.log).  The GPU runs the same case in FP64 and mixed precision,
+    and its per-iteration changes are compared with the reference's tri.sta."""
+    import shutil
+    import tempfile
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import cpu_sample  # the reference binary: CPU baseline leg only
+    from tritok_amd import files as F
+    from tritok_amd import integrals as I
+    from tritok_amd import workloads
+    from tritok_amd.case import Case
+    host = workloads.cavity(n, re=100.0, urdw=0.1, nnlit=nnlit)
+    view = host.view()
+    m = view.mesh
+    q = I.quad(4, 4, 4, 4, 2)
+    work = tempfile.mkdtemp(prefix="tt_reference_")
+    try:
+        mac = I.macro_integrals(view, q)
+        F.write_macro_int(os.path.join(work, "tri.macro.int"), {**{k: mac[k] for k in F.MACRO}, "ont": mac["ont"]},
+                          m.nnodes, m.nbnodes)
+        del mac
+        sd = I.sd_integrals(view, q)
+        F.write_sdkm_int(os.path.join(work, "tri.sdkm.int.1"), {**{k: sd[k] for k in F.SDKM}, "ont": sd["ont"]},
+                         m.nnodes, m.nbnodes)
+        del sd
+        host.write_geo(os.path.join(work, "case.geo"))
+        log("reference: %d^3 cavity, %d Picard iterations on one core ..." % (n, nnlit))
+        secs, sta_ref = cpu_sample.reference_run(work, None, workloads.bic_cavity(),
+                                                 workloads.inp_text(re=100.0, urdw=0.1, nnlit=nnlit))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    out = {"case": "lid-driven cavity %d^3 cells (%d nodes), Re = 100, URDW 0.1, from rest, %d Picard iterations"
+                   % (n, m.nnodes, len(sta_ref)),
+           "reference": {"binary": "oracle/_ref/Tritok (amdflang -O2, MPICH, 1 rank)", "cores": 1,
+                         "picard_loop_s": secs, "iters_per_s": len(sta_ref) / secs}}
+    for prec in ("fp64", "fp16"):
+        with Case(host, prec=prec, quad=q) as c:
+            sta = []
+            for _ in range(2):
+                rc, err, _nit = c.step()
+                sta.append(err)
+            t0 = time.perf_counter()
+            k = 0
+            for _ in range(len(sta_ref) - 2):
+                rc, err, _nit = c.step()
+                sta.append(err)
+                k += 1
+            dt = time.perf_counter() - t0
+        sta = np.array(sta)[:len(sta_ref)]
+        cols = [j for j in range(6) if np.abs(sta_ref[:, j]).max() > 0]
+        rel = float(max(np.max(np.abs(sta[:, j] - sta_ref[:, j]) / np.abs(sta_ref[:, j])) for j in cols))
+        out[prec] = {"iters_per_s": k / dt, "over_reference": (k / dt) / out["reference"]["iters_per_s"],
+                     "max_rel_diff_per_iteration_changes": rel}
+    log("reference: %.2f Picard its/s on one core; GPU fp64 %.1f, mixed %.1f its/s" % (
+        out["reference"]["iters_per_s"], out["fp64"]["iters_per_s"], out["fp16"]["iters_per_s"]))
+    return out
+
+
 def mhsd_leg(n, f=7, tstep=0.1, nstep=2, nnlit=3):
     """The single-domain modified-Helmholtz path (EQNS imH 3, 5, 6 of the
     reference) on the n^3 cube: u prescribed on walls 1, 3, 5, the normal flux
@@ -492,6 +556,8 @@ def main():
                     help="reduced precision of the mixed leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--reference-n", type=int, default=8,
+                    help="n: time the reference binary on the n^3 cavity beside the GPU (0: skip)")
     ap.add_argument("--accuracy-n", type=int, default=20,
                     help="cube size of the converged mixed-vs-FP64 comparison (0: skip)")
     ap.add_argument("--no-kinematics", action="store_true",
@@ -632,6 +698,9 @@ def main():
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import cpu_sample  # the oracle: CPU baseline leg only
         cpu = cpu_sample.picard_rate(host, case, legs["fp64"]["nits"], budget_s=args.cpu_seconds, log=log)
+    ref_cpu = None
+    if not args.no_cpu and args.reference_n > 0:
+        ref_cpu = reference_leg(args.reference_n)
 
     case.close()  # free the 20^3 case before the other legs
     acc = accuracy_leg(args.accuracy_n, args.mixed, args.anchor_period) if args.accuracy_n > 0 else None
@@ -670,7 +739,7 @@ def main():
                           ("boundary_vorticity", "km", "temperature", "vte", "convergence", "step_total")},
         "lsqr_iters_per_step": {"fp64": (legs["fp64"]["nits"].sum(0)[3:10] / args.steps).tolist(),
                                 "mixed": (nit[3:10] / args.steps).tolist()},
-        "setup_s": setup_s, "device": dev, "roofline": roofline, "cpu_baseline": cpu,
+        "setup_s": setup_s, "device": dev, "roofline": roofline, "cpu_baseline": cpu, "reference_cpu": ref_cpu,
         "p99_solve_latency_s": serving[args.mixed]["latency_s"]["p99"] if serving else None,
         "serving": serving, "batched_integrals": batched, "batched_solves": bsolves, "mixed_kinematics": mixkin,
         "aca_kinematics": acakin, "single_domain_mh": mh_sd,

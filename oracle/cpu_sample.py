@@ -139,3 +139,49 @@ def picard_rate(host, case, nits, budget_s=20.0, log=print):
                        "mean LSQR counts per step (%.0f kinematics, %.0f vorticity, %.0f energy); %.1f s of CPU "
                        "work" % (rows, nb, cores, km_its, vte_its, t_its, sample_s)),
             "seconds_per_iteration": per_iter}
+
+
+REFBIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "Tritok")
+
+
+def _ref_solve_seconds(log_path):
+    """The 'solve' line of the reference's CPU-time table (tri.log): the
+    seconds spent in its Picard loop (main.f90's timing, lib_Time.f90)."""
+    with open(log_path) as f:
+        for line in f:
+            p = [x.strip() for x in line.split("|")]
+            if len(p) >= 3 and p[1] == "solve":
+                return float(p[2])
+    raise RuntimeError("no solve time in " + log_path)
+
+
+def _ref_sta(path):
+    """Rows of the reference's tri.sta (step, iteration, nine relative changes)."""
+    rows = []
+    with open(path) as f:
+        for line in f:
+            p = line.split()
+            if len(p) >= 11 and p[0].isdigit() and p[1].isdigit():
+                rows.append([float(x) for x in p[2:11]])
+    return np.array(rows)
+
+
+def reference_run(workdir, geo_text, bic_text, inp_text, timeout_s=120.0):
+    """Runs the reference binary (compiled from the reference's own sources by
+    oracle/Makefile.ref) in workdir, which must hold the integral caches
+    tri.macro.int and tri.sdkm.int.1 (so that its set-up reads them instead of
+    integrating); returns its Picard-loop seconds and its tri.sta rows.
+    One MPI rank, serial Fortran: one host core."""
+    import subprocess
+    if not os.path.exists(REFBIN):
+        raise RuntimeError("reference binary not built (oracle/_ref/Tritok)")
+    if geo_text is not None:
+        with open(os.path.join(workdir, "case.geo"), "w") as f:
+            f.write(geo_text)
+    with open(os.path.join(workdir, "case.bic"), "w") as f:
+        f.write(bic_text)
+    with open(os.path.join(workdir, "tri.inp"), "w") as f:
+        f.write(inp_text.replace("MDIR ./\n", "MDIR ./\nLGEO case.geo\nLBIC case.bic\n"))
+    subprocess.run([REFBIN], cwd=workdir, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                   timeout=timeout_s)
+    return _ref_solve_seconds(os.path.join(workdir, "tri.log")), _ref_sta(os.path.join(workdir, "tri.sta"))

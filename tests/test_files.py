@@ -113,3 +113,23 @@ def test_g10_5_edge_values(tmp_path):
     line = (tmp_path / "t").read_text().splitlines()[2]
     items = [line[15 + 12 * k:15 + 12 * k + 10] for k in range(9)]
     assert items[3:] == ["12345.    ", "99999.    ", ".10000E+06", ".15000E-04", ".32000E+06", ".10000-119"]
+
+
+def test_reference_log_and_history_parsers(tmp_path):
+    """The CPU-baseline leg reads the reference's Picard-loop CPU time from the
+    table in tri.log and its per-iteration changes from tri.sta."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import cpu_sample
+    (tmp_path / "tri.log").write_text(
+        "   CPU time information [s]\n|-------------------------------|-------------|\n"
+        "| z Dw solve                    |       1.002 |\n| solve                         |       4.044 |\n"
+        "| TOTAL                         |       8.390 |\n")
+    assert cpu_sample._ref_solve_seconds(str(tmp_path / "tri.log")) == 4.044
+    (tmp_path / "tri.sta").write_text(
+        "TriTok v8.0, Februar 2025\n tstep  tsite          vx          vy\n"
+        "     1      1  .10000E+01  .20000E-01  .30000E-02  .40000E-03  .50000E-04  .60000E-05  0.0000      0.0000      0.0000    \n"
+        "     1      2  .11000E+01  .21000E-01  .31000E-02  .41000E-03  .51000E-04  .61000E-05  0.0000      0.0000      0.0000    \n")
+    sta = cpu_sample._ref_sta(str(tmp_path / "tri.sta"))
+    assert sta.shape == (2, 9) and sta[1, 0] == 1.1 and sta[0, 5] == 6e-6
