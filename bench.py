@@ -269,6 +269,33 @@ def batched_solves_leg(count, seed=20251203):
                                   "frac": gbs / HBM_PEAK_GBS,
                                   "bytes": "per system iteration 2*((%d+4)*nnz + 4*(m+1)) + 8*(3m + 6n)" % vb}}
     B.close()
+    # the same systems on the cell-block format (groups of up to 32 systems
+    # per launch of each LSQR kernel)
+    t0 = time.perf_counter()
+    KB = S.BlockBatch(systems, rows_per_cell=51)
+    out["blocks"] = {"setup_s": time.perf_counter() - t0,
+                     "form": "per-cell 51 x 51 blocks of the scaled matrix, fused A v / A^T u' cell kernel, groups "
+                             "of up to 32 systems of similar size per launch"}
+    cells = m / 51.0
+    for prec, vb in (("fp64", 8), ("fp32", 4), ("fp16", 2)):
+        KB.set_precision(prec)
+        KB.solve(bs, x0, 5000, 1e-7)  # warm-up: tuning and the solve histories that size the launch batches
+        xs, istop, nit, ms = KB.solve(bs, x0, 5000, 1e-7)
+        alg = float(np.sum(nit * (cells * (51 * 51 * vb + 51 * 4 + 51 * 8 + 51 * 4) + 16 * m + 8 * 6 * nunk)))
+        gbs = alg / (ms * 1e-3) / 1e9
+        err = max(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)) for a, b in zip(xs, x64))
+        log("batched solves on blocks %s: %.1f ms, %d..%d iterations, %.0f GB/s algorithmic, max rel diff vs CSR "
+            "FP64 %.2e" % (prec, ms, nit.min(), nit.max(), gbs, err))
+        out["blocks"][prec] = {
+            "kernel_ms": ms, "systems_per_s": count / (ms * 1e-3),
+            "iterations": {"min": int(nit.min()), "mean": float(nit.mean()), "max": int(nit.max())},
+            "istop": {int(k): int(v) for k, v in zip(*np.unique(istop, return_counts=True))},
+            "max_rel_l2_vs_csr_fp64": err,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS,
+                         "bytes": "per system iteration cells*(51*51*%d (blocks) + 51*16 (column map, v, A^T u' "
+                                  "partials)) + 16m + 48n" % vb}}
+    KB.close()
     return out
 
 

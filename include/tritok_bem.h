@@ -166,6 +166,34 @@ int tt_lsqr_batch_tune(tt_lsqr_batch *h, const double *b, const double *x, int32
                        double *best_scale, double ms[4]);
 void tt_lsqr_batch_free(tt_lsqr_batch *h);
 
+/* Many independent subdomain systems solved on the cell-block format (no
+ * reference counterpart for the batching; per system prelsqr2 + slvlsqr2,
+ * lib_Solve_v11.f:4401-4535).  The systems are given as for
+ * tt_lsqr_batch_setup (1-based CSR, one after another), with rows in groups
+ * of rows_per_cell (51: vorticity/energy systems, 27: kinematics systems)
+ * whose entries lie in at most rows_per_cell distinct columns per group --
+ * the structure of the reference's subdomain systems; anything else is
+ * rejected.  Each system is stored as per-cell dense blocks of its scaled
+ * matrix and solved with the fused cell-block LSQR; systems are solved in
+ * groups of up to 32 of similar size, every iteration of a group one launch
+ * of each LSQR kernel for all its systems, each system with its own scalars
+ * and stop (its iterates do not depend on the group).  A solve takes b (sum
+ * of m) and the hot start x (sum of n, overwritten by the solutions), the
+ * reference's maxit and REAL(4) tolerance, and returns each system's
+ * iteration count and istop; *ms (may be NULL) = device time of the solve. */
+typedef struct tt_block_batch tt_block_batch;
+int tt_block_batch_setup(int32_t nsys, const int32_t *m, const int32_t *n, const int64_t *nnz,
+                         const int32_t *ia, const int32_t *ja, const double *a, int32_t rows_per_cell,
+                         tt_block_batch **out);
+/* Blocks the iterations read (no reference counterpart): TT_PREC_FP64
+ * (default), TT_PREC_FP32 or TT_PREC_FP16 copies of the scaled blocks with
+ * FP32 products; the hot-start residual, vectors, scalars and stopping tests
+ * stay FP64. */
+int tt_block_batch_set_precision(tt_block_batch *h, int32_t prec);
+int tt_block_batch_solve(tt_block_batch *h, const double *b, double *x, int32_t maxit, float eps4, int32_t *nit,
+                         int32_t *istop, double *ms);
+void tt_block_batch_free(tt_block_batch *h);
+
 /* Replaces CRSxV (lib_util.f:1351-1372): y = A x on the device for a CSR
  * matrix given by host arrays (m rows, n columns). */
 int tt_csr_mv(int32_t m, int32_t n, int32_t nnz, const int32_t *ia, const int32_t *ja,

@@ -313,3 +313,47 @@ def test_lsqr_batch_gang_width_tuner_keeps_oracle_parity():
         assert istop[k] == istop_ref and abs(nit[k] - nit_ref) <= 2
         assert rel_l2(xs[k], x_ref) < 1e-8
     B.close()
+
+
+def test_block_batch_matches_oracle_per_system():
+    """Subdomain systems of different meshes and components solved on the
+    cell-block format in groups: each agrees with the oracle's slvlsqr2 as the
+    single-system solvers do, and repeats bit for bit; FP32 and FP16 blocks
+    stay close to the FP64 solutions."""
+    from tritok_amd import solvers as S
+    systems = [_vte_system(load_golden("dc2")), _vte_system(load_golden("jit2"), e=1),
+               _vte_system(load_golden("dc2"), e=2), _vte_system(load_golden("nc2"), e=0)]
+    # more systems than one launch group holds (16): the groups are formed by size
+    systems = systems * 5
+    rng = np.random.default_rng(11)
+    bs = [rng.standard_normal(len(s[0]) - 1) for s in systems]
+    x0s = [0.1 * rng.standard_normal(s[3]) for s in systems]
+    B = S.BlockBatch(systems, rows_per_cell=51)
+    for eps in (1e-7, 1e-10):
+        xs, istop, nit, ms = B.solve(bs, x0s, 5000, eps)
+        print("block batch eps", eps, "device ms", ms, flush=True)
+        for k, (ia, ja, a, n) in enumerate(systems[:4]):
+            m = len(ia) - 1
+            ao = np.array(a, np.float64)
+            q = O.prelsqr2(m, n, ia, ja, ao)
+            x_ref, istop_ref, nit_ref = O.slvlsqr2(m, n, ia, ja, ao, q, bs[k], x0s[k], 5000, eps)
+            print(k, "nit", nit[k], nit_ref, "istop", istop[k], istop_ref, "rel", rel_l2(xs[k], x_ref), flush=True)
+            assert istop[k] == istop_ref and abs(nit[k] - nit_ref) <= 2
+            assert rel_l2(xs[k], x_ref) < (1e-8 if eps < 1e-9 else 1e-5)
+    xs2, _, nit2, _ = B.solve(bs, x0s, 5000, 1e-10)
+    assert all(np.array_equal(u, w) for u, w in zip(xs, xs2)) and np.array_equal(nit, nit2)
+    for prec, tol in (("fp32", 1e-4), ("fp16", 3e-2)):
+        B.set_precision(prec)
+        xp, istop_p, nit_p, ms = B.solve(bs, x0s, 5000, 1e-10)
+        err = max(rel_l2(u, w) for u, w in zip(xp, xs))
+        print(prec, "device ms", ms, "max rel vs FP64", err, "nit", nit_p.min(), nit_p.max(), flush=True)
+        assert np.all(istop_p > 0) and err < tol
+    B.close()
+
+
+def test_block_batch_rejects_systems_without_cell_structure():
+    from tritok_amd import solvers as S
+    from tritok_amd._lib import TTError
+    ia, ja, a = _random_csr(510, 300, 9, 5)  # rows spread over 300 columns: no 51-row cell structure
+    with pytest.raises(TTError):
+        S.BlockBatch([(ia, ja, a, 300)], rows_per_cell=51)

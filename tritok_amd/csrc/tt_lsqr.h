@@ -151,7 +151,7 @@ struct LsqrJob {
   LsqrResult res;
 };
 
-// block_lsqr on up to 16 systems of one block shape at once, all on stream
+// block_lsqr on up to 32 systems of one block shape at once, all on stream
 // st: every iteration runs the cell kernels of the systems still iterating in
 // one launch and their unknown kernels in one launch per gather width.  The
 // kernel bodies, grids per system and reductions are those of block_lsqr, so

@@ -218,7 +218,7 @@ struct UnkArgs {
   LsqrState *st;
 };
 // systems of one batched launch: system i runs on workgroups [first[i], first[i+1])
-constexpr int kMaxJobs = 16;
+constexpr int kMaxJobs = 32;
 template <typename Args>
 struct Jobs {
   Args a[kMaxJobs];

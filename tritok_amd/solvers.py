@@ -97,6 +97,57 @@ _btune = _lib._sig("tt_lsqr_batch_tune", i32, P, PD, PD, i32, C.c_float, PD, PD)
 _bfree = _lib._sig("tt_lsqr_batch_free", None, P)
 
 
+_kbsetup = _lib._sig("tt_block_batch_setup", i32, i32, PI, PI, C.POINTER(C.c_int64), PI, PI, PD, i32, C.POINTER(P))
+_kbprec = _lib._sig("tt_block_batch_set_precision", i32, P, i32)
+_kbsolve = _lib._sig("tt_block_batch_solve", i32, P, PD, PD, i32, C.c_float, PI, PI, C.POINTER(C.c_double))
+_kbfree = _lib._sig("tt_block_batch_free", None, P)
+
+
+class BlockBatch:
+    """Independent subdomain systems [(ia, ja, a, n), ...] (1-based CSR, rows
+    in groups of rows_per_cell sharing their columns) solved on the cell-block
+    format, in groups of up to 32 systems per launch."""
+
+    def __init__(self, systems, rows_per_cell=51):
+        self.m = np.array([len(s[0]) - 1 for s in systems], np.int32)
+        self.n = np.array([int(s[3]) for s in systems], np.int32)
+        self.nnz = np.array([len(s[1]) for s in systems], np.int64)
+        ia = np.concatenate([_i32(s[0]) for s in systems])
+        ja = np.concatenate([_i32(s[1]) for s in systems])
+        a = np.concatenate([np.asarray(s[2], np.float64) for s in systems])
+        h = P()
+        check(_kbsetup(len(systems), self.m.ctypes.data_as(PI), self.n.ctypes.data_as(PI),
+                       self.nnz.ctypes.data_as(C.POINTER(C.c_int64)), ia.ctypes.data_as(PI), ja.ctypes.data_as(PI),
+                       a.ctypes.data_as(PD), int(rows_per_cell), C.byref(h)), "tt_block_batch_setup")
+        self._h = h
+        self.noff = np.concatenate([[0], np.cumsum(self.n)])
+
+    def set_precision(self, prec):
+        """Blocks the iterations read: "fp64", "fp32" or "fp16"."""
+        check(_kbprec(self._h, SD_PREC[prec]), "tt_block_batch_set_precision")
+
+    def solve(self, b, x0, maxit, eps4):
+        """b, x0: lists (or concatenated arrays).  Returns (xs, istop, nit, ms)."""
+        b = _f64(np.concatenate(b) if isinstance(b, (list, tuple)) else b)
+        x = np.array(np.concatenate(x0) if isinstance(x0, (list, tuple)) else x0, dtype=np.float64)
+        k = len(self.m)
+        nit, istop, ms = np.zeros(k, np.int32), np.zeros(k, np.int32), C.c_double()
+        check(_kbsolve(self._h, b.ctypes.data_as(PD), x.ctypes.data_as(PD), int(maxit), float(eps4),
+                       nit.ctypes.data_as(PI), istop.ctypes.data_as(PI), C.byref(ms)), "tt_block_batch_solve")
+        return [x[self.noff[i]:self.noff[i + 1]] for i in range(k)], istop, nit, ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _kbfree(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class LsqrBatch:
     """Independent CSR systems [(ia, ja, a, n), ...] (1-based) solved together
     in one launch; prelsqr2 of each at construction (self.q, and the scaled
