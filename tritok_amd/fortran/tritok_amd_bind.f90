@@ -361,6 +361,38 @@ MODULE tritok_amd_bind
       TYPE(C_PTR), VALUE :: h
     END SUBROUTINE
 
+    ! the same systems on per-cell dense blocks (rows in groups of 51 or 27 per cell)
+    FUNCTION tt_block_batch_setup(nsys, m, n, nnz, ia, ja, a, rows_per_cell, h) &
+        BIND(C, NAME="tt_block_batch_setup")
+      IMPORT :: C_INT, C_INT32_T, C_INT64_T, C_DOUBLE, C_PTR
+      INTEGER(C_INT32_T), VALUE :: nsys, rows_per_cell
+      INTEGER(C_INT32_T), INTENT(IN) :: m(*), n(*), ia(*), ja(*)
+      INTEGER(C_INT64_T), INTENT(IN) :: nnz(*)
+      REAL(C_DOUBLE), INTENT(IN) :: a(*)
+      TYPE(C_PTR) :: h
+      INTEGER(C_INT) :: tt_block_batch_setup
+    END FUNCTION
+    FUNCTION tt_block_batch_set_precision(h, prec) BIND(C, NAME="tt_block_batch_set_precision")
+      IMPORT :: C_INT, C_INT32_T, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      INTEGER(C_INT32_T), VALUE :: prec
+      INTEGER(C_INT) :: tt_block_batch_set_precision
+    END FUNCTION
+    FUNCTION tt_block_batch_solve(h, b, x, maxit, eps4, nit, istop, ms) BIND(C, NAME="tt_block_batch_solve")
+      IMPORT :: C_INT, C_INT32_T, C_DOUBLE, C_FLOAT, C_PTR
+      TYPE(C_PTR), VALUE :: h
+      REAL(C_DOUBLE), INTENT(IN) :: b(*)
+      REAL(C_DOUBLE) :: x(*), ms
+      INTEGER(C_INT32_T), VALUE :: maxit
+      REAL(C_FLOAT), VALUE :: eps4
+      INTEGER(C_INT32_T) :: nit(*), istop(*)
+      INTEGER(C_INT) :: tt_block_batch_solve
+    END FUNCTION
+    SUBROUTINE tt_block_batch_free(h) BIND(C, NAME="tt_block_batch_free")
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: h
+    END SUBROUTINE
+
     ! host preprocessing: ReadMeshGid3D + GenMakroMesh, ReadBic, ReadInputFile
     ! (strings NUL-terminated: TRIM(s)//C_NULL_CHAR)
     FUNCTION tt_host_read_geo(path, h) BIND(C, NAME="tt_host_read_geo")
