@@ -78,8 +78,19 @@ __global__ void __launch_bounds__(kRowsPerBlock) prod_h_kernel(int nb, int nn, i
   }
 }
 
+// one column's three products into the boundary (D = 0) or domain (D = 1)
+// slots q*6 + c*2 + D
+template <int D>
+__device__ __forceinline__ void prod_d_add(double (&acc)[18], const double (&m)[3], const double (&w)[3]) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[q * 6 + c * 2 + D] += m[q] * w[c];
+}
+
 // [Dx Dy Dz](i, k) * vort(k, c), k split into boundary (0) / domain (1):
-// 18 sums per row, output index m*6 + c*2 + dom
+// 18 sums per row, output index m*6 + c*2 + dom; four columns' loads issued
+// together, added in column order
 __global__ void __launch_bounds__(kRowsPerBlock) prod_d_kernel(int nb, int nn, int ks_cnt, const double *__restrict__ Dx,
                                                                const double *__restrict__ Dy,
                                                                const double *__restrict__ Dz,
@@ -91,26 +102,33 @@ __global__ void __launch_bounds__(kRowsPerBlock) prod_d_kernel(int nb, int nn, i
 #pragma unroll
   for (int o = 0; o < 18; ++o) acc[o] = 0.0;
   if (i < nb) {
-    for (int k = k0; k < k1; ++k) {
-      const double w0 = vort[k], w1 = vort[k + nn], w2 = vort[k + 2 * (size_t)nn];
+    int k = k0;
+    for (; k + 4 <= k1; k += 4) {
+      double m[4][3], w[4][3];
+      int bnd[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const size_t a = i + (size_t)nb * (k + u);
+        m[u][0] = Dx[a], m[u][1] = Dy[a], m[u][2] = Dz[a];
+        w[u][0] = vort[k + u], w[u][1] = vort[k + u + nn], w[u][2] = vort[k + u + 2 * (size_t)nn];
+        bnd[u] = isbnd[k + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (bnd[u])  // uniform across the wavefront
+          prod_d_add<0>(acc, m[u], w[u]);
+        else
+          prod_d_add<1>(acc, m[u], w[u]);
+      }
+    }
+    for (; k < k1; ++k) {
       const size_t a = i + (size_t)nb * k;
       const double m[3] = {Dx[a], Dy[a], Dz[a]};
-      const int dom = isbnd[k] ? 0 : 1;
-      if (dom == 0) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          acc[q * 6 + 0] += m[q] * w0;
-          acc[q * 6 + 2] += m[q] * w1;
-          acc[q * 6 + 4] += m[q] * w2;
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          acc[q * 6 + 1] += m[q] * w0;
-          acc[q * 6 + 3] += m[q] * w1;
-          acc[q * 6 + 5] += m[q] * w2;
-        }
-      }
+      const double w[3] = {vort[k], vort[k + nn], vort[k + 2 * (size_t)nn]};
+      if (isbnd[k])
+        prod_d_add<0>(acc, m, w);
+      else
+        prod_d_add<1>(acc, m, w);
     }
 #pragma unroll
     for (int o = 0; o < 18; ++o) part[((size_t)ks * 18 + o) * nb + i] = acc[o];
@@ -205,9 +223,31 @@ __global__ void __launch_bounds__(kRowsPerBlock) dcorr_kernel(int nb, int ks_cnt
   const int chunk = (nb + ks_cnt - 1) / ks_cnt, j0 = ks * chunk, j1 = min(nb, j0 + chunk);
   if (i >= nb) return;
   double a0 = 0.0, a1 = 0.0;
-  for (int j = j0; j < j1; ++j) {
+  int j = j0;
+  // columns with a change (uniform across the wavefront), four columns'
+  // loads issued together, added in column order
+  for (; j + 4 <= j1; j += 4) {
+    double d[4], m0[4], m1[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      d[u] = dw[j + u];
+      m0[u] = m1[u] = 0.0;
+      if (d[u] != 0.0) {
+        const size_t col = (size_t)(gbn[j + u] - 1) * nb;
+        m0[u] = D0[i + col];
+        if (nm > 1) m1[u] = D1[i + col];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (d[u] != 0.0) {
+        a0 += m0[u] * d[u];
+        if (nm > 1) a1 += m1[u] * d[u];
+      }
+  }
+  for (; j < j1; ++j) {
     const double d = dw[j];
-    if (d == 0.0) continue;  // uniform across the wavefront
+    if (d == 0.0) continue;
     const size_t col = (size_t)(gbn[j] - 1) * nb;
     a0 += D0[i + col] * d;
     if (nm > 1) a1 += D1[i + col] * d;
@@ -550,13 +590,18 @@ int lo_split(int nb, int ncols) {
   return std::max(1, std::min(ks, std::max(1, ncols / 64)));
 }
 
-int split_for(int nb, int ncols) {
+int split_for(int nb, int ncols, int waves = 2048) {
   // enough wavefronts to keep HBM busy (~8 per CU on 256 CUs)
   int rows = (nb + kRowsPerBlock - 1) / kRowsPerBlock;
-  int ks = (2048 + rows - 1) / rows;
+  int ks = (waves + rows - 1) / rows;
   ks = std::max(1, std::min(ks, std::max(1, ncols / 64)));
   return ks;
 }
+// prod_d_kernel: about 6 MB of loads in flight over the chip (1,024
+// wavefronts of four columns x three matrices) streams fastest; twice that
+// is ~15% slower (9,602 x 68,921: 5.8 against 4.9 TB/s; 13,826 x 117,649:
+// 5.7-5.9 against 5.2; 1,538 x 4,913: 5.5 against 5.0)
+int split_d(int nb, int nn) { return split_for(nb, nn, 1024); }
 
 }  // namespace
 
@@ -608,7 +653,7 @@ void dense_setup(DenseKin &d, const DevMesh &m, const int *kode_host, SdPrec pre
   rocblas_destroy_handle(h);
   d.YH.alloc((size_t)12 * nb);
   d.YD.alloc((size_t)18 * nb);
-  int ksd = split_for(nb, nn), ksh = split_for(nb, nb);
+  int ksd = split_d(nb, nn), ksh = split_for(nb, nb);
   d.part.alloc((size_t)std::max(ksd * 18, ksh * 12) * nb);
   d.dfi.alloc(nb), d.xs.alloc(nb), d.dw.alloc(nb);
 }
@@ -621,7 +666,7 @@ void step_fp64(DenseKin &d, const DevMesh &m, const double *vel, double *vort, c
                bool save) {
   const int nb = d.nb, nn = d.nn;
   const unsigned rb = ceil_div(nb, kRowsPerBlock);
-  const int ksh = split_for(nb, nb), ksd = split_for(nb, nn);
+  const int ksh = split_for(nb, nb), ksd = split_d(nb, nn);
   const SdDev &M = d.mats;
   if (save) {
     hipLaunchKernelGGL(copy_kernel, dim3(ceil_div((size_t)nn * 3, 256)), dim3(256), 0, s, (size_t)nn * 3, vel,
@@ -786,7 +831,7 @@ void dense_set_precision(DenseKin &d, SdPrec prec, hipStream_t s) {
   d.dfi_a.alloc((size_t)3 * nb), d.sol_a.alloc((size_t)3 * nb);
   d.vel_a.alloc((size_t)3 * nn), d.vort_a.alloc((size_t)3 * nn);
   d.dv4.alloc(nb), d.dw4.alloc(nn);
-  const int need = std::max({lo_split(nb, nn) * 18, lo_split(nb, nb) * 12, split_for(nb, nn) * 18,
+  const int need = std::max({lo_split(nb, nn) * 18, lo_split(nb, nb) * 12, split_d(nb, nn) * 18,
                              split_for(nb, nb) * 12});
   if ((size_t)need * nb > d.part.n) d.part.alloc((size_t)need * nb);
 }
