@@ -194,6 +194,21 @@ int tt_block_batch_solve(tt_block_batch *h, const double *b, double *x, int32_t 
                          int32_t *istop, double *ms);
 void tt_block_batch_free(tt_block_batch *h);
 
+/* Shared blocks in batched solves (no reference counterpart; a tuning and
+ * test hook): when systems of one batched solve (tt_block_batch_solve,
+ * tt_case_picard_step_batch, the batched server) have FP32 or FP16 blocks,
+ * column maps and gather tables that are equal bit for bit -- the subdomain
+ * systems of cases on one mesh whose matrices differ only below the blocks'
+ * precision, compared on the device once per pair -- they read one copy of
+ * the blocks, and a cell-kernel launch whose systems all do interleaves their
+ * workgroups so that each block is fetched once into an XCD's L2 for all of
+ * them.  Each system's arithmetic is unchanged, so are its results.  on = 0
+ * turns this off for the process (default on).  The statistics count the
+ * interleaved launches since the library was loaded and the systems they
+ * carried. */
+int tt_lsqr_set_block_sharing(int32_t on);
+int tt_lsqr_block_sharing_stats(int64_t *launches, int64_t *systems);
+
 /* Replaces CRSxV (lib_util.f:1351-1372): y = A x on the device for a CSR
  * matrix given by host arrays (m rows, n columns). */
 int tt_csr_mv(int32_t m, int32_t n, int32_t nnz, const int32_t *ia, const int32_t *ja,

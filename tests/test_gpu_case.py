@@ -208,6 +208,60 @@ def test_batched_steps_match_cases_alone(prec):
             c.close()
 
 
+def test_shared_block_batch_matches_cases_alone():
+    """Cases on one 6^3 mesh with different Reynolds and Rayleigh numbers have
+    equal FP16 least-squares blocks (their matrices differ only below FP16);
+    stepped together they read one copy of the blocks in interleaved
+    cell-kernel launches, and every case still gives, step for step, the
+    bits it gives alone -- as it does with sharing switched off."""
+    from tritok_amd import workloads
+    from tritok_amd.case import Case, step_batch
+    from tritok_amd.solvers import block_sharing_stats, set_block_sharing
+    hosts = [workloads.cavity(6, re=50.0), workloads.cavity(6, re=350.0),
+             workloads.natural_convection(6, ra=1e4), workloads.natural_convection(6, ra=2e5),
+             workloads.cavity(6, re=120.0), workloads.natural_convection(4, ra=1e4)]
+    nstep = 5
+    alone = []
+    for i, h in enumerate(hosts):
+        print("alone", i, flush=True)
+        with Case(h, prec="fp16") as c:
+            steps = [c.step() for _ in range(nstep)]
+            alone.append(([(rc, e.copy(), n.copy()) for rc, e, n in steps], c.fields()))
+
+    def batched(share):
+        set_block_sharing(share)
+        l0, s0 = block_sharing_stats()
+        cases = [Case(h, prec="fp16") for h in hosts]
+        try:
+            got = [[] for _ in hosts]
+            for _ in range(nstep):
+                st, err, nit = step_batch(cases)
+                for i in range(len(hosts)):
+                    assert st[i] == 1, (i, st[i])
+                    got[i].append((int(st[i]), err[i].copy(), nit[i].copy()))
+            fields = [c.fields() for c in cases]
+        finally:
+            for c in cases:
+                c.close()
+            set_block_sharing(True)
+        l1, s1 = block_sharing_stats()
+        return got, fields, l1 - l0, s1 - s0
+
+    for share in (True, False):
+        print("batch, sharing", share, flush=True)
+        got, fields, launches, systems = batched(share)
+        print("interleaved launches", launches, "systems", systems, flush=True)
+        if share:
+            assert launches > 0 and systems >= 2 * launches, (launches, systems)
+        else:
+            assert launches == 0
+        for i, (steps, f) in enumerate(alone):
+            for (ra, ea, na), (rb, eb, nb) in zip(steps, got[i]):
+                assert ra == rb and np.array_equal(ea, eb) and np.array_equal(na, nb), (share, i)
+            for k in FIELDS:
+                assert np.array_equal(f[k], fields[i][k]), (share, i, k)
+
+
 def test_case_is_bitwise_repeatable(gauss_table):
     g = load_golden("nc2")
     outs = []

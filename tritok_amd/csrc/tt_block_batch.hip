@@ -176,3 +176,21 @@ extern "C" int tt_block_batch_solve(tt_block_batch *h, const double *b, double *
 }
 
 extern "C" void tt_block_batch_free(tt_block_batch *h) { delete h; }
+
+extern "C" int tt_lsqr_set_block_sharing(int32_t on) {
+  return guarded("tt_lsqr_set_block_sharing", [&] {
+    if (on < 0 || on > 1) throw std::runtime_error("on must be 0 or 1");
+    block_sharing(on);
+    return 0;
+  });
+}
+
+extern "C" int tt_lsqr_block_sharing_stats(int64_t *launches, int64_t *systems) {
+  return guarded("tt_lsqr_block_sharing_stats", [&] {
+    if (!launches || !systems) throw std::runtime_error("NULL argument");
+    long long l = 0, n = 0;
+    block_sharing_stats(&l, &n);
+    *launches = l, *systems = n;
+    return 0;
+  });
+}

@@ -9,6 +9,7 @@
 //  * CsrSys: a general CSR matrix (the C-ABI entry points), with a transposed
 //    copy for A^T*u.
 #pragma once
+#include <utility>
 #include <vector>
 
 #include "tt_hip.h"
@@ -89,6 +90,12 @@ struct BlockSys {
   };
   SolveHistory hist[3];
   LsqrGraphs graphs[3];   // captured iteration chunks per block precision
+  // identity for batched solves: systems whose reduced-precision blocks,
+  // column maps and gather tables are equal bit for bit share one block
+  // stream per launch (block_lsqr_multi); comparisons made so far, per block
+  // precision: (other system's uid, equal)
+  unsigned long long uid = 0;
+  std::vector<std::pair<unsigned long long, bool>> twins[3];
 };
 
 // Builds the solver structure from the full unscaled blocks on the device
@@ -142,6 +149,7 @@ LsqrResult block_lsqr(BlockSys &s, LsqrWork &wk, const double *b, double *x, int
 // arguments; res is filled in.
 struct LsqrJob {
   BlockSys *s = nullptr;
+  const BlockSys *share_sys = nullptr;  // set by block_lsqr_multi: an earlier system with the same blocks
   LsqrWork *wk = nullptr;  // a work area of its own
   const double *b = nullptr;
   double *x = nullptr;
@@ -157,7 +165,16 @@ struct LsqrJob {
 // kernel bodies, grids per system and reductions are those of block_lsqr, so
 // each system's iterates, stop and result are bit for bit those of a solve on
 // its own.
+// Systems whose FP32 or FP16 blocks, column maps and gather tables compare
+// equal bit for bit read one copy of the blocks; when every system of a
+// cell-kernel launch does, their workgroups are interleaved so that the
+// systems' workgroups on the same cells run together on one XCD (each block
+// fetched once into that L2).  Per-system arithmetic is unchanged.  on = 0
+// turns this off (process-wide); the counters report the interleaved
+// launches and the systems they carried.
 void block_lsqr_multi(std::vector<LsqrJob> &jobs, int prec, hipStream_t st);
+void block_sharing(int on);
+void block_sharing_stats(long long *launches, long long *systems);
 
 // Picks the cell kernel's grid for block precision prec by timing candidate
 // grids on a scratch solve (once per system shape and precision in a process).

@@ -224,6 +224,11 @@ struct Jobs {
   Args a[kMaxJobs];
   int first[kMaxJobs + 1];
   int n;
+  // il > 0 (cell kernels of systems that read one shared block array, each
+  // with gs workgroups): workgroups interleaved so that the systems'
+  // workgroups of one logical index l are dispatched together on one XCD
+  // (l % 8): b = 8*il*(l/8) + 8*i + l%8 for system i
+  int il = 0, gs = 0;
 };
 // the system of workgroup b (uniform over the workgroup)
 template <typename Args>
@@ -710,9 +715,25 @@ __global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw(CellArgs a) {
 }
 template <typename T, int R, int L, int NW>
 __global__ void __launch_bounds__(64 * NW, 4) lsqr_Aw_multi(Jobs<CellArgs> J) {
+  if (J.il > 0) {
+    const int b = blockIdx.x, i = (b / kXcd) % J.il, l = kXcd * (b / (kXcd * J.il)) + b % kXcd;
+    if (l >= J.gs) return;
+    cell_Aw<T, R, L, NW>(TT_CELL_ARGS(T, J.a[i]), l, J.gs);
+    return;
+  }
   const int i = job_of(J, blockIdx.x);
   cell_Aw<T, R, L, NW>(TT_CELL_ARGS(T, J.a[i]), blockIdx.x - J.first[i], J.first[i + 1] - J.first[i]);
 }
+
+// byte-wise comparison of two device arrays of 32-bit words: *flag = 1 where any word differs
+__global__ void differ_kernel(size_t nw, const uint32_t *__restrict__ x, const uint32_t *__restrict__ y,
+                              int *__restrict__ flag) {
+  int d = 0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += (size_t)gridDim.x * blockDim.x)
+    d |= x[i] != y[i];
+  if (d) *flag = 1;
+}
+__global__ void set_int_kernel(int *p, int v) { *p = v; }
 
 // ---------------------------------------------------------------------------
 // iteration k, unknown part (k = 0: initialisation after r0)
@@ -1016,6 +1037,9 @@ void stepB(BlockSys &s, LsqrWork &wk, int k, int prec, hipStream_t st) {
   }
 }
 
+std::atomic<int> g_sharing{1};
+std::atomic<long long> g_shared_launches{0}, g_shared_systems{0};
+
 // ---- several systems per launch (block_lsqr_multi): the same kernel bodies
 // on workgroup ranges, one range per system
 template <typename T, int R, int L>
@@ -1026,17 +1050,52 @@ void launch_A_multi(const std::vector<LsqrJob *> &act, int k, int prec, hipStrea
   J.first[0] = 0;
   for (int i = 0; i < J.n; ++i) {
     BlockSys &s = *act[i]->s;
-    const T *A = prec == kBlocks16   ? (const T *)s.A16.p
-                 : prec == kBlocks32 ? (const T *)s.A32.p
+    const BlockSys &sa = act[i]->share_sys ? *act[i]->share_sys : s;  // equal blocks: read one copy
+    const T *A = prec == kBlocks16   ? (const T *)sa.A16.p
+                 : prec == kBlocks32 ? (const T *)sa.A32.p
                                      : (const T *)s.A.p;
     int grid = 0;
     J.a[i] = cell_args<T>(s, *act[i]->wk, k, A, grid);
     J.first[i + 1] = J.first[i] + grid;
   }
-  if constexpr (NW == 1)
+  if constexpr (NW == 1) {
     hipLaunchKernelGGL((lsqr_A_multi<T, R, L>), dim3(J.first[J.n]), dim3(64), 0, st, J);
-  else
-    hipLaunchKernelGGL((lsqr_Aw_multi<T, R, L, NW>), dim3(J.first[J.n]), dim3(64 * NW), 0, st, J);
+  } else {
+    // systems reading the same blocks: one launch per such class with their
+    // workgroups interleaved at the class's first system's grid, so that each
+    // block is fetched once into an XCD's L2 for all of them; the other
+    // systems in one launch of workgroup ranges
+    std::vector<std::vector<int>> cls;
+    for (int i = 0; i < J.n; ++i) {
+      size_t c = 0;
+      while (c < cls.size() && J.a[cls[c][0]].A != J.a[i].A) ++c;
+      if (c == cls.size()) cls.emplace_back();
+      cls[c].push_back(i);
+    }
+    Jobs<CellArgs> rest;
+    rest.n = 0;
+    rest.first[0] = 0;
+    for (const auto &c : cls) {
+      if (c.size() == 1) {
+        const int i = c[0];
+        rest.a[rest.n] = J.a[i];
+        rest.first[rest.n + 1] = rest.first[rest.n] + (J.first[i + 1] - J.first[i]);
+        ++rest.n;
+        continue;
+      }
+      Jobs<CellArgs> I;
+      I.n = (int)c.size();
+      for (int q = 0; q < I.n; ++q) I.a[q] = J.a[c[q]];
+      I.il = I.n;
+      I.gs = J.first[c[0] + 1] - J.first[c[0]];  // results do not depend on the grid
+      const int gp = (I.gs + kXcd - 1) / kXcd * kXcd;
+      hipLaunchKernelGGL((lsqr_Aw_multi<T, R, L, NW>), dim3(gp * I.n), dim3(64 * NW), 0, st, I);
+      g_shared_launches += 1;
+      g_shared_systems += I.n;
+    }
+    if (rest.n > 0)
+      hipLaunchKernelGGL((lsqr_Aw_multi<T, R, L, NW>), dim3(rest.first[rest.n]), dim3(64 * NW), 0, st, rest);
+  }
 }
 template <int R, int L>
 void stepA_multi_rl(const std::vector<LsqrJob *> &act, int k, int prec, hipStream_t st) {
@@ -1111,6 +1170,43 @@ void probe_collect(LsqrProbe *p, int nit) {
 }
 
 std::atomic<unsigned> g_work_gen{0};
+std::atomic<unsigned long long> g_sys_uid{0};
+
+// whether two systems' blocks of precision prec (FP32 or FP16), column maps
+// and gather tables are equal bit for bit; compared on the device once per
+// pair of systems
+bool lo_blocks_equal(BlockSys &x, BlockSys &y, int prec, hipStream_t st) {
+  if (x.uid == y.uid) return true;
+  for (const auto &e : x.twins[prec])
+    if (e.first == y.uid) return e.second;
+  bool eq = x.ncell == y.ncell && x.R == y.R && x.L == y.L && x.n == y.n && x.gw == y.gw &&
+            (prec == kBlocks16 ? x.has16 && y.has16 : x.has32 && y.has32);
+  if (eq) {
+    DBuf<int> flag;
+    flag.alloc(1);
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, flag.p, 0);
+    auto cmp = [&](const void *p, const void *q, size_t bytes) {
+      const size_t nw = bytes / 4;
+      const unsigned blocks = (unsigned)std::min<size_t>(2048, std::max<size_t>(1, ceil_div(nw, 256)));
+      hipLaunchKernelGGL(differ_kernel, dim3(blocks), dim3(256), 0, st, nw, static_cast<const uint32_t *>(p),
+                         static_cast<const uint32_t *>(q), flag.p);
+    };
+    if (prec == kBlocks16)
+      cmp(x.A16.p, y.A16.p, (size_t)x.ncell * block_stride16(x.R, x.L) * sizeof(_Float16));
+    else
+      cmp(x.A32.p, y.A32.p, (size_t)x.ncell * x.stride * sizeof(float));
+    cmp(x.colmap.p, y.colmap.p, (size_t)x.ncell * x.L * sizeof(int));
+    cmp(x.gtab.p, y.gtab.p, (size_t)x.n * x.gw * sizeof(int));
+    TT_LAUNCH_CHECK();
+    int h = 1;
+    TT_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    TT_HIP(hipStreamSynchronize(st));
+    eq = h == 0;
+  }
+  x.twins[prec].push_back({y.uid, eq});
+  y.twins[prec].push_back({x.uid, eq});
+  return eq;
+}
 
 // records 2^(lvl+1) iterations starting at an odd index (the state copies
 // alternate with the parity of the index, so such a chunk replays at any odd
@@ -1162,6 +1258,8 @@ void block_setup(BlockSys &s, int ncell, int R, int L, int n, const std::vector<
   if (!((R == 51 && L == 51) || (R == 27 && L == 27)))
     throw std::runtime_error("block LSQR supports 51x51 and 27x27 cell blocks");
   s.ncell = ncell, s.R = R, s.L = L, s.n = n, s.stride = block_stride(R, L);
+  s.uid = ++g_sys_uid;
+  for (auto &t : s.twins) t.clear();
   // transpose gather lists in cell order
   std::vector<int> cnt(n + 1, 0), idx;
   for (size_t k = 0; k < colmap.size(); ++k)
@@ -1512,6 +1610,16 @@ void block_lsqr_multi(std::vector<LsqrJob> &jobs, int prec, hipStream_t st) {
     const int p = s.hist[j.slot].predict();
     pred = (p < 0 || pred < 0) ? -1 : std::max(pred, p);
   }
+  // systems with bit-identical reduced-precision blocks share their cell-kernel launches
+  for (size_t i = 0; i < jobs.size(); ++i) {
+    jobs[i].share_sys = nullptr;
+    if (prec == kBlocks64 || !g_sharing.load()) continue;
+    for (size_t q = 0; q < i; ++q)
+      if (!jobs[q].share_sys && lo_blocks_equal(*jobs[i].s, *jobs[q].s, prec, st)) {
+        jobs[i].share_sys = jobs[q].s;
+        break;
+      }
+  }
   std::vector<LsqrJob *> act;
   for (LsqrJob &j : jobs) act.push_back(&j);
   // starting states (as block_lsqr's), initial residuals, iteration 0's
@@ -1580,6 +1688,12 @@ void block_lsqr_multi(std::vector<LsqrJob> &jobs, int prec, hipStream_t st) {
     batch = pred >= 0 ? std::min(64, more << std::min(extra, 4)) : std::max(more, std::min(64, k / 4));
     ++extra;
   }
+}
+
+void block_sharing(int on) { g_sharing = on ? 1 : 0; }
+void block_sharing_stats(long long *launches, long long *systems) {
+  if (launches) *launches = g_shared_launches.load();
+  if (systems) *systems = g_shared_systems.load();
 }
 
 }  // namespace tt

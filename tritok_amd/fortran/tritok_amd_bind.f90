@@ -393,6 +393,18 @@ MODULE tritok_amd_bind
       TYPE(C_PTR), VALUE :: h
     END SUBROUTINE
 
+    ! batched solves: systems with bit-identical reduced-precision blocks read one copy
+    FUNCTION tt_lsqr_set_block_sharing(on) BIND(C, NAME="tt_lsqr_set_block_sharing")
+      IMPORT :: C_INT, C_INT32_T
+      INTEGER(C_INT32_T), VALUE :: on
+      INTEGER(C_INT) :: tt_lsqr_set_block_sharing
+    END FUNCTION
+    FUNCTION tt_lsqr_block_sharing_stats(launches, systems) BIND(C, NAME="tt_lsqr_block_sharing_stats")
+      IMPORT :: C_INT, C_INT64_T
+      INTEGER(C_INT64_T) :: launches, systems
+      INTEGER(C_INT) :: tt_lsqr_block_sharing_stats
+    END FUNCTION
+
     ! host preprocessing: ReadMeshGid3D + GenMakroMesh, ReadBic, ReadInputFile
     ! (strings NUL-terminated: TRIM(s)//C_NULL_CHAR)
     FUNCTION tt_host_read_geo(path, h) BIND(C, NAME="tt_host_read_geo")

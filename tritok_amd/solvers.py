@@ -101,6 +101,23 @@ _kbsetup = _lib._sig("tt_block_batch_setup", i32, i32, PI, PI, C.POINTER(C.c_int
 _kbprec = _lib._sig("tt_block_batch_set_precision", i32, P, i32)
 _kbsolve = _lib._sig("tt_block_batch_solve", i32, P, PD, PD, i32, C.c_float, PI, PI, C.POINTER(C.c_double))
 _kbfree = _lib._sig("tt_block_batch_free", None, P)
+_share = _lib._sig("tt_lsqr_set_block_sharing", i32, i32)
+_share_stats = _lib._sig("tt_lsqr_block_sharing_stats", i32, C.POINTER(C.c_int64), C.POINTER(C.c_int64))
+
+
+def set_block_sharing(on):
+    """Batched solves whose systems have bit-identical FP32/FP16 blocks read
+    one copy of the blocks, their cell-kernel workgroups interleaved
+    (default on; process-wide)."""
+    check(_share(1 if on else 0), "tt_lsqr_set_block_sharing")
+
+
+def block_sharing_stats():
+    """(interleaved shared-block launches, systems they carried) since the
+    library loaded."""
+    a, b = C.c_int64(), C.c_int64()
+    check(_share_stats(C.byref(a), C.byref(b)), "tt_lsqr_block_sharing_stats")
+    return a.value, b.value
 
 
 class BlockBatch:
