@@ -68,7 +68,7 @@ def _csr_pattern(view, kind, e=0):
     return ia, ja, nc * rows_per_cell, n
 
 
-LSQR_ITS = (20, 620)  # two runs of the sample solve; their difference is timed per iteration
+LSQR_ITS = (20, 1220)  # two runs of the sample solve; their difference is timed per iteration
 
 
 def _lsqr_iter_seconds(ia, ja, m, n, its=LSQR_ITS):
@@ -96,7 +96,7 @@ def picard_rate(host, case, nits, budget_s=20.0, log=print):
     cores = _cores()
     t_start = time.perf_counter()
     # ---- dense right-hand side: a row sample of the seven matrices
-    rows = max(64, min(nb, int(1.2e9 // (4 * nb + 3 * nn))))  # about 10 GB of FP64 operands
+    rows = max(64, min(nb, int(2.4e9 // (4 * nb + 3 * nn))))  # up to about 19 GB of FP64 operands (all rows at 20^3)
     H = [np.full((rows, nb), 1e-3, order="F") for _ in range(4)]
     D = [np.full((rows, nn), 1e-3, order="F") for _ in range(3)]
     lbn, gbn = _arr(m.lbn, nn), _arr(m.gbn, nb)
