@@ -478,9 +478,12 @@ __global__ void mhsd_relax_kernel(int nn, int nf, const int *__restrict__ kode, 
   if (i < nf && qkode[i] < 0) qu[i] = (1.0 - ur) * qu[i] + ur * x[-qkode[i] - 1];
 }
 
+// column slices for about 1,024 wavefronts over the chip: with eight columns'
+// loads in flight per wavefront that is the ~4 MB in flight these streams run
+// fastest at (8^3 solve 0.35 ms at 1,024 wavefronts, 0.48 ms at 8,192)
 int split_cols(int nr, int ncols) {
   const int rows = (nr + kRows - 1) / kRows;
-  int ks = (8192 + rows - 1) / rows;
+  int ks = (1024 + rows - 1) / rows;
   return std::max(1, std::min(ks, std::max(1, ncols / 64)));
 }
 
