@@ -319,15 +319,23 @@ def mixed_kinematics_leg(n, tol=1e-10):
     x, y, z = p[:, 0], p[:, 1], p[:, 2]
     B = np.stack([np.sin(np.pi * y) * np.sin(np.pi * z), np.sin(np.pi * z) * np.sin(np.pi * x),
                   np.sin(np.pi * x) * np.sin(np.pi * y), x - 0.5], axis=1)
-    X64, it64, rr64, ms64, mv64 = sd.refine(1, "fp64", B, tol=1e-14, maxit=4)
+    # each solve timed three times, the median kept (single runs of a
+    # millisecond-long solve pick up the odd stall of the box)
+    def timed(prec, **kw):
+        runs = [sd.refine(1, prec, B, **kw) for _ in range(3)]
+        order = sorted(range(3), key=lambda i: runs[i][3])
+        return runs[order[1]]
+
+    sd.refine(1, "fp64", B, tol=1e-14, maxit=4)
+    X64, it64, rr64, ms64, mv64 = timed("fp64", tol=1e-14, maxit=4)
     out = {"workload": "kinematics x equation of the %d^3 heated cube (%d x %d), right-hand sides u_x, u_y, u_z, T "
                        "(smooth fields at the boundary nodes), FP64 iterative refinement to %g" % (n, nb, nb, tol),
-           "nbnodes": nb, "setup_s": setup,
+           "nbnodes": nb, "setup_s": setup, "timing": "median of three solves after a first untimed one",
            "fp64": {"corrections": it64, "relres": rr64, "ms": ms64, "ms_product": mv64}}
     bytes_per = {"fp32": 4, "bf16": 2, "fp16": 2}
     for prec in ("fp32", "bf16", "fp16"):
         sd.refine(1, prec, B, tol=tol, maxit=60)  # first use builds the tile copy
-        X, it, rr, ms, mv = sd.refine(1, prec, B, tol=tol, maxit=60)
+        X, it, rr, ms, mv = timed(prec, tol=tol, maxit=60)
         err = float(np.linalg.norm(X - X64) / np.linalg.norm(X64))
         gbs = nb * nb * bytes_per[prec] / (mv * 1e-3) / 1e9
         tflops = 2.0 * nb * nb * 16 / (mv * 1e-3) / 1e12  # the instructions' 16 columns
