@@ -68,7 +68,7 @@ def _csr_pattern(view, kind, e=0):
     return ia, ja, nc * rows_per_cell, n
 
 
-LSQR_ITS = (20, 3020)  # two runs of the sample solve; their difference is timed per iteration
+LSQR_ITS = (20, 6020)  # two runs of the sample solve; their difference is timed per iteration
 
 
 def _lsqr_iter_seconds(ia, ja, m, n, its=LSQR_ITS):
