@@ -584,16 +584,11 @@ __global__ void copy_kernel(size_t n, const double *__restrict__ a, double *__re
   if (k < n) b[k] = a[k];
 }
 
-int lo_split(int nb, int ncols, int waves = 2048) {
+int lo_split(int nb, int ncols) {
   int rows = (nb + kLoBlockRows - 1) / kLoBlockRows;
-  int ks = (waves + rows - 1) / rows;
+  int ks = (2048 + rows - 1) / rows;
   return std::max(1, std::min(ks, std::max(1, ncols / 64)));
 }
-// the one- and two-matrix products (inverse, D corrections) stream fastest
-// with twice the wavefronts of the four-matrix H product (9,602^2 FP16:
-// lgemv 36.6 against 40.4 us, ldcorr 62.7 against 76.4 us; lprod_h 208.9
-// against 194.5 us)
-int lo_split_small(int nb) { return lo_split(nb, nb, 4096); }
 
 int split_for(int nb, int ncols, int waves = 2048) {
   // enough wavefronts to keep HBM busy (~8 per CU on 256 CUs)
@@ -737,7 +732,7 @@ void step_delta(DenseKin &d, const DevMesh &m, const double *vel, double *vort, 
   };
   const int nb = d.nb, nn = d.nn, ldl = d.ldl;
   const unsigned rb = ceil_div(nb, kLoBlockRows);
-  const int ksh = lo_split(nb, nb), ksd = lo_split(nb, nn), kss = lo_split_small(nb);
+  const int ksh = lo_split(nb, nb), ksd = lo_split(nb, nn);
   const float *sc = d.rscale.p;
   hipLaunchKernelGGL(delta4_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, m.gbn.p, vel, d.vel_a.p,
                      (const int *)nullptr, d.dv4.p);
@@ -756,20 +751,20 @@ void step_delta(DenseKin &d, const DevMesh &m, const double *vel, double *vort, 
   for (int e = 0; e < 3; ++e) {
     hipLaunchKernelGGL(combine_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, e, d.YH.p, d.YD.p, m.nx.p,
                        m.ny.p, m.nz.p, d.kode.p, m.gbn.p, vort, d.dfi.p);
-    hipLaunchKernelGGL((lgemv_kernel<L>), dim3(rb, kss), dim3(64), 0, s, nb, ldl, kss, lo(7 + e),
+    hipLaunchKernelGGL((lgemv_kernel<L>), dim3(rb, ksh), dim3(64), 0, s, nb, ldl, ksh, lo(7 + e),
                        sc + (size_t)(7 + e) * ldl, (const double *)d.dfi.p, (const double *)(d.dfi_a.p + (size_t)nb * e),
                        d.part.p);
-    hipLaunchKernelGGL(reduce_base_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, 1, kss, d.part.p,
+    hipLaunchKernelGGL(reduce_base_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, 1, ksh, d.part.p,
                        d.sol_a.p + (size_t)nb * e, d.xs.p);
     hipLaunchKernelGGL(relax_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nn, e, urBw[e], d.xs.p, d.kode.p,
                        m.gbn.p, vort, d.dw.p);
     TT_LAUNCH_CHECK();
     if (e < 2) {
       const int nm = 2 - e;
-      hipLaunchKernelGGL((ldcorr_kernel<L>), dim3(rb, kss), dim3(64), 0, s, nb, nn, kss, nm, lo(4 + e + 1),
+      hipLaunchKernelGGL((ldcorr_kernel<L>), dim3(rb, ksh), dim3(64), 0, s, nb, nn, ksh, nm, lo(4 + e + 1),
                          nm > 1 ? lo(4 + e + 2) : lo(4 + e + 1), sc + (size_t)(4 + e + 1) * ldl,
                          sc + (size_t)(4 + (nm > 1 ? e + 2 : e + 1)) * ldl, m.gbn.p, (const double *)d.dw.p, d.part.p);
-      hipLaunchKernelGGL(add_slots_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nm, kss, d.part.p,
+      hipLaunchKernelGGL(add_slots_kernel, dim3(ceil_div(nb, 256)), dim3(256), 0, s, nb, nm, ksh, d.part.p,
                          (e + 1) * 6 + e * 2, (e + 2) * 6 + e * 2, d.YD.p);
       TT_LAUNCH_CHECK();
     }
@@ -836,8 +831,8 @@ void dense_set_precision(DenseKin &d, SdPrec prec, hipStream_t s) {
   d.dfi_a.alloc((size_t)3 * nb), d.sol_a.alloc((size_t)3 * nb);
   d.vel_a.alloc((size_t)3 * nn), d.vort_a.alloc((size_t)3 * nn);
   d.dv4.alloc(nb), d.dw4.alloc(nn);
-  const int need = std::max({lo_split(nb, nn) * 18, lo_split(nb, nb) * 12, lo_split_small(nb) * 2,
-                             split_d(nb, nn) * 18, split_for(nb, nb) * 12});
+  const int need = std::max({lo_split(nb, nn) * 18, lo_split(nb, nb) * 12, split_d(nb, nn) * 18,
+                             split_for(nb, nb) * 12});
   if ((size_t)need * nb > d.part.n) d.part.alloc((size_t)need * nb);
 }
 
