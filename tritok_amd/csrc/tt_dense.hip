@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "tt_dense.h"
+#include "tt_sums.h"
 
 namespace tt {
 
@@ -141,8 +142,7 @@ __global__ void reduce_parts_kernel(int nb, int no, int ks_cnt, const double *__
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)nb * no) return;
   int o = (int)(idx / nb), i = (int)(idx % nb);
-  double s = 0.0;
-  for (int k = 0; k < ks_cnt; ++k) s += part[((size_t)k * no + o) * nb + i];
+  const double s = slice_sum(part + (size_t)o * nb + i, (size_t)no * nb, ks_cnt);
   out[idx] = accumulate ? out[idx] + s : s;
 }
 
@@ -262,8 +262,7 @@ __global__ void add_slots_kernel(int nb, int nm, int ks_cnt, const double *__res
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   for (int t = 0; t < nm; ++t) {
-    double s = 0.0;
-    for (int k = 0; k < ks_cnt; ++k) s += part[((size_t)k * nm + t) * nb + i];
+    const double s = slice_sum(part + (size_t)t * nb + i, (size_t)nm * nb, ks_cnt);
     int slot = t == 0 ? s0 : s1;
     YD[(size_t)slot * nb + i] += s;
   }
@@ -574,8 +573,7 @@ __global__ void reduce_base_kernel(int nb, int no, int ks_cnt, const double *__r
   size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)nb * no) return;
   int o = (int)(idx / nb), i = (int)(idx % nb);
-  double s = 0.0;
-  for (int k = 0; k < ks_cnt; ++k) s += part[((size_t)k * no + o) * nb + i];
+  const double s = slice_sum(part + (size_t)o * nb + i, (size_t)no * nb, ks_cnt);
   out[idx] = base[idx] + s;
 }
 
@@ -1001,8 +999,7 @@ __global__ void mx_update_kernel(int n, int nrhs, int ks_cnt, int npad, const fl
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * nrhs) return;
   const int i = (int)(t % n), c = (int)(t / n);
-  double s = 0.0;
-  for (int k = 0; k < ks_cnt; ++k) s += (double)part[((size_t)k * nrhs + c) * npad + i];
+  const double s = slice_sum(part + (size_t)c * npad + i, (size_t)nrhs * npad, ks_cnt);
   X[t] += s * (double)rscale[i] * cscale[c];
 }
 
@@ -1041,8 +1038,7 @@ __global__ void combine4_kernel(int n, int nrhs, int ks_cnt, const double *__res
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (size_t)n * nrhs) return;
   const int i = (int)(t % n), c = (int)(t / n);
-  double s = 0.0;
-  for (int k = 0; k < ks_cnt; ++k) s += part[((size_t)k * nrhs + c) * n + i];
+  const double s = slice_sum(part + (size_t)c * n + i, (size_t)nrhs * n, ks_cnt);
   out[t] = base[t] + sign * s;
 }
 

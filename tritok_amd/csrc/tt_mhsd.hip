@@ -41,6 +41,7 @@
 
 #include "tt_compress.h"
 #include "tt_integrals.h"
+#include "tt_sums.h"
 
 namespace tt {
 namespace {
@@ -452,8 +453,7 @@ __global__ void mhsd_reduce_kernel(int nr, int ks_cnt, const double *__restrict_
                                    const double *__restrict__ base, double *__restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nr) return;
-  double s = 0.0;
-  for (int k = 0; k < ks_cnt; ++k) s += part[(size_t)k * nr + i];
+  const double s = slice_sum(part + i, (size_t)nr, ks_cnt);
   out[i] = base ? base[i] + s : s;
 }
 
